@@ -1,0 +1,296 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes bindings to ``liboracle.so``, the plain-C restatement of the reference
+SNCH-LBVH algorithm (``oracle/snch_oracle.c``).  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module, and only as the checker; the library's product path never
+touches it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+U32 = np.uint32
+I32 = np.int32
+F32 = np.float32
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class OrcTree(ctypes.Structure):
+    _fields_ = [
+        ("n_obj", ctypes.c_uint32),
+        ("nodes", ctypes.c_void_p),
+        ("boxes", ctypes.c_void_p),
+        ("cones", ctypes.c_void_p),
+        ("V", ctypes.c_void_p),
+        ("F", ctypes.c_void_p),
+        ("slots", ctypes.c_void_p),
+        ("owned", ctypes.c_void_p),
+    ]
+
+
+def _setup(L):
+    vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+    L.orc_edge_table.argtypes = [vp, i32, vp, vp, vp]
+    L.orc_edge_table.restype = i32
+    L.orc_build.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
+    L.orc_build.restype = ctypes.c_int
+    L.orc_leaf_box_cone.argtypes = [vp, vp, i32, vp, vp, vp, vp, ctypes.c_int]
+    L.orc_morton3.argtypes = [f32, f32, f32]
+    L.orc_morton3.restype = ctypes.c_uint32
+    L.orc_expand_bits.argtypes = [ctypes.c_uint32]
+    L.orc_expand_bits.restype = ctypes.c_uint32
+    L.orc_cone_merge.argtypes = [vp] * 6 + [ctypes.c_int]
+    L.orc_cone_overlap.argtypes = [vp, vp, vp, f32]
+    L.orc_cone_overlap.restype = ctypes.c_int
+    L.orc_math.argtypes = [ctypes.c_int, vp, vp, vp, i32]
+    L.orc_closest_point.argtypes = [vp, vp, i64, vp, vp, vp, ctypes.c_int]
+    L.orc_closest_silhouette.argtypes = [vp, vp, i64, ctypes.c_int, vp, vp, vp, ctypes.c_int]
+    L.orc_ray.argtypes = [vp, vp, vp, vp, i64, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int]
+    L.orc_sample_in_sphere.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, ctypes.c_int]
+    L.orc_bf_closest_point.argtypes = [vp, vp, i32, vp, i64, vp, vp, ctypes.c_int]
+    L.orc_bf_closest_silhouette.argtypes = [vp, vp, i32, vp, i64, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.orc_bf_ray.argtypes = [vp, vp, i32, vp, vp, vp, i64, vp, vp, ctypes.c_int]
+    L.orc_tri_dist64.argtypes = [vp, vp, vp, vp, i64, vp]
+    L.orc_tri_dist32.argtypes = [vp, vp, vp, vp, i64, vp]
+    L.orc_edge_sil32.argtypes = [vp, vp, vp, vp, i64, ctypes.c_int, vp]
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def edge_table(F):
+    """Reference edge ids per (triangle, local edge), edge slots (opp0, lo, hi,
+    opp1) and per-triangle owned edges (scene.cuh:1135-1225)."""
+    F = _c(F, I32).reshape(-1, 3)
+    nF = F.shape[0]
+    tri_edge = np.empty((nF, 3), I32)
+    slots = np.empty((max(3 * nF, 1), 4), I32)
+    owned = np.empty((nF, 3), I32)
+    E = lib().orc_edge_table(_p(F), nF, _p(tri_edge), _p(slots), _p(owned))
+    return tri_edge, slots[:E].copy(), owned
+
+
+class Tree:
+    """A tree in the reference layout (bvh.cuh:27-33, 380-613)."""
+
+    def __init__(self, V, F, slots, owned, nodes, boxes, cones, sorted_codes=None, sorted_idx=None,
+                 collision=False, scene_box=None):
+        self.V, self.F, self.slots, self.owned = V, F, slots, owned
+        self.nodes, self.boxes, self.cones = nodes, boxes, cones
+        self.sorted_codes, self.sorted_idx = sorted_codes, sorted_idx
+        self.collision = collision
+        self.scene_box = scene_box
+        self.n_obj = F.shape[0]
+        self._s = OrcTree(self.n_obj, _p(nodes), _p(boxes), _p(cones), _p(V), _p(F), _p(slots), _p(owned))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self._s)
+
+
+CONSERVATIVE_RADIUS = 1
+
+
+def build_tree(V, F, edges=None, conservative_radius=True):
+    """Full reference construct() on the CPU.  conservative_radius selects the
+    enclosing cone radius (default) instead of the reference's midpoint radius."""
+    V = _c(V, F32).reshape(-1, 3)
+    F = _c(F, I32).reshape(-1, 3)
+    if edges is None:
+        edges = edge_table(F)
+    _, slots, owned = edges
+    slots = _c(slots, I32)
+    owned = _c(owned, I32)
+    N = F.shape[0]
+    NN = 2 * N - 1
+    nodes = np.empty((NN, 4), U32)
+    boxes = np.empty((NN, 6), F32)
+    cones = np.empty((NN, 5), F32)
+    codes = np.empty(N, U32)
+    sidx = np.empty(N, U32)
+    sbox = np.empty(6, F32)
+    col = lib().orc_build(_p(V), _p(F), N, _p(slots), _p(owned), _p(nodes), _p(boxes), _p(cones), _p(codes),
+                          _p(sidx), _p(sbox), CONSERVATIVE_RADIUS if conservative_radius else 0)
+    if col < 0:
+        raise ValueError("empty mesh")
+    return Tree(V, F, slots, owned, nodes, boxes, cones, codes, sidx, bool(col), sbox)
+
+
+def leaf_box_cone(V, F, edges=None, conservative_radius=True):
+    V = _c(V, F32).reshape(-1, 3)
+    F = _c(F, I32).reshape(-1, 3)
+    if edges is None:
+        edges = edge_table(F)
+    _, slots, owned = edges
+    n = F.shape[0]
+    boxes = np.empty((n, 6), F32)
+    cones = np.empty((n, 5), F32)
+    lib().orc_leaf_box_cone(_p(V), _p(F), n, _p(_c(slots, I32)), _p(_c(owned, I32)), _p(boxes), _p(cones),
+                            1 if conservative_radius else 0)
+    return boxes, cones
+
+
+def math(fn, x, y=None):
+    """fn: 0 acos, 1 asin, 2 atan2(x, y), 3 sin, 4 cos (deterministic FP32 set)."""
+    x = _c(x, F32).ravel()
+    y = _c(y if y is not None else np.zeros_like(x), F32).ravel()
+    out = np.empty_like(x)
+    lib().orc_math(fn, _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def cone_merge(a, b, oa, ob, on, conservative_radius=True):
+    out = np.empty(5, F32)
+    args = [_c(v, F32) for v in (a, b, oa, ob, on)]
+    lib().orc_cone_merge(*[_p(v) for v in args], _p(out), 1 if conservative_radius else 0)
+    return out
+
+
+def cone_overlap(cone, o, box, dist2):
+    return bool(lib().orc_cone_overlap(_p(_c(cone, F32)), _p(_c(o, F32)), _p(_c(box, F32)), float(dist2)))
+
+
+def closest_point(tree, q, nthreads=0, counts=False):
+    q = _c(q, F32).reshape(-1, 3)
+    n = q.shape[0]
+    idx = np.empty(n, U32)
+    dist = np.empty(n, F32)
+    cnt = np.zeros(3, np.uint64) if counts else None
+    lib().orc_closest_point(tree.ref, _p(q), n, _p(idx), _p(dist), _p(cnt), nthreads)
+    return (idx, dist, cnt) if counts else (idx, dist)
+
+
+def closest_silhouette(tree, q, flip=False, nthreads=0, counts=False):
+    q = _c(q, F32).reshape(-1, 3)
+    n = q.shape[0]
+    edge = np.empty(n, U32)
+    dist = np.empty(n, F32)
+    cnt = np.zeros(3, np.uint64) if counts else None
+    lib().orc_closest_silhouette(tree.ref, _p(q), n, int(bool(flip)), _p(edge), _p(dist), _p(cnt), nthreads)
+    return (edge, dist, cnt) if counts else (edge, dist)
+
+
+def ray(tree, o, d, tmax=None, any_hit=False, nthreads=0, counts=False):
+    o = _c(o, F32).reshape(-1, 3)
+    d = _c(d, F32).reshape(-1, 3)
+    n = o.shape[0]
+    tmax = _c(np.full(n, np.inf) if tmax is None else tmax, F32)
+    idx = np.empty(n, U32)
+    t = np.empty(n, F32)
+    uv = np.empty((n, 2), F32)
+    hit = np.empty(n, np.uint8)
+    cnt = np.zeros(3, np.uint64) if counts else None
+    lib().orc_ray(tree.ref, _p(o), _p(d), _p(tmax), n, int(bool(any_hit)), _p(idx), _p(t), _p(uv), _p(hit),
+                  _p(cnt), nthreads)
+    out = (hit.astype(bool), t, uv, idx)
+    return out + (cnt,) if counts else out
+
+
+def sample_in_sphere(tree, c, r, u, nthreads=0):
+    c = _c(c, F32).reshape(-1, 3)
+    n = c.shape[0]
+    r = _c(r, F32).ravel()
+    u = _c(u, F32).reshape(n, 3)
+    idx = np.empty(n, I32)
+    pdf = np.empty(n, F32)
+    pt = np.empty((n, 3), F32)
+    lib().orc_sample_in_sphere(tree.ref, _p(c), _p(r), _p(u), n, _p(idx), _p(pdf), _p(pt), nthreads)
+    return idx, pdf, pt
+
+
+def bf_closest_point(V, F, q, nthreads=0):
+    V = _c(V, F32).reshape(-1, 3)
+    F = _c(F, I32).reshape(-1, 3)
+    q = _c(q, F32).reshape(-1, 3)
+    n = q.shape[0]
+    idx = np.empty(n, U32)
+    dist = np.empty(n, np.float64)
+    lib().orc_bf_closest_point(_p(V), _p(F), F.shape[0], _p(q), n, _p(idx), _p(dist), nthreads)
+    return idx, dist
+
+
+def bf_closest_silhouette(V, slots, q, flip=False, nthreads=0):
+    V = _c(V, F32).reshape(-1, 3)
+    slots = _c(slots, I32).reshape(-1, 4)
+    q = _c(q, F32).reshape(-1, 3)
+    n = q.shape[0]
+    edge = np.empty(n, U32)
+    dist = np.empty(n, np.float64)
+    lib().orc_bf_closest_silhouette(_p(V), _p(slots), slots.shape[0], _p(q), n, int(bool(flip)), _p(edge),
+                                    _p(dist), nthreads)
+    return edge, dist
+
+
+def bf_ray(V, F, o, d, tmax=None, nthreads=0):
+    V = _c(V, F32).reshape(-1, 3)
+    F = _c(F, I32).reshape(-1, 3)
+    o = _c(o, F32).reshape(-1, 3)
+    d = _c(d, F32).reshape(-1, 3)
+    n = o.shape[0]
+    tmax = _c(np.full(n, np.inf) if tmax is None else tmax, F32)
+    idx = np.empty(n, U32)
+    t = np.empty(n, np.float64)
+    lib().orc_bf_ray(_p(V), _p(F), F.shape[0], _p(o), _p(d), _p(tmax), n, _p(idx), _p(t), nthreads)
+    return idx, t
+
+
+def tri_dist(V, F, tri, q, f64=False):
+    V = _c(V, F32).reshape(-1, 3)
+    F = _c(F, I32).reshape(-1, 3)
+    tri = _c(tri, U32).ravel()
+    q = _c(q, F32).reshape(-1, 3)
+    out = np.empty(tri.size, np.float64 if f64 else F32)
+    (lib().orc_tri_dist64 if f64 else lib().orc_tri_dist32)(_p(V), _p(F), _p(tri), _p(q), tri.size, _p(out))
+    return out
+
+
+def edge_sil_dist(V, slots, edge, q, flip=False):
+    """FP32 reference silhouette distance of the given edge (inf if not a silhouette)."""
+    V = _c(V, F32).reshape(-1, 3)
+    slots = _c(slots, I32).reshape(-1, 4)
+    edge = _c(edge, U32).ravel()
+    q = _c(q, F32).reshape(-1, 3)
+    out = np.empty(edge.size, F32)
+    lib().orc_edge_sil32(_p(V), _p(slots), _p(edge), _p(q), edge.size, int(bool(flip)), _p(out))
+    return out
+
+
+def edge_sil_dist64(V, slots, edge, q, flip=False):
+    """FP64 silhouette distance of the given edge under the reference predicate (inf if not one)."""
+    V = _c(V, F32).reshape(-1, 3)
+    slots = _c(slots, I32).reshape(-1, 4)
+    edge = _c(edge, U32).ravel()
+    q = _c(q, F32).reshape(-1, 3)
+    out = np.empty(edge.size, np.float64)
+    L = lib()
+    L.orc_edge_sil64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    L.orc_edge_sil64(_p(V), _p(slots), _p(edge), _p(q), edge.size, int(bool(flip)), _p(out))
+    return out

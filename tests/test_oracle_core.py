@@ -1,0 +1,443 @@
+"""Pins the CPU oracle (oracle/snch_oracle.c) without the reference, which has
+no tests, fixtures or golden vectors of its own: closed forms, exhaustive
+checks, an independent top-down radix-tree builder and FP64 brute force."""
+import numpy as np
+import pytest
+
+from snch_lbvh_amd import workloads as W
+
+
+# ---------------------------------------------------------------- math
+@pytest.mark.parametrize("fn,lo,hi", [(0, -1, 1), (1, -1, 1), (3, -7, 7), (4, -7, 7)])
+def test_detmath_unary_error_bound(oracle, fn, lo, hi):
+    x = np.linspace(lo, hi, 400001).astype(np.float32)
+    x = np.concatenate([x, np.float32([-1, -0.5, 0, 0.5, 1, 1e-30, -1e-30])])
+    got = oracle.math(fn, x).astype(np.float64)
+    ref = {0: np.arccos, 1: np.arcsin, 3: np.sin, 4: np.cos}[fn](x.astype(np.float64))
+    err = np.abs(got - ref)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert np.all((err <= 3 * ulp) | (err <= 2e-7 * 0.1)), (fn, float(np.max(err / ulp)))
+
+
+def test_detmath_atan2_error_bound(oracle):
+    rng = np.random.default_rng(0)
+    y = rng.normal(size=300000).astype(np.float32)
+    x = rng.normal(size=300000).astype(np.float32)
+    y[:8] = [0, 0, -0.0, 1, -1, 0, 3, 0]
+    x[:8] = [1, -1, -1, 0, 0, 0, np.inf, -np.inf]
+    got = oracle.math(2, y, x).astype(np.float64)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    err = np.abs(got - ref)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert np.all((err <= 3 * ulp) | (err <= 1e-8)), float(np.max(err / np.maximum(ulp, 1e-45)))
+    assert got[0] == 0.0 and got[1] == np.float32(np.pi) and got[2] == -np.float32(np.pi)
+
+
+# ---------------------------------------------------------------- Morton
+def _naive_expand(v):
+    out = 0
+    for b in range(10):
+        out |= ((v >> b) & 1) << (3 * b)
+    return out
+
+
+def test_expand_bits_exhaustive(oracle):
+    L = oracle.lib()
+    for v in range(1024):
+        assert L.orc_expand_bits(v) == _naive_expand(v)
+
+
+def test_morton_closed_form(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(1)
+    for x, y, z in rng.random((2000, 3)).astype(np.float32):
+        cx = [min(max(np.float32(c) * np.float32(1024), 0), 1023) for c in (x, y, z)]
+        ix = [int(c) for c in cx]
+        ref = _naive_expand(ix[0]) * 4 + _naive_expand(ix[1]) * 2 + _naive_expand(ix[2])
+        assert L.orc_morton3(float(x), float(y), float(z)) == ref
+    # clamping and the whole-box centre
+    assert L.orc_morton3(-1.0, 2.0, 0.5) == _naive_expand(1023) * 2 + _naive_expand(512)
+    assert L.orc_morton3(0.5, 0.5, 0.5) == 0b111 << 27
+
+
+# ---------------------------------------------------------------- edge table
+def _edge_table_literal(F):
+    """Literal restatement of scene.cuh:1135-1225 with a dict in place of std::map."""
+    index_map, E, tri_edges = {}, 0, []
+    for tri in F:
+        te = []
+        for j in range(3):
+            I, J = int(tri[j]), int(tri[(j + 1) % 3])
+            e = (min(I, J), max(I, J))
+            if e not in index_map:
+                index_map[e] = E
+                E += 1
+            te.append(index_map[e])
+        tri_edges.append(te)
+    slots = [[-1, -1, -1, -1] for _ in range(E)]
+    for i, tri in enumerate(F):
+        for j in range(3):
+            I, J, K = (j - 1) % 3, j, (j + 1) % 3
+            e = tri_edges[i][j]
+            orient = 1
+            if tri[J] > tri[K]:
+                J, K, orient = K, J, -1
+            slots[e][0 if orient == 1 else 3] = int(tri[I])
+            slots[e][1] = int(tri[J])
+            slots[e][2] = int(tri[K])
+    seen, owned = set(), []
+    for i in range(len(F)):
+        o = []
+        for j in range(3):
+            e = tri_edges[i][j]
+            if e not in seen:
+                seen.add(e)
+                o.append(e)
+        owned.append(o + [-1] * (3 - len(o)))
+    return np.array(tri_edges), np.array(slots), np.array(owned)
+
+
+def _meshes():
+    yield W.icosphere(2, 0.02, seed=5)
+    yield W.cut_torus(8, 6, seed=2)
+    rng = np.random.default_rng(3)
+    # triangle soup with shared, repeated and non-manifold edges and a degenerate triangle
+    F = rng.integers(0, 12, size=(60, 3)).astype(np.int32)
+    F[0] = [4, 4, 7]
+    V = rng.random((12, 3)).astype(np.float32)
+    yield V, F
+
+
+@pytest.mark.parametrize("mesh", list(range(3)))
+def test_edge_table_matches_literal(oracle, mesh):
+    V, F = list(_meshes())[mesh]
+    te, slots, owned = oracle.edge_table(F)
+    te2, slots2, owned2 = _edge_table_literal(F)
+    np.testing.assert_array_equal(te, te2)
+    np.testing.assert_array_equal(slots, slots2)
+    np.testing.assert_array_equal(owned, owned2)
+
+
+def test_cut_torus_boundary_count(oracle):
+    V, F = W.cut_torus(16, 8)
+    _, slots, _ = oracle.edge_table(F)
+    boundary = np.sum((slots[:, 0] == -1) | (slots[:, 3] == -1))
+    assert boundary == 2 * 8
+    assert slots.shape[0] == (3 * F.shape[0] + 16) // 2
+
+
+# ---------------------------------------------------------------- topology
+def _topdown(keys):
+    """Independent recursive radix tree with Karras numbering over unique keys."""
+    N = len(keys)
+    left = np.full(max(N - 1, 1), -1, np.int64)
+    right = np.full(max(N - 1, 1), -1, np.int64)
+
+    def clz(x):
+        return 64 - int(x).bit_length()
+
+    stack = [(0, 0, N - 1)]
+    while stack:
+        node, f, l = stack.pop()
+        dn = clz(keys[f] ^ keys[l])
+        lo, hi = f, l  # last index sharing more than dn bits with keys[f]
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if clz(keys[f] ^ keys[mid]) > dn:
+                lo = mid
+            else:
+                hi = mid
+        g = lo
+        lc = g + (N - 1) if g == f else g
+        rc = g + 1 + (N - 1) if g + 1 == l else g + 1
+        left[node], right[node] = lc, rc
+        if g != f:
+            stack.append((g, f, g))
+        if g + 1 != l:
+            stack.append((g + 1, g + 1, l))
+    return left, right
+
+
+@pytest.mark.parametrize("which", ["ico3", "ico5", "torus", "soup"])
+def test_topology_equals_topdown_builder(oracle, which):
+    if which == "ico3":
+        V, F = W.icosphere(3, 0.02, seed=2)
+    elif which == "ico5":
+        V, F = W.icosphere(5, 0.02, seed=1)
+    elif which == "torus":
+        V, F = W.cut_torus(64, 32)
+    else:
+        rng = np.random.default_rng(9)
+        V = rng.random((300, 3)).astype(np.float32)
+        F = rng.integers(0, 300, size=(500, 3)).astype(np.int32)
+    t = oracle.build_tree(V, F)
+    codes = t.sorted_codes.astype(np.uint64)
+    idx = t.sorted_idx.astype(np.uint64)
+    # sorted order is the stable order of (code, original index); codes recomputed
+    # from the leaf boxes with float32 numpy arithmetic (bvh.cuh:292-302)
+    lb, _ = oracle.leaf_box_cone(V, F)
+    lo, hi = t.scene_box[:3], t.scene_box[3:]
+    np.testing.assert_array_equal(lo, lb[:, :3].min(0))
+    np.testing.assert_array_equal(hi, lb[:, 3:].max(0))
+    cen = (lb[:, 3:] + lb[:, :3]) * np.float32(0.5)
+    p = (cen - lo) / (hi - lo)
+    L = oracle.lib()
+    raw = np.array([L.orc_morton3(float(a), float(b), float(c)) for a, b, c in p], np.uint64)
+    order = np.lexsort((np.arange(len(F)), raw))
+    assert np.array_equal(order, idx)
+    assert np.array_equal(raw[order], codes)
+    keys = (codes << np.uint64(32)) | idx
+    left, right = _topdown([int(k) for k in keys])
+    N = len(F)
+    np.testing.assert_array_equal(t.nodes[: N - 1, 1], left)
+    np.testing.assert_array_equal(t.nodes[: N - 1, 2], right)
+    assert np.all(t.nodes[N - 1:, 3] == idx)
+    assert t.nodes[0, 0] == 0xFFFFFFFF
+    for i in range(N - 1):
+        assert t.nodes[t.nodes[i, 1], 0] == i and t.nodes[t.nodes[i, 2], 0] == i
+
+
+def _leaf_ranges(t):
+    N = t.n_obj
+    rng_ = np.zeros((2 * N - 1, 2), np.int64)
+    rng_[N - 1:, 0] = np.arange(N)
+    rng_[N - 1:, 1] = np.arange(N)
+    order = []
+    stack = [0]
+    while stack:
+        n = stack.pop()
+        order.append(n)
+        if n < N - 1:
+            stack += [int(t.nodes[n, 1]), int(t.nodes[n, 2])]
+    for n in reversed(order):
+        if n < N - 1:
+            l, r = int(t.nodes[n, 1]), int(t.nodes[n, 2])
+            rng_[n] = [rng_[l, 0], rng_[r, 1]]
+    return rng_
+
+
+def test_aabb_refit_closed_form(oracle):
+    V, F = W.icosphere(4, 0.02, seed=3)
+    t = oracle.build_tree(V, F)
+    N = t.n_obj
+    leaves = t.boxes[N - 1:]
+    rr = _leaf_ranges(t)
+    for n in range(N - 1):
+        a, b = rr[n]
+        seg = leaves[a:b + 1]
+        np.testing.assert_array_equal(t.boxes[n, :3], seg[:, :3].min(0))
+        np.testing.assert_array_equal(t.boxes[n, 3:], seg[:, 3:].max(0))
+    # leaf boxes: vertex extent inflated by FLT_EPSILON
+    eps = np.float32(np.finfo(np.float32).eps)
+    lb, _ = oracle.leaf_box_cone(V, F)
+    tri = V[F]
+    np.testing.assert_array_equal(lb[:, :3], (tri - eps).min(1))
+    np.testing.assert_array_equal(lb[:, 3:], (tri + eps).max(1))
+
+
+def _angle(a, b):
+    c = np.clip(np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b)), -1, 1)
+    return np.arccos(c)
+
+
+@pytest.mark.parametrize("cons", [True, False])
+def test_cone_refit_invariants(oracle, cons):
+    V, F = W.cut_torus(24, 16)
+    t = oracle.build_tree(V, F, conservative_radius=cons)
+    N = t.n_obj
+    for n in range(N - 1):
+        p = t.cones[n]
+        for c in (t.nodes[n, 1], t.nodes[n, 2]):
+            ch = t.cones[c]
+            if ch[3] < 0 or p[3] >= np.float32(np.pi):
+                continue
+            assert _angle(p[:3], ch[:3]) + ch[3] <= p[3] + 1e-5
+    if cons:
+        # the sphere (node centroid, radius) encloses every owned edge end point below the node
+        rr = _leaf_ranges(t)
+        objs = t.nodes[N - 1:, 3]
+        for n in range(0, N - 1, 7):
+            a, b = rr[n]
+            es = t.owned[objs[a:b + 1]].ravel()
+            es = es[es >= 0]
+            if es.size == 0 or t.cones[n, 3] < 0:
+                continue
+            c = 0.5 * (t.boxes[n, :3].astype(np.float64) + t.boxes[n, 3:])
+            pts = V[t.slots[es][:, 1:3].ravel()].astype(np.float64)
+            assert np.max(np.linalg.norm(pts - c, axis=1)) <= t.cones[n, 4] * (1 + 1e-5) + 1e-6
+
+
+def test_cone_merge_identities(oracle):
+    c = np.float32([0, 0, 1, 0.3, 0.1])
+    o = np.float32([0, 0, 0])
+    m = oracle.cone_merge(c, c, o, o, o)
+    np.testing.assert_allclose(m[:4], c[:4], atol=1e-7)
+    # antiparallel axes (reference Q28): finite axis, half angle covers both
+    a = np.float32([0, 0, 1, 0.2, 0.0])
+    b = np.float32([0, 0, -1, 0.2, 0.0])
+    m = oracle.cone_merge(a, b, o, o, o)
+    assert np.all(np.isfinite(m))
+    assert m[3] >= np.pi / 2 + 0.2 - 1e-5
+    # invalid cones are ignored
+    inv = np.float32([0, 0, 0, -np.pi, 0])
+    np.testing.assert_array_equal(oracle.cone_merge(a, inv, o, o, o), a)
+    assert oracle.cone_merge(inv, inv, o, o, o)[3] < 0
+
+
+# ---------------------------------------------------------------- queries
+def _cp_agree(oracle, V, F, q, idx, dist, tol=1e-6):
+    bi, bd = oracle.bf_closest_point(V, F, q)
+    d64 = oracle.tri_dist(V, F, idx, q, f64=True)
+    assert np.all(d64 - bd <= tol * np.maximum(1, bd)), float(np.max(d64 - bd))
+    assert np.all(np.abs(dist - bd) <= 1e-6 + 1e-6 * bd)
+    return np.mean(idx == bi)
+
+
+def test_closest_point_vs_fp64_bruteforce(oracle):
+    V, F, q = W.config_small(4096)
+    t = oracle.build_tree(V, F)
+    idx, dist = oracle.closest_point(t, q)
+    _cp_agree(oracle, V, F, q, idx, dist)
+
+
+def test_closest_point_sphere_kat(oracle):
+    V, F = W.icosphere(5, 0.0)
+    t = oracle.build_tree(V, F)
+    q = W.unit_directions(2000, 11) * np.float32(1.7)
+    _, dist = oracle.closest_point(t, q)
+    # chordal error of a level-5 icosphere is below 1e-3
+    assert np.all(np.abs(dist - 0.7) < 1.5e-3)
+
+
+def test_closest_silhouette_vs_fp64_bruteforce(oracle):
+    for V, F, q in [W.config_small(4096),
+                    (lambda V, F: (V, F, W.uniform_points(4096, *W.padded_box(V), 4)))(*W.cut_torus(64, 48))]:
+        t = oracle.build_tree(V, F)
+        e, d = oracle.closest_silhouette(t, q)
+        be, bd = oracle.bf_closest_silhouette(V, t.slots, q)
+        fin = np.isfinite(bd)
+        assert np.array_equal(np.isfinite(d), fin)
+        d64 = oracle.edge_sil_dist64(V, t.slots, e[fin], q[fin])
+        assert np.all(d64 - bd[fin] <= 1e-6), float(np.max(d64 - bd[fin]))
+
+
+def test_silhouette_sphere_kats(oracle):
+    V, F = W.icosphere(4, 0.0)
+    t = oracle.build_tree(V, F)
+    inside = W.unit_directions(500, 3) * np.float32(0.5)
+    _, d_in = oracle.closest_silhouette(t, inside)
+    assert np.all(np.isinf(d_in))
+    rho = 2.0
+    outside = W.unit_directions(500, 4) * np.float32(rho)
+    _, d_out = oracle.closest_silhouette(t, outside)
+    # the polyhedral silhouette lies within one edge length of the smooth tangent circle
+    hmax = np.max(np.linalg.norm(V[F] - V[np.roll(F, 1, axis=1)], axis=2))
+    assert np.all(np.abs(d_out - np.sqrt(rho * rho - 1)) < hmax)
+
+
+def test_boundary_edges_always_silhouettes(oracle):
+    V, F = W.cut_torus(32, 16, amp=0.0)
+    t = oracle.build_tree(V, F)
+    b = np.where((t.slots[:, 0] == -1) | (t.slots[:, 3] == -1))[0]
+    mid = 0.5 * (V[t.slots[b, 1]] + V[t.slots[b, 2]])
+    q = mid + np.float32([0.0, -0.05, 0.0])
+    _, d = oracle.closest_silhouette(t, q)
+    assert np.all(d <= 0.05 + 1e-6)
+
+
+def test_ray_vs_fp64_bruteforce_and_kat(oracle):
+    V, F, q = W.config_small(4096)
+    t = oracle.build_tree(V, F)
+    dirs = W.unit_directions(len(q), 5)
+    hit, tt, uv, idx = oracle.ray(t, q, dirs)
+    bi, bt = oracle.bf_ray(V, F, q, dirs)
+    assert np.array_equal(hit, np.isfinite(bt))
+    np.testing.assert_allclose(tt[hit], bt[hit], rtol=1e-5, atol=1e-6)
+    anyhit, _, _, _ = oracle.ray(t, q, dirs, any_hit=True)
+    assert np.array_equal(anyhit, hit)
+    V0, F0 = W.icosphere(3, 0.0)
+    t0 = oracle.build_tree(V0, F0)
+    d = W.unit_directions(1000, 8)
+    hit0, t_0, _, _ = oracle.ray(t0, np.zeros_like(d), d)
+    assert hit0.all() and np.all(t_0 <= 1 + 1e-6) and np.all(t_0 >= 0.98)
+
+
+def _path_probabilities(t, c, r):
+    """Exact descent probabilities of sample.cuh:23-92 in float64."""
+    N = t.n_obj
+
+    def inter(box):
+        cl = np.maximum(box[:3], np.minimum(c, box[3:]))
+        return np.sum((cl - c) ** 2) <= r * r
+
+    def weight(box):
+        cen = 0.5 * (box[:3].astype(np.float64) + box[3:])
+        return 1.0 / (4 * np.pi * max(np.linalg.norm(c - cen), 1e-4))
+
+    probs = {}
+    stack = [(0, 1.0)]
+    while stack:
+        n, p = stack.pop()
+        if n >= N - 1 or N == 1:
+            probs[int(t.nodes[n, 3])] = probs.get(int(t.nodes[n, 3]), 0.0) + p
+            continue
+        l, rr = int(t.nodes[n, 1]), int(t.nodes[n, 2])
+        wl = weight(t.boxes[l]) if inter(t.boxes[l]) else 0.0
+        wr = weight(t.boxes[rr]) if inter(t.boxes[rr]) else 0.0
+        if wl + wr > 0:
+            if wl > 0:
+                stack.append((l, p * wl / (wl + wr)))
+            if wr > 0:
+                stack.append((rr, p * wr / (wl + wr)))
+    return probs
+
+
+def test_sample_in_sphere_distribution(oracle):
+    V, F = W.icosphere(2, 0.02, seed=4)
+    t = oracle.build_tree(V, F)
+    c = np.float32([0.3, 0.8, 0.2])
+    r = np.float32(0.6)
+    n = 200000
+    u = np.random.default_rng(5).random((n, 3)).astype(np.float32)
+    idx, pdf, pt = oracle.sample_in_sphere(t, np.tile(c, (n, 1)), np.full(n, r), u)
+    probs = _path_probabilities(t, c.astype(np.float64), float(r))
+    # every returned index is a reachable leaf and its pdf is path probability / area
+    area = 0.5 * np.linalg.norm(np.cross(V[F[:, 2]] - V[F[:, 0]], V[F[:, 1]] - V[F[:, 0]]), axis=1)
+    ok = idx >= 0
+    exp_pdf = np.array([probs[i] for i in idx[ok]]) / area[idx[ok]]
+    np.testing.assert_allclose(pdf[ok], exp_pdf, rtol=2e-4)
+    assert np.all(pdf[~ok] == 0)
+    # chi-square of the histogram against the exact leaf probabilities
+    keys = sorted(probs)
+    cnt = np.array([np.sum(idx == k) for k in keys], np.float64)
+    miss = n - ok.sum()
+    p = np.array([probs[k] for k in keys])
+    exp = n * p
+    # leaves whose triangle fails the sphere test, and descents that reach two
+    # zero-weight children, return -1: fold them into one bin
+    fails = np.array([cnt[i] == 0 and exp[i] > 50 for i in range(len(keys))])
+    exp_miss = exp[fails].sum() + n * (1.0 - p.sum())
+    chi = np.sum((cnt[~fails] - exp[~fails]) ** 2 / exp[~fails]) + (miss - exp_miss) ** 2 / max(exp_miss, 1)
+    assert chi < len(keys) + 6 * np.sqrt(2 * len(keys)), chi
+    # sample points lie on the returned triangle
+    tri = V[F[idx[ok]]].astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    assert np.max(np.abs(np.sum((pt[ok] - tri[:, 0]) * nrm, 1))) < 1e-5
+
+
+def test_degenerate_sizes(oracle):
+    V = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.5]])
+    for F in (np.int32([[0, 1, 2]]), np.int32([[0, 1, 2], [1, 3, 2]])):
+        t = oracle.build_tree(V, F)
+        q = np.float32([[0.2, 0.2, 1.0], [2, 2, 2]])
+        idx, d = oracle.closest_point(t, q)
+        bi, bd = oracle.bf_closest_point(V, F, q)
+        np.testing.assert_allclose(d, bd, rtol=1e-6)
+        e, sd = oracle.closest_silhouette(t, q)
+        be, bsd = oracle.bf_closest_silhouette(V, t.slots, q)
+        np.testing.assert_allclose(sd, bsd, rtol=1e-6)
+        hit, tt, _, ri = oracle.ray(t, np.float32([[0.2, 0.2, 1.0]]), np.float32([[0, 0, -1]]))
+        assert hit[0] and abs(tt[0] - 1.0) < 1e-6 and ri[0] == 0
+    with pytest.raises(ValueError):
+        oracle.build_tree(V, np.zeros((0, 3), np.int32))
