@@ -1,0 +1,149 @@
+/*
+ * snch.h -- C ABI of the MI355X (gfx950) SNCH-LBVH library (libsnch.so).
+ *
+ * The reference (tyanyuy3125/snch-lbvh) is a header-only CUDA template
+ * library; its host entry points are lbvh::scene<2|3> and lbvh::bvh, and its
+ * device entry points are query_device()/sample_object_in_sphere() called
+ * one thread per query from the user's own kernels.  This ABI replaces both
+ * with plain C: host-side scene/build handles and batched query launches on a
+ * caller-supplied HIP stream.  No C++ or HIP types cross the boundary.
+ *
+ * Conventions
+ *  - Every function returns SNCH_OK (0) or an SNCH_ERR_* code; the message of
+ *    the most recent failure on the calling thread is snch_last_error().
+ *  - Query inputs and outputs are DEVICE pointers (hipMalloc'ed or torch CUDA
+ *    tensors); mesh inputs to snch_scene_create_* are HOST pointers.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Query
+ *    launches are asynchronous; builds complete before returning.
+ *  - A scene handle is externally synchronised: one thread at a time per
+ *    handle; different handles may be used from different threads.
+ *  - Index sentinels follow the reference: 0xFFFFFFFF for "no primitive"
+ *    (query.cuh:254), +inf distance (query.cuh:343), -1 for "no sample"
+ *    (sample.cuh:255).
+ */
+#ifndef SNCH_H
+#define SNCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNCH_OK 0
+#define SNCH_ERR_INVALID 1  /* bad argument (null pointer, index out of range, NaN vertex) */
+#define SNCH_ERR_EMPTY 2    /* empty mesh: the reference silently returns (bvh.cuh:383-386) */
+#define SNCH_ERR_HIP 3      /* HIP runtime failure */
+#define SNCH_ERR_STATE 4    /* "BVH is not built yet." (scene.cuh:686, :1250) */
+#define SNCH_ERR_OVERFLOW 5 /* a traversal exceeded its stack (query.cuh:315, :420 assert) */
+
+typedef struct snch_context snch_context;
+typedef struct snch_scene snch_scene;
+
+/* Build flags */
+#define SNCH_BUILD_REFERENCE_RADIUS 1u /* use the reference's cone radius (midpoints, sqrt merge);
+                                          default is the enclosing radius, which never prunes a
+                                          true nearest silhouette */
+
+typedef struct snch_build_opts {
+    uint32_t flags;     /* SNCH_BUILD_* */
+    uint32_t leaf_size; /* 1 = the reference topology (bvh.cuh:200-229); 0 means 1 */
+} snch_build_opts;
+
+/* Traversal tuning (chosen by the autotuner; defaults are safe) */
+typedef struct snch_query_opts {
+    uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (256) */
+    uint32_t coherent;   /* 1 = reorder queries by Morton code before traversal */
+    uint64_t *stats;     /* optional DEVICE uint64[4], accumulated: child box tests, leaf
+                            evaluations, silhouette-edge tests, nodes visited */
+} snch_query_opts;
+
+const char *snch_last_error(void);
+int snch_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+int snch_context_create(int device, snch_context **out);
+int snch_context_destroy(snch_context *ctx);
+
+/* ---- scenes ------------------------------------------------------------- */
+/* scene<3>(vertices, indices) + compute_silhouettes() (scene.cuh:1128-1204).
+ * V: nV x 3 floats, F: nF x 3 int32 (host).  Builds the edge table with the
+ * reference's edge ids (first appearance over triangle i, local edge
+ * (v_j, v_{j+1})), edge slots (opp0, lo, hi, opp1) and edge ownership. */
+int snch_scene_create_tri(snch_context *ctx, const float *V, uint32_t nV, const int32_t *F, uint32_t nF,
+                          snch_scene **out);
+/* scene<2>(vertices, indices) + compute_silhouettes() (scene.cuh:624-652).
+ * V: nV x 2 floats, S: nS x 2 int32 line segments (host). */
+int snch_scene_create_seg(snch_context *ctx, const float *V, uint32_t nV, const int32_t *S, uint32_t nS,
+                          snch_scene **out);
+int snch_scene_destroy(snch_scene *scene);
+/* dim (2|3), primitive count, silhouette count (edges in 3D, vertices in 2D) */
+int snch_scene_info(const snch_scene *scene, uint32_t *dim, uint32_t *n_prims, uint32_t *n_silhouettes);
+
+/* scene<3>::build_bvh() -> bvh::construct() (scene.cuh:1205-1229, bvh.cuh:380-613),
+ * entirely on the GPU.  Synchronises `stream` before returning. */
+int snch_scene_build(snch_scene *scene, const snch_build_opts *opts, void *stream);
+/* Milliseconds of GPU time of the last build, measured with HIP events. */
+int snch_scene_build_time(const snch_scene *scene, float *ms);
+
+/* ---- queries (device pointers; asynchronous on `stream`) ---------------- */
+/* query_device(bvh, nearest(p), distance_calculator) (query.cuh:238-318).
+ * q: n x dim floats.  out_idx: primitive index, out_dist: distance. */
+int snch_closest_point(const snch_scene *scene, const float *q, uint32_t n, uint32_t *out_idx, float *out_dist,
+                       void *stream);
+/* query_device(bvh, nearest_silhouette(p, flip), silhouette_distance_calculator)
+ * (query.cuh:325-423).  The reference returns only the distance; out_sil (may
+ * be NULL) also receives the silhouette id (edge id in 3D, vertex id in 2D).
+ * seed_radius (may be NULL): per-query pruning cap r0; the search is exact
+ * within r0 and falls back to the full search when nothing is found there, so
+ * outputs equal those of the unseeded call. */
+int snch_closest_silhouette(const snch_scene *scene, const float *q, uint32_t n, int flip, const float *seed_radius,
+                            uint32_t *out_sil, float *out_dist, void *stream);
+/* query_device(bvh, ray_intersect<TestOnly>(ray(o, d), tmax), intersect_test)
+ * (query.cuh:79-169).  tmax may be NULL (= +inf).  any_hit selects TestOnly:
+ * only out_hit is meaningful.  out_idx/out_t/out_uv may be NULL. */
+int snch_ray(const snch_scene *scene, const float *o, const float *d, const float *tmax, uint32_t n, int any_hit,
+             uint8_t *out_hit, float *out_t, float *out_uv, uint32_t *out_idx, void *stream);
+/* sample_object_in_sphere(bvh, sphere_intersect(c, r), intersect_sphere,
+ * measurement_getter, green_weight, u[0]) then, if out_pt != NULL,
+ * sample_on_object(bvh, idx, sample_on_object, u[1], u[2]) (sample.cuh:7-92).
+ * c: n x dim, r: n, u: n x 3.  out_idx = -1 and out_pdf = 0 when nothing is
+ * sampled. */
+int snch_sample_in_sphere(const snch_scene *scene, const float *c, const float *r, const float *u, uint32_t n,
+                          int32_t *out_idx, float *out_pdf, float *out_pt, void *stream);
+/* sample_on_object(bvh, idx, sample_on_object, u, v) (sample.cuh:7-21; scene.cuh:1231-1240) */
+int snch_sample_on_object(const snch_scene *scene, const int32_t *idx, const float *uv, uint32_t n, float *out_pt,
+                          void *stream);
+/* Same queries with explicit traversal options. */
+int snch_closest_point_ex(const snch_scene *scene, const float *q, uint32_t n, uint32_t *out_idx, float *out_dist,
+                          const snch_query_opts *opts, void *stream);
+int snch_ray_ex(const snch_scene *scene, const float *o, const float *d, const float *tmax, uint32_t n, int any_hit,
+                uint8_t *out_hit, float *out_t, float *out_uv, uint32_t *out_idx, const snch_query_opts *opts,
+                void *stream);
+int snch_closest_silhouette_ex(const snch_scene *scene, const float *q, uint32_t n, int flip,
+                               const float *seed_radius, uint32_t *out_sil, float *out_dist,
+                               const snch_query_opts *opts, void *stream);
+
+/* ---- inspection (host outputs) ------------------------------------------ */
+/* The tree in the reference's layout (bvh.cuh:27-33, :625-633): nodes are
+ * (parent, left, right, object) per node, internal nodes 0..N-2, leaves
+ * N-1..2N-2; boxes are (lo xyz, hi xyz); cones (axis xyz, half_angle, radius)
+ * in 3D or (axis xy, half_angle, radius) in 2D.  Any output may be NULL. */
+int snch_scene_export(const snch_scene *scene, uint32_t *nodes, float *boxes, float *cones, uint32_t *sorted_codes,
+                      uint32_t *sorted_idx);
+/* Edge (3D) or vertex (2D) silhouette table: slots (opp0, lo, hi, opp1) per
+ * edge / (prev, v, next, -1) per vertex, and per-primitive owned ids (3 per
+ * triangle, 2 per segment, -1 padded). */
+int snch_scene_silhouettes(const snch_scene *scene, int32_t *slots, int32_t *owned);
+
+/* ---- device memory helpers for callers without their own allocator ------- */
+int snch_device_malloc(void **ptr, uint64_t bytes);
+int snch_device_free(void *ptr);
+int snch_memcpy_h2d(void *dst, const void *src, uint64_t bytes, void *stream);
+int snch_memcpy_d2h(void *dst, const void *src, uint64_t bytes, void *stream);
+int snch_stream_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNCH_H */

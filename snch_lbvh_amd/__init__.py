@@ -1,0 +1,255 @@
+"""SNCH-LBVH for MI355X: Python mirror of the reference's ``lbvh::scene`` API.
+
+The compute path is ``libsnch.so`` (hand-written HIP kernels for gfx950 behind
+the C ABI in ``include/snch.h``); this module only binds it with ctypes and
+passes device pointers of torch CUDA tensors.  There is no CPU fallback: if
+the library or a GPU is missing, construction raises.
+
+Reference API -> this module
+  lbvh::scene<3>(vb, ve, ib, ie); compute_silhouettes()  -> Scene(V, F)
+  scene.build_bvh()                                    -> Scene.build_bvh()
+  query_device(bvh, nearest(p), distance_calculator)   -> Scene.closest_point(q)
+  query_device(bvh, nearest_silhouette(p, flip), ...)  -> Scene.closest_silhouette(q, flip)
+  query_device(bvh, ray_intersect<TestOnly>(r, tmax))  -> Scene.ray_intersect(o, d, tmax, any_hit)
+  sample_object_in_sphere(...) + sample_on_object(...) -> Scene.sample_in_sphere(c, r, u)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsnch.so")
+_lib = None
+
+SNCH_OK = 0
+ERRORS = {1: "invalid argument", 2: "empty mesh", 3: "HIP error", 4: "not built", 5: "stack overflow"}
+BUILD_REFERENCE_RADIUS = 1
+
+
+class SnchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"snch error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libsnch.so (import torch first so that one HIP runtime serves both)."""
+    global _lib
+    if _lib is None:
+        import torch  # noqa: F401  (shares its libamdhip64 with libsnch)
+
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m snch_lbvh_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32)]
+
+
+class QueryOpts(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p)]
+
+
+EXPORTED = [
+    "snch_last_error", "snch_version", "snch_context_create", "snch_context_destroy", "snch_scene_create_tri",
+    "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_info", "snch_scene_build", "snch_scene_build_time",
+    "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette", "snch_closest_silhouette_ex",
+    "snch_ray", "snch_ray_ex",
+    "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
+    "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
+]
+
+
+def _declare(L):
+    vp, u32, i32, c_int = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int
+    L.snch_last_error.restype = ctypes.c_char_p
+    L.snch_context_create.argtypes = [c_int, ctypes.POINTER(vp)]
+    L.snch_context_destroy.argtypes = [vp]
+    L.snch_scene_create_tri.argtypes = [vp, vp, u32, vp, u32, ctypes.POINTER(vp)]
+    L.snch_scene_create_seg.argtypes = [vp, vp, u32, vp, u32, ctypes.POINTER(vp)]
+    L.snch_scene_destroy.argtypes = [vp]
+    L.snch_scene_info.argtypes = [vp, vp, vp, vp]
+    L.snch_scene_build.argtypes = [vp, vp, vp]
+    L.snch_scene_build_time.argtypes = [vp, vp]
+    L.snch_closest_point.argtypes = [vp, vp, u32, vp, vp, vp]
+    L.snch_closest_silhouette.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp]
+    L.snch_closest_silhouette_ex.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp, vp]
+    L.snch_closest_point_ex.argtypes = [vp, vp, u32, vp, vp, vp, vp]
+    L.snch_ray_ex.argtypes = [vp, vp, vp, vp, u32, c_int, vp, vp, vp, vp, vp, vp]
+    L.snch_ray.argtypes = [vp, vp, vp, vp, u32, c_int, vp, vp, vp, vp, vp]
+    L.snch_sample_in_sphere.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp]
+    L.snch_sample_on_object.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.snch_scene_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.snch_scene_silhouettes.argtypes = [vp, vp, vp]
+    for name in EXPORTED:
+        if name != "snch_last_error":
+            getattr(L, name).restype = i32
+
+
+def check(rc):
+    if rc != SNCH_OK:
+        raise SnchError(rc, lib().snch_last_error().decode())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+_contexts = {}
+
+
+def context(device=0):
+    if device not in _contexts:
+        c = ctypes.c_void_p()
+        check(lib().snch_context_create(device, ctypes.byref(c)))
+        _contexts[device] = c
+    return _contexts[device]
+
+
+def _dev_f32(x, cols, device):
+    import torch
+
+    t = torch.as_tensor(x, dtype=torch.float32, device=f"cuda:{device}")
+    if cols:
+        t = t.reshape(-1, cols)
+    return t.contiguous()
+
+
+class Scene:
+    """``lbvh::scene<3>`` on one GPU (scene.cuh:705-1268)."""
+
+    def __init__(self, vertices, indices, device=0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("snch_lbvh_amd needs a ROCm GPU; none is visible")
+        self.device = device
+        torch.cuda.set_device(device)
+        V = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, 3))
+        F = np.ascontiguousarray(np.asarray(indices, np.int32).reshape(-1, 3))
+        self.V, self.F = V, F
+        h = ctypes.c_void_p()
+        check(lib().snch_scene_create_tri(context(device), V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0],
+                                          ctypes.byref(h)))
+        self._h = h
+        dim, n_prims, n_sil = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().snch_scene_info(h, ctypes.byref(dim), ctypes.byref(n_prims), ctypes.byref(n_sil)))
+        self.n_prims, self.n_silhouettes = n_prims.value, n_sil.value
+        self.built = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.snch_scene_destroy(h)
+            self._h = None
+
+    # scene<3>::compute_silhouettes (scene.cuh:1167-1204) runs at construction
+    def compute_silhouettes(self):
+        return self
+
+    def build_bvh(self, reference_radius=False, leaf_size=1):
+        """scene<3>::build_bvh (scene.cuh:1205-1229) -> bvh::construct on the GPU."""
+        opts = BuildOpts(BUILD_REFERENCE_RADIUS if reference_radius else 0, leaf_size)
+        check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
+        self.built = True
+        return self
+
+    def build_ms(self):
+        ms = ctypes.c_float()
+        check(lib().snch_scene_build_time(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def closest_point(self, q, stats=None):
+        """stats: optional int64 CUDA tensor of 4 counters (box tests, leaf
+        evaluations, edge tests, nodes visited), accumulated."""
+        import torch
+
+        q = _dev_f32(q, 3, self.device)
+        n = q.shape[0]
+        idx = torch.empty(n, dtype=torch.int32, device=q.device)
+        dist = torch.empty(n, dtype=torch.float32, device=q.device)
+        opts = QueryOpts(0, 0, stats.data_ptr() if stats is not None else None)
+        check(lib().snch_closest_point_ex(self._h, _ptr(q), n, _ptr(idx), _ptr(dist), ctypes.byref(opts), _stream()))
+        return idx, dist
+
+    def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None):
+        import torch
+
+        q = _dev_f32(q, 3, self.device)
+        n = q.shape[0]
+        seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
+        sil = torch.empty(n, dtype=torch.int32, device=q.device)
+        dist = torch.empty(n, dtype=torch.float32, device=q.device)
+        opts = QueryOpts(block_size, 0, stats.data_ptr() if stats is not None else None)
+        check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
+                                               _ptr(dist), ctypes.byref(opts), _stream()))
+        return sil, dist
+
+    def ray_intersect(self, o, d, tmax=None, any_hit=False, stats=None):
+        import torch
+
+        o = _dev_f32(o, 3, self.device)
+        d = _dev_f32(d, 3, self.device)
+        n = o.shape[0]
+        tm = _dev_f32(tmax, 0, self.device).reshape(-1) if tmax is not None else None
+        hit = torch.empty(n, dtype=torch.uint8, device=o.device)
+        t = torch.empty(n, dtype=torch.float32, device=o.device)
+        uv = torch.empty((n, 2), dtype=torch.float32, device=o.device)
+        idx = torch.empty(n, dtype=torch.int32, device=o.device)
+        opts = QueryOpts(0, 0, stats.data_ptr() if stats is not None else None)
+        check(lib().snch_ray_ex(self._h, _ptr(o), _ptr(d), _ptr(tm), n, int(bool(any_hit)), _ptr(hit), _ptr(t),
+                                _ptr(uv), _ptr(idx), ctypes.byref(opts), _stream()))
+        return hit.bool(), t, uv, idx
+
+    def sample_in_sphere(self, c, r, u, with_point=True):
+        import torch
+
+        c = _dev_f32(c, 3, self.device)
+        n = c.shape[0]
+        r = _dev_f32(r, 0, self.device).reshape(-1)
+        u = _dev_f32(u, 3, self.device)
+        idx = torch.empty(n, dtype=torch.int32, device=c.device)
+        pdf = torch.empty(n, dtype=torch.float32, device=c.device)
+        pt = torch.empty((n, 3), dtype=torch.float32, device=c.device) if with_point else None
+        check(lib().snch_sample_in_sphere(self._h, _ptr(c), _ptr(r), _ptr(u), n, _ptr(idx), _ptr(pdf), _ptr(pt),
+                                          _stream()))
+        return idx, pdf, pt
+
+    def sample_on_object(self, idx, uv):
+        import torch
+
+        idx = torch.as_tensor(idx, dtype=torch.int32, device=f"cuda:{self.device}").contiguous()
+        uv = _dev_f32(uv, 2, self.device)
+        pt = torch.empty((idx.shape[0], 3), dtype=torch.float32, device=idx.device)
+        check(lib().snch_sample_on_object(self._h, _ptr(idx), _ptr(uv), idx.shape[0], _ptr(pt), _stream()))
+        return pt
+
+    def export(self):
+        """The tree in the reference layout (host numpy arrays)."""
+        N = self.n_prims
+        NN = 2 * N - 1
+        nodes = np.empty((NN, 4), np.uint32)
+        boxes = np.empty((NN, 6), np.float32)
+        cones = np.empty((NN, 5), np.float32)
+        codes = np.empty(N, np.uint32)
+        sidx = np.empty(N, np.uint32)
+        check(lib().snch_scene_export(self._h, nodes.ctypes.data, boxes.ctypes.data, cones.ctypes.data,
+                                      codes.ctypes.data, sidx.ctypes.data))
+        return dict(nodes=nodes, boxes=boxes, cones=cones, sorted_codes=codes, sorted_idx=sidx)
+
+    def silhouettes(self):
+        slots = np.empty((self.n_silhouettes, 4), np.int32)
+        owned = np.empty((self.n_prims, 3), np.int32)
+        check(lib().snch_scene_silhouettes(self._h, slots.ctypes.data, owned.ctypes.data))
+        return slots, owned

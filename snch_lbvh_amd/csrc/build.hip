@@ -1,0 +1,424 @@
+// LBVH construction for triangle scenes on gfx950 (replaces bvh::construct,
+// bvh.cuh:380-613, and the per-object getters of scene.cuh:870-961).
+//
+//   k_init_scene_box  scene-box accumulators to (+inf, -inf)
+//   k_leafgen         one thread per triangle: leaf AABB (vertex extent
+//                     +- FLT_EPSILON), leaf normal cone from the owned
+//                     silhouette edges, the packed silhouette-edge records,
+//                     and a wave64 + LDS reduction of the scene AABB finished
+//                     with one ordered-int atomic per block and component
+//   k_morton          30-bit Morton code of each leaf-box centroid
+//   radix sort        stable (code, object index) sort, sort.hip
+//   k_karras          Karras (2012) radix-tree emission.  Keys are compared
+//                     as the 64-bit (code << 32 | index): with unique codes
+//                     this is exactly the reference's 32-bit tree, and with
+//                     duplicate codes it is the reference's 64-bit fallback
+//                     (bvh.cuh:459-476), so no uniqueness pass or host
+//                     round trip is needed.
+//   k_refit           one thread per leaf climbs with an agent-scope
+//                     acquire-release counter per internal node; the second
+//                     arrival merges the children's AABB and then their cone
+//                     (the parent box centroid feeds the cone merge, as in
+//                     bvh.cuh:598) and writes both the reference-layout node
+//                     data and the packed traversal record.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace snch {
+
+namespace {
+
+constexpr int BT = 256;
+
+__device__ __forceinline__ int f2ord(float f)
+{
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__device__ __forceinline__ v3 ld3(const float4* V, int i)
+{
+    float4 a = V[i];
+    return mk(a.x, a.y, a.z);
+}
+
+__global__ void k_init_scene_box(int* bits)
+{
+    int t = threadIdx.x;
+    if (t < 3)
+        bits[t] = f2ord(F_INF);
+    else if (t < 6)
+        bits[t] = f2ord(-F_INF);
+}
+
+// normal of face f of a silhouette edge (scene.cuh:747-772); slots (opp0, lo, hi, opp1)
+__device__ __forceinline__ v3 edge_face_normal_raw(const float4* V, int4 s, int f)
+{
+    v3 pa, pb, pc;
+    if (f == 0) {
+        pa = ld3(V, s.y);
+        pb = ld3(V, s.z);
+        pc = ld3(V, s.w);
+    } else {
+        pa = ld3(V, s.z);
+        pb = ld3(V, s.y);
+        pc = ld3(V, s.x);
+    }
+    return cross3(mk(pb.x - pa.x, pb.y - pa.y, pb.z - pa.z), mk(pc.x - pa.x, pc.y - pa.y, pc.z - pa.z));
+}
+
+__global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, const int32_t* __restrict__ F,
+                                                const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
+                                                uint32_t nF, int conservative, float4* __restrict__ leafbox,
+                                                float4* __restrict__ leafcone, float4* __restrict__ tedge,
+                                                int* __restrict__ scene_bits)
+{
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    box3 b;
+    b.lo = mk(F_INF, F_INF, F_INF);
+    b.hi = mk(-F_INF, -F_INF, -F_INF);
+    if (i < nF) {
+        const v3 p0 = ld3(V, F[3 * i]), p1 = ld3(V, F[3 * i + 1]), p2 = ld3(V, F[3 * i + 2]);
+        b = box_of_point(p0);
+        box_expand(b, p1);
+        box_expand(b, p2);
+        const v3 c = box_centroid(b);
+        // leaf cone (scene.cuh:887-961)
+        cone3 r;
+        r.axis = mk(0.0f, 0.0f, 0.0f);
+        r.half_angle = F_PI;
+        r.radius = 0.0f;
+        bool any = false, all_two = true;
+        const uint32_t e0 = estart[i], e1 = estart[i + 1];
+        v3 nn[3][2];
+        for (uint32_t k = 0; k < 3; ++k) {
+            if (e0 + k >= e1)
+                break;
+            const uint32_t e = e0 + k;
+            const int4 s = slots[e];
+            const bool has0 = s.w != -1, has1 = s.x != -1;
+            v3 raw0 = mk(0.0f, 0.0f, 0.0f), raw1 = mk(0.0f, 0.0f, 0.0f);
+            v3 n = mk(0.0f, 0.0f, 0.0f);
+            if (has0) {
+                raw0 = edge_face_normal_raw(V, s, 0);
+                n = mk(n.x + raw0.x, n.y + raw0.y, n.z + raw0.z);
+            }
+            if (has1) {
+                raw1 = edge_face_normal_raw(V, s, 1);
+                n = mk(n.x + raw1.x, n.y + raw1.y, n.z + raw1.z);
+            }
+            n = normalize3(n);
+            const v3 pa = ld3(V, s.y), pb = ld3(V, s.z);
+            r.axis.x += n.x;
+            r.axis.y += n.y;
+            r.axis.z += n.z;
+            if (conservative) {
+                r.radius = smax(r.radius, len3(mk(pa.x - c.x, pa.y - c.y, pa.z - c.z)));
+                r.radius = smax(r.radius, len3(mk(pb.x - c.x, pb.y - c.y, pb.z - c.z)));
+            } else {
+                const v3 m = mk((pa.x + pb.x) / 2.0f, (pa.y + pb.y) / 2.0f, (pa.z + pb.z) / 2.0f);
+                r.radius = smax(r.radius, len3(mk(m.x - c.x, m.y - c.y, m.z - c.z)));
+            }
+            all_two = all_two && has0 && has1;
+            any = true;
+            nn[k][0] = has0 ? normalize3(raw0) : mk(0.0f, 0.0f, 0.0f);
+            nn[k][1] = has1 ? normalize3(raw1) : mk(0.0f, 0.0f, 0.0f);
+            // packed silhouette-edge record (each edge has exactly one owner)
+            tedge[4 * e + 0] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(e));
+            tedge[4 * e + 1] =
+                make_float4(pb.x, pb.y, pb.z, __uint_as_float((has0 ? 1u : 0u) | (has1 ? 2u : 0u)));
+            tedge[4 * e + 2] = make_float4(nn[k][0].x, nn[k][0].y, nn[k][0].z, 0.0f);
+            tedge[4 * e + 3] = make_float4(nn[k][1].x, nn[k][1].y, nn[k][1].z, 0.0f);
+        }
+        if (!any) {
+            r.half_angle = -F_PI;
+        } else if (!all_two) {
+            r.half_angle = F_PI;
+        } else {
+            const float an = len3(r.axis);
+            if (an > F_EPS) {
+                r.axis.x /= an;
+                r.axis.y /= an;
+                r.axis.z /= an;
+                r.half_angle = 0.0f;
+                for (uint32_t k = 0; k < 3; ++k) {
+                    if (e0 + k >= e1)
+                        break;
+                    for (int jj = 0; jj < 2; ++jj) {
+                        const v3 nv = nn[k][jj];
+                        const float ang = dm::acosf_(clamp11(dot3(r.axis, nv)));
+                        r.half_angle = smax(r.half_angle, ang);
+                    }
+                }
+            }
+        }
+        leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, b.lo.z, 0.0f);
+        leafbox[2 * i + 1] = make_float4(b.hi.x, b.hi.y, b.hi.z, 0.0f);
+        leafcone[2 * i] = make_float4(r.axis.x, r.axis.y, r.axis.z, r.half_angle);
+        leafcone[2 * i + 1] = make_float4(r.radius, 0.0f, 0.0f, 0.0f);
+    }
+    // scene AABB: wave64 butterfly, then LDS across the block's 4 waves
+    float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            v[k] = fmn(v[k], __shfl_xor(v[k], o, 64));
+#pragma unroll
+        for (int k = 3; k < 6; ++k)
+            v[k] = fmx(v[k], __shfl_xor(v[k], o, 64));
+    }
+    __shared__ float red[BT / 64][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            red[w][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float acc = red[0][k];
+        for (int q = 1; q < BT / 64; ++q)
+            acc = k < 3 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
+        if (k < 3)
+            atomicMin(scene_bits + k, f2ord(acc));
+        else
+            atomicMax(scene_bits + k, f2ord(acc));
+    }
+}
+
+// bvh.cuh:292-302 and morton_code.cuh:61-70
+__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ scene_bits,
+                                               uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n)
+        return;
+    const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]), loz = ord2f(scene_bits[2]);
+    const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]), hiz = ord2f(scene_bits[5]);
+    const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
+    box3 b;
+    b.lo = mk(a.x, a.y, a.z);
+    b.hi = mk(c.x, c.y, c.z);
+    v3 p = box_centroid(b);
+    p.x -= lox;
+    p.y -= loy;
+    p.z -= loz;
+    p.x /= (hix - lox);
+    p.y /= (hiy - loy);
+    p.z /= (hiz - loz);
+    codes[i] = morton3(p);
+    idx[i] = i;
+}
+
+// delta of the virtual 64-bit key (code << 32 | idx)
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
+                                      uint32_t a, uint32_t b)
+{
+    const uint32_t ca = codes[a], cb = codes[b];
+    if (ca != cb)
+        return __clz(ca ^ cb);
+    return 32 + __clz(idx[a] ^ idx[b]);
+}
+
+// bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
+__global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
+                                               uint32_t N, uint32_t* __restrict__ parent, uint2* __restrict__ child,
+                                               uint32_t* __restrict__ flags)
+{
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    if (i >= N - 1)
+        return;
+    uint32_t first, last;
+    if (i == 0) {
+        first = 0;
+        last = N - 1;
+        parent[0] = INVALID;
+    } else {
+        const int L = kdelta(codes, idx, i, i - 1);
+        const int R = kdelta(codes, idx, i, i + 1);
+        const int d = (R > L) ? 1 : -1;
+        const int dmin = L < R ? L : R;
+        int lmax = 2;
+        int itmp = (int)i + d * lmax;
+        int delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+        while (delta > dmin) {
+            lmax <<= 1;
+            itmp = (int)i + d * lmax;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+        }
+        int l = 0;
+        for (int t = lmax >> 1; t > 0; t >>= 1) {
+            itmp = (int)i + (l + t) * d;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+            if (delta > dmin)
+                l += t;
+        }
+        const uint32_t j = (uint32_t)((int)i + l * d);
+        first = d < 0 ? j : i;
+        last = d < 0 ? i : j;
+    }
+    // find_split; first != last always holds for internal nodes and the
+    // virtual 64-bit keys are unique, so the equal-key branch of the
+    // reference (bvh.cuh:175-178) cannot trigger
+    const int dnode = kdelta(codes, idx, first, last);
+    int split = (int)first;
+    int stride = (int)(last - first);
+    do {
+        stride = (stride + 1) >> 1;
+        const int mid = split + stride;
+        if (mid < (int)last) {
+            if (kdelta(codes, idx, first, (uint32_t)mid) > dnode)
+                split = mid;
+        }
+    } while (stride > 1);
+    const uint32_t g = (uint32_t)split;
+    uint32_t left = g, right = g + 1;
+    if (first == g)
+        left += N - 1;
+    if (last == g + 1)
+        right += N - 1;
+    child[i] = make_uint2(left, right);
+    parent[left] = i;
+    parent[right] = i;
+    flags[i] = 0u;
+}
+
+__device__ __forceinline__ box3 ldbox(const float4* nb, uint32_t n)
+{
+    const float4 a = nb[2 * n], c = nb[2 * n + 1];
+    box3 b;
+    b.lo = mk(a.x, a.y, a.z);
+    b.hi = mk(c.x, c.y, c.z);
+    return b;
+}
+__device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
+{
+    const float4 a = nc[2 * n], c = nc[2 * n + 1];
+    cone3 r;
+    r.axis = mk(a.x, a.y, a.z);
+    r.half_angle = a.w;
+    r.radius = c.x;
+    return r;
+}
+
+__global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
+                                              const uint32_t* __restrict__ estart,
+                                              const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
+                                              const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
+                                              const uint2* __restrict__ child, uint32_t* flags, uint32_t N,
+                                              int conservative, float4* nbox, float4* ncone,
+                                              float4* __restrict__ tnode, float4* __restrict__ tcone,
+                                              float4* __restrict__ tleaf)
+{
+    const uint32_t j = blockIdx.x * BT + threadIdx.x;
+    if (j >= N)
+        return;
+    const uint32_t orig = sidx[j];
+    if (orig >= N)
+        return; // not a permutation: leave the node unwritten rather than read out of bounds
+    const uint32_t node = N - 1 + j;
+    nbox[2 * node] = leafbox[2 * orig];
+    nbox[2 * node + 1] = leafbox[2 * orig + 1];
+    ncone[2 * node] = leafcone[2 * orig];
+    ncone[2 * node + 1] = leafcone[2 * orig + 1];
+    {
+        const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], c = V[F[3 * orig + 2]];
+        const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
+        tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
+        tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
+        tleaf[3 * j + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(ec));
+    }
+    uint32_t p = parent[node];
+    // the climb is at most the tree depth (<= 64 key bits); the cap only
+    // guards against a malformed parent array
+    for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
+        const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u)
+            return; // the sibling subtree is not finished; its last thread continues
+        const uint2 ch = child[p];
+        const box3 lb = ldbox(nbox, ch.x), rb = ldbox(nbox, ch.y);
+        const cone3 lc = ldcone(ncone, ch.x), rc = ldcone(ncone, ch.y);
+        const box3 pb = box_merge(lb, rb);
+        const cone3 pc = cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0);
+        nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
+        nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
+        ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
+        ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
+        const uint32_t lref = ch.x < N - 1 ? ch.x : (LEAF_BIT | (ch.x - (N - 1)));
+        const uint32_t rref = ch.y < N - 1 ? ch.y : (LEAF_BIT | (ch.y - (N - 1)));
+        tnode[4 * p + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
+        tnode[4 * p + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
+        tnode[4 * p + 2] = make_float4(rb.lo.z, rb.hi.x, rb.hi.y, rb.hi.z);
+        tnode[4 * p + 3] = make_float4(__uint_as_float(lref), __uint_as_float(rref), 0.0f, 0.0f);
+        tcone[3 * p + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
+        tcone[3 * p + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
+        tcone[3 * p + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+        p = parent[p];
+    }
+}
+
+__global__ void k_single_root(uint32_t* parent) { parent[0] = INVALID; }
+
+inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
+
+} // namespace
+
+hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st)
+{
+    const uint32_t N = s->nF, NN = 2 * N - 1, E = s->nE;
+    const int conservative = (flags & SNCH_BUILD_REFERENCE_RADIUS) ? 0 : 1;
+    BuildScratch& w = s->scratch;
+    hipError_t e;
+#define SNCH_TRY(x)              \
+    do {                         \
+        if ((e = (x)) != hipSuccess) \
+            return e;            \
+    } while (0)
+    SNCH_TRY(w.leafbox.ensure(sizeof(float4) * 2 * N));
+    SNCH_TRY(w.leafcone.ensure(sizeof(float4) * 2 * N));
+    SNCH_TRY(w.scene_bits.ensure(sizeof(int) * 8));
+    SNCH_TRY(w.codes.ensure(sizeof(uint32_t) * N));
+    SNCH_TRY(w.idx.ensure(sizeof(uint32_t) * N));
+    SNCH_TRY(w.codes_alt.ensure(sizeof(uint32_t) * N));
+    SNCH_TRY(w.idx_alt.ensure(sizeof(uint32_t) * N));
+    SNCH_TRY(w.parent.ensure(sizeof(uint32_t) * NN));
+    SNCH_TRY(w.child.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
+    SNCH_TRY(s->tedge.ensure(sizeof(float4) * 4 * (E > 0 ? E : 1)));
+    SNCH_TRY(s->nbox.ensure(sizeof(float4) * 2 * NN));
+    SNCH_TRY(s->ncone.ensure(sizeof(float4) * 2 * NN));
+
+    hipLaunchKernelGGL(k_init_scene_box, dim3(1), dim3(64), 0, st, w.scene_bits.as<int>());
+    hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                       s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
+                       w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
+    hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(), w.scene_bits.as<int>(),
+                       N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+    bool in_alt = false;
+    SNCH_TRY(radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
+                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt));
+    const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
+    const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
+    if (N > 1)
+        hipLaunchKernelGGL(k_karras, dim3(blocks(N - 1)), dim3(BT), 0, st, codes, sidx, N, w.parent.as<uint32_t>(),
+                           w.child.as<uint2>(), w.flags.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>());
+    hipLaunchKernelGGL(k_refit, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                       s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
+                       w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
+                       s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(),
+                       s->tleaf.as<float4>());
+    s->sorted_in_alt = in_alt;
+#undef SNCH_TRY
+    return hipGetLastError();
+}
+
+} // namespace snch

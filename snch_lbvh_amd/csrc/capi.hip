@@ -1,0 +1,506 @@
+// C ABI of libsnch (include/snch.h): handles, validation, the host-side edge
+// table, device uploads and the query entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace snch {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return SNCH_ERR_HIP;
+}
+
+DeviceBuffer::~DeviceBuffer()
+{
+    if (p)
+        (void)hipFree(p);
+}
+
+hipError_t DeviceBuffer::ensure(size_t n)
+{
+    if (n <= bytes && p)
+        return hipSuccess;
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n > 0 ? n : 16);
+    if (e == hipSuccess)
+        bytes = n;
+    return e;
+}
+
+// one error word per device for traversal stack overflow (never expected:
+// the tree depth is bounded by the 64 key bits)
+uint32_t* traversal_error_word()
+{
+    static std::mutex mu;
+    static std::vector<uint32_t*> words;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)words.size() <= dev)
+        words.resize(dev + 1, nullptr);
+    if (!words[dev]) {
+        void* p = nullptr;
+        if (hipMalloc(&p, 64) == hipSuccess) {
+            (void)hipMemset(p, 0, 64);
+            words[dev] = static_cast<uint32_t*>(p);
+        }
+    }
+    return words[dev];
+}
+
+// Edge table of scene<3> (scene.cuh:1135-1225): edge id = first appearance
+// over (triangle i, local edge j = (v_j, v_{j+1})); slots (opp0, lo, hi,
+// opp1); each edge owned by the first triangle listing it.  Computed by
+// sorting (lo, hi, position) instead of a std::map; the owned edges of a
+// triangle are then a run of consecutive ids starting at estart[i].
+static void edge_table_3d(snch_scene* s)
+{
+    const uint32_t nF = s->nF;
+    const size_t n = 3 * (size_t)nF;
+    std::vector<uint64_t> key(n);
+    std::vector<uint32_t> order(n);
+    const int32_t* F = s->F.data();
+    for (size_t p = 0; p < n; ++p) {
+        const size_t i = p / 3, j = p % 3;
+        uint32_t I = (uint32_t)F[3 * i + j], J = (uint32_t)F[3 * i + (j + 1) % 3];
+        if (I > J)
+            std::swap(I, J);
+        key[p] = ((uint64_t)I << 32) | J;
+    }
+    std::iota(order.begin(), order.end(), 0u);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return key[a] != key[b] ? key[a] < key[b] : a < b;
+    });
+    // group -> first position
+    std::vector<uint32_t> group_of(n), first_pos;
+    first_pos.reserve(n / 2 + 1);
+    for (size_t k = 0; k < n; ++k) {
+        if (k == 0 || key[order[k]] != key[order[k - 1]])
+            first_pos.push_back(order[k]);
+        group_of[order[k]] = (uint32_t)(first_pos.size() - 1);
+    }
+    const uint32_t E = (uint32_t)first_pos.size();
+    // ids in order of first appearance: a position is the first of its group
+    // iff first_pos[group] == position; scanning positions in order numbers them
+    std::vector<uint32_t> gid(E, 0xFFFFFFFFu);
+    std::vector<int32_t> tri_edge(n);
+    uint32_t next = 0;
+    s->owned.assign(3 * (size_t)nF, -1);
+    s->estart.assign((size_t)nF + 1, 0);
+    for (size_t p = 0; p < n; ++p) {
+        const uint32_t g = group_of[p];
+        if (first_pos[g] == p) {
+            gid[g] = next++;
+            const size_t i = p / 3;
+            int k = 0;
+            while (s->owned[3 * i + k] != -1)
+                ++k;
+            s->owned[3 * i + k] = (int32_t)gid[g];
+        }
+        tri_edge[p] = (int32_t)gid[g];
+        if (p % 3 == 2)
+            s->estart[p / 3 + 1] = next;
+    }
+    s->nE = E;
+    s->slots.assign(4 * (size_t)E, -1);
+    for (uint32_t i = 0; i < nF; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            int I = j - 1 < 0 ? 2 : j - 1, J = j, K = j + 1 > 2 ? 0 : j + 1, orientation = 1;
+            const int32_t e = tri_edge[3 * (size_t)i + j];
+            if (F[3 * i + J] > F[3 * i + K]) {
+                std::swap(J, K);
+                orientation = -1;
+            }
+            int32_t* sl = &s->slots[4 * (size_t)e];
+            sl[orientation == 1 ? 0 : 3] = F[3 * i + I];
+            sl[1] = F[3 * i + J];
+            sl[2] = F[3 * i + K];
+        }
+    }
+}
+
+} // namespace snch
+
+using namespace snch;
+
+snch::TreeView snch_scene::view() const
+{
+    TreeView t{};
+    t.n_leaves = nF;
+    t.tnode = tnode.as<float4>();
+    t.tcone = tcone.as<float4>();
+    t.tleaf = tleaf.as<float4>();
+    t.tedge = tedge.as<float4>();
+    t.nbox = nbox.as<float4>();
+    t.ncone = ncone.as<float4>();
+    return t;
+}
+
+#define SNCH_CHECK_ARG(cond, msg)   \
+    do {                            \
+        if (!(cond)) {              \
+            set_error(msg);         \
+            return SNCH_ERR_INVALID; \
+        }                           \
+    } while (0)
+
+#define SNCH_HIP(call, what)                \
+    do {                                    \
+        hipError_t e_ = (call);             \
+        if (e_ != hipSuccess)               \
+            return hip_fail(e_, what);      \
+    } while (0)
+
+extern "C" {
+
+const char* snch_last_error(void) { return g_last_error.c_str(); }
+
+int snch_version(void) { return 10000; }
+
+int snch_context_create(int device, snch_context** out)
+{
+    SNCH_CHECK_ARG(out, "snch_context_create: out is NULL");
+    int count = 0;
+    SNCH_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    SNCH_CHECK_ARG(device >= 0 && device < count, "snch_context_create: no such device");
+    SNCH_HIP(hipSetDevice(device), "hipSetDevice");
+    auto* c = new snch_context();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+        c->n_cu = prop.multiProcessorCount;
+    *out = c;
+    return SNCH_OK;
+}
+
+int snch_context_destroy(snch_context* ctx)
+{
+    delete ctx;
+    return SNCH_OK;
+}
+
+int snch_scene_create_tri(snch_context* ctx, const float* V, uint32_t nV, const int32_t* F, uint32_t nF,
+                          snch_scene** out)
+{
+    SNCH_CHECK_ARG(ctx && out, "snch_scene_create_tri: null context or output");
+    *out = nullptr;
+    if (nF == 0) {
+        set_error("snch_scene_create_tri: empty mesh");
+        return SNCH_ERR_EMPTY;
+    }
+    SNCH_CHECK_ARG(V && F && nV > 0, "snch_scene_create_tri: null vertex or index array");
+    SNCH_CHECK_ARG(nF < 0x40000000u, "snch_scene_create_tri: too many triangles (limit 2^30)");
+    for (size_t k = 0; k < 3 * (size_t)nV; ++k)
+        SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_tri: non-finite vertex coordinate");
+    for (size_t k = 0; k < 3 * (size_t)nF; ++k)
+        SNCH_CHECK_ARG(F[k] >= 0 && (uint32_t)F[k] < nV, "snch_scene_create_tri: vertex index out of range");
+    SNCH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto* s = new snch_scene();
+    s->ctx = ctx;
+    s->dim = 3;
+    s->nV = nV;
+    s->nF = nF;
+    s->V.assign(V, V + 3 * (size_t)nV);
+    s->F.assign(F, F + 3 * (size_t)nF);
+    edge_table_3d(s);
+    std::vector<float> V4(4 * (size_t)nV, 0.0f);
+    for (size_t k = 0; k < nV; ++k) {
+        V4[4 * k] = V[3 * k];
+        V4[4 * k + 1] = V[3 * k + 1];
+        V4[4 * k + 2] = V[3 * k + 2];
+    }
+    hipError_t e;
+    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dF.ensure(sizeof(int32_t) * s->F.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dF.p, s->F.data(), sizeof(int32_t) * s->F.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(s->slots.size(), 4))) != hipSuccess ||
+        (s->slots.size() &&
+         (e = hipMemcpy(s->dSlots.p, s->slots.data(), sizeof(int32_t) * s->slots.size(), hipMemcpyHostToDevice)) !=
+             hipSuccess) ||
+        (e = s->dEstart.ensure(sizeof(uint32_t) * s->estart.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dEstart.p, s->estart.data(), sizeof(uint32_t) * s->estart.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        delete s;
+        return hip_fail(e, "snch_scene_create_tri: upload");
+    }
+    *out = s;
+    return SNCH_OK;
+}
+
+int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const int32_t* S, uint32_t nS,
+                          snch_scene** out)
+{
+    (void)ctx;
+    (void)V;
+    (void)nV;
+    (void)S;
+    (void)nS;
+    if (out)
+        *out = nullptr;
+    set_error("snch_scene_create_seg: 2D scenes are not available in this build");
+    return SNCH_ERR_INVALID;
+}
+
+int snch_scene_destroy(snch_scene* scene)
+{
+    if (scene) {
+        (void)hipSetDevice(scene->ctx->device);
+        delete scene;
+    }
+    return SNCH_OK;
+}
+
+int snch_scene_info(const snch_scene* s, uint32_t* dim, uint32_t* n_prims, uint32_t* n_sil)
+{
+    SNCH_CHECK_ARG(s, "snch_scene_info: null scene");
+    if (dim)
+        *dim = s->dim;
+    if (n_prims)
+        *n_prims = s->nF;
+    if (n_sil)
+        *n_sil = s->nE;
+    return SNCH_OK;
+}
+
+int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
+{
+    SNCH_CHECK_ARG(s, "snch_scene_build: null scene");
+    const uint32_t flags = opts ? opts->flags : 0u;
+    const uint32_t leaf = opts && opts->leaf_size ? opts->leaf_size : 1u;
+    SNCH_CHECK_ARG(leaf == 1u, "snch_scene_build: leaf_size other than 1 is not available in this build");
+    SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
+    SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    (void)hipEventRecord(e0, st);
+    hipError_t e = build_tree_3d(s, flags, st);
+    (void)hipEventRecord(e1, st);
+    hipError_t es = hipStreamSynchronize(st);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess)
+        return hip_fail(e, "snch_scene_build");
+    if (es != hipSuccess)
+        return hip_fail(es, "snch_scene_build: synchronize");
+    s->built = true;
+    s->build_flags = flags;
+    s->last_build_ms = ms;
+    return SNCH_OK;
+}
+
+int snch_scene_build_time(const snch_scene* s, float* ms)
+{
+    SNCH_CHECK_ARG(s && ms, "snch_scene_build_time: null argument");
+    *ms = s->last_build_ms;
+    return SNCH_OK;
+}
+
+#define SNCH_NEED_BUILT(s, fn)                                   \
+    do {                                                         \
+        SNCH_CHECK_ARG(s, fn ": null scene");                    \
+        if (!(s)->built) {                                       \
+            set_error(fn ": BVH is not built yet.");             \
+            return SNCH_ERR_STATE;                               \
+        }                                                        \
+    } while (0)
+
+int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_idx, float* out_dist,
+                          const snch_query_opts* opts, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_closest_point");
+    SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_closest_point: null pointer");
+    SNCH_HIP(launch_closest_point(s->view(), q, n, out_idx, out_dist, opts ? opts->stats : nullptr,
+                                  static_cast<hipStream_t>(stream)),
+             "snch_closest_point");
+    return SNCH_OK;
+}
+
+int snch_closest_point(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_idx, float* out_dist,
+                       void* stream)
+{
+    return snch_closest_point_ex(s, q, n, out_idx, out_dist, nullptr, stream);
+}
+
+int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, int flip, const float* seed,
+                               uint32_t* out_sil, float* out_dist, const snch_query_opts* opts, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_closest_silhouette");
+    SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
+    uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
+    SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
+                   "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
+    SNCH_HIP(launch_closest_silhouette(s->view(), q, n, flip, seed, out_sil, out_dist, block,
+                                       opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
+             "snch_closest_silhouette");
+    return SNCH_OK;
+}
+
+int snch_closest_silhouette(const snch_scene* s, const float* q, uint32_t n, int flip, const float* seed,
+                            uint32_t* out_sil, float* out_dist, void* stream)
+{
+    return snch_closest_silhouette_ex(s, q, n, flip, seed, out_sil, out_dist, nullptr, stream);
+}
+
+int snch_ray_ex(const snch_scene* s, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
+                uint8_t* out_hit, float* out_t, float* out_uv, uint32_t* out_idx, const snch_query_opts* opts,
+                void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_ray");
+    SNCH_CHECK_ARG(n == 0 || (o && d && out_hit), "snch_ray: null pointer");
+    SNCH_HIP(launch_ray(s->view(), o, d, tmax, n, any_hit, out_hit, out_t, out_uv, out_idx,
+                        opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
+             "snch_ray");
+    return SNCH_OK;
+}
+
+int snch_ray(const snch_scene* s, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
+             uint8_t* out_hit, float* out_t, float* out_uv, uint32_t* out_idx, void* stream)
+{
+    return snch_ray_ex(s, o, d, tmax, n, any_hit, out_hit, out_t, out_uv, out_idx, nullptr, stream);
+}
+
+int snch_sample_in_sphere(const snch_scene* s, const float* c, const float* r, const float* u, uint32_t n,
+                          int32_t* out_idx, float* out_pdf, float* out_pt, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_sample_in_sphere");
+    SNCH_CHECK_ARG(n == 0 || (c && r && u && out_idx && out_pdf), "snch_sample_in_sphere: null pointer");
+    SNCH_HIP(launch_sample(s->view(), s->dV.as<float4>(), s->dF.as<int32_t>(), c, r, u, n, out_idx, out_pdf, out_pt,
+                           static_cast<hipStream_t>(stream)),
+             "snch_sample_in_sphere");
+    return SNCH_OK;
+}
+
+int snch_sample_on_object(const snch_scene* s, const int32_t* idx, const float* uv, uint32_t n, float* out_pt,
+                          void* stream)
+{
+    SNCH_CHECK_ARG(s, "snch_sample_on_object: null scene");
+    SNCH_CHECK_ARG(n == 0 || (idx && uv && out_pt), "snch_sample_on_object: null pointer");
+    SNCH_HIP(launch_sample_on_object(s->dV.as<float4>(), s->dF.as<int32_t>(), s->nF, idx, uv, n, out_pt,
+                                     static_cast<hipStream_t>(stream)),
+             "snch_sample_on_object");
+    return SNCH_OK;
+}
+
+int snch_scene_export(const snch_scene* s, uint32_t* nodes, float* boxes, float* cones, uint32_t* sorted_codes,
+                      uint32_t* sorted_idx)
+{
+    SNCH_NEED_BUILT(s, "snch_scene_export");
+    SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    const uint32_t N = s->nF, NN = 2 * N - 1;
+    const BuildScratch& w = s->scratch;
+    const uint32_t* dcodes = s->sorted_in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
+    const uint32_t* didx = s->sorted_in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
+    std::vector<uint32_t> sidx(N), parent(NN), child(2 * (size_t)(N > 1 ? N - 1 : 1));
+    SNCH_HIP(hipMemcpy(sidx.data(), didx, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
+    if (sorted_codes)
+        SNCH_HIP(hipMemcpy(sorted_codes, dcodes, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
+    if (sorted_idx)
+        std::memcpy(sorted_idx, sidx.data(), sizeof(uint32_t) * N);
+    if (nodes) {
+        SNCH_HIP(hipMemcpy(parent.data(), w.parent.p, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost), "export");
+        if (N > 1)
+            SNCH_HIP(hipMemcpy(child.data(), w.child.p, sizeof(uint32_t) * 2 * (N - 1), hipMemcpyDeviceToHost),
+                     "export");
+        for (uint32_t i = 0; i < NN; ++i) {
+            nodes[4 * i] = parent[i];
+            if (i + 1 < N) {
+                nodes[4 * i + 1] = child[2 * i];
+                nodes[4 * i + 2] = child[2 * i + 1];
+                nodes[4 * i + 3] = INVALID;
+            } else {
+                nodes[4 * i + 1] = INVALID;
+                nodes[4 * i + 2] = INVALID;
+                nodes[4 * i + 3] = sidx[i - (N - 1)];
+            }
+        }
+    }
+    if (boxes || cones) {
+        std::vector<float> b(8 * (size_t)NN), c(8 * (size_t)NN);
+        SNCH_HIP(hipMemcpy(b.data(), s->nbox.p, sizeof(float) * b.size(), hipMemcpyDeviceToHost), "export");
+        SNCH_HIP(hipMemcpy(c.data(), s->ncone.p, sizeof(float) * c.size(), hipMemcpyDeviceToHost), "export");
+        for (uint32_t i = 0; i < NN; ++i) {
+            if (boxes) {
+                boxes[6 * i + 0] = b[8 * i + 0];
+                boxes[6 * i + 1] = b[8 * i + 1];
+                boxes[6 * i + 2] = b[8 * i + 2];
+                boxes[6 * i + 3] = b[8 * i + 4];
+                boxes[6 * i + 4] = b[8 * i + 5];
+                boxes[6 * i + 5] = b[8 * i + 6];
+            }
+            if (cones) {
+                cones[5 * i + 0] = c[8 * i + 0];
+                cones[5 * i + 1] = c[8 * i + 1];
+                cones[5 * i + 2] = c[8 * i + 2];
+                cones[5 * i + 3] = c[8 * i + 3];
+                cones[5 * i + 4] = c[8 * i + 4];
+            }
+        }
+    }
+    return SNCH_OK;
+}
+
+int snch_scene_silhouettes(const snch_scene* s, int32_t* slots, int32_t* owned)
+{
+    SNCH_CHECK_ARG(s, "snch_scene_silhouettes: null scene");
+    if (slots && !s->slots.empty())
+        std::memcpy(slots, s->slots.data(), sizeof(int32_t) * s->slots.size());
+    if (owned)
+        std::memcpy(owned, s->owned.data(), sizeof(int32_t) * s->owned.size());
+    return SNCH_OK;
+}
+
+int snch_device_malloc(void** ptr, uint64_t bytes)
+{
+    SNCH_CHECK_ARG(ptr, "snch_device_malloc: null pointer");
+    SNCH_HIP(hipMalloc(ptr, bytes ? bytes : 16), "hipMalloc");
+    return SNCH_OK;
+}
+
+int snch_device_free(void* ptr)
+{
+    SNCH_HIP(hipFree(ptr), "hipFree");
+    return SNCH_OK;
+}
+
+int snch_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    SNCH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)), "memcpy h2d");
+    return SNCH_OK;
+}
+
+int snch_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    SNCH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)), "memcpy d2h");
+    SNCH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "synchronize");
+    return SNCH_OK;
+}
+
+int snch_stream_synchronize(void* stream)
+{
+    SNCH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return SNCH_OK;
+}
+
+} // extern "C"

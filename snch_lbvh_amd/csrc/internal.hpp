@@ -1,0 +1,116 @@
+// Internal types of libsnch: device tree layout, scene state, launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/snch.h"
+#include "geom.hpp"
+
+namespace snch {
+
+// ---------------------------------------------------------------------------
+// HBM layout of a built tree (N leaves, N-1 internal nodes, E silhouette edges)
+//
+//   tnode[i]  4 x float4 = 64 B per internal node i (64-B aligned): the boxes
+//             of BOTH children plus their references, so one visit is one
+//             64-B load:
+//               [0] L.lo.x L.lo.y L.lo.z L.hi.x
+//               [1] L.hi.y L.hi.z R.lo.x R.lo.y
+//               [2] R.lo.z R.hi.x R.hi.y R.hi.z
+//               [3] ref(L) ref(R) 0      0
+//             ref(c) = c for an internal child, LEAF_BIT | j for the leaf in
+//             sorted position j.
+//   tcone[i]  3 x float4 = 48 B per internal node: the children's normal
+//             cones, read only by the silhouette query:
+//               [0] L.axis L.half_angle  [1] R.axis R.half_angle
+//               [2] L.radius R.radius 0 0
+//   tleaf[j]  3 x float4 = 48 B per leaf, in Morton (sorted) order:
+//               [k] vertex k xyz, w = (original triangle, first owned edge,
+//               owned edge count) for k = 0, 1, 2
+//   tedge[e]  4 x float4 = 64 B per silhouette edge, in edge-id order (the
+//             edges owned by one triangle have consecutive ids):
+//               [0] pa xyz, edge id  [1] pb xyz, flags (bit0 has face 0,
+//               bit1 has face 1)  [2] n0 xyz  [3] n1 xyz (unit face normals)
+// The reference layout (node{parent,left,right,object}, aabb, cone per node;
+// bvh.cuh:27-33) is kept alongside for export and for the refit's own use.
+// ---------------------------------------------------------------------------
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+
+struct TreeView {
+    uint32_t n_leaves;
+    uint32_t pad_;
+    const float4* __restrict__ tnode;
+    const float4* __restrict__ tcone;
+    const float4* __restrict__ tleaf;
+    const float4* __restrict__ tedge;
+    const float4* __restrict__ nbox;  // own box per node, reference numbering (root = 0)
+    const float4* __restrict__ ncone; // own cone per node, reference numbering
+};
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DeviceBuffer();
+    hipError_t ensure(size_t n);
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct BuildScratch {
+    DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, flags,
+        status;
+};
+
+} // namespace snch
+
+struct snch_context {
+    int device = 0;
+    int n_cu = 0;
+};
+
+struct snch_scene {
+    snch_context* ctx = nullptr;
+    uint32_t dim = 3;
+    uint32_t nV = 0, nF = 0, nE = 0;
+    // host copies of the inputs and of the silhouette table
+    std::vector<float> V;
+    std::vector<int32_t> F;
+    std::vector<int32_t> slots;  // 4 per silhouette
+    std::vector<int32_t> owned;  // 3 per primitive
+    std::vector<uint32_t> estart; // first owned silhouette id per primitive (nF + 1)
+    // device inputs
+    snch::DeviceBuffer dV, dF, dSlots, dEstart;
+    // device tree
+    snch::DeviceBuffer tnode, tcone, tleaf, tedge, nbox, ncone;
+    snch::BuildScratch scratch;
+    bool built = false;
+    bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
+    uint32_t build_flags = 0;
+    float last_build_ms = 0.0f;
+    snch::TreeView view() const;
+};
+
+namespace snch {
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// build.hip
+hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st);
+// sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                            int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt);
+// query.hip
+hipError_t launch_closest_point(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
+                                uint64_t* stats, hipStream_t st);
+hipError_t launch_closest_silhouette(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
+                                     uint32_t* sil, float* dist, uint32_t block, uint64_t* stats, hipStream_t st);
+hipError_t launch_ray(const TreeView& t, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
+                      uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats, hipStream_t st);
+hipError_t launch_sample(const TreeView& t, const float4* V, const int32_t* F, const float* c, const float* r,
+                         const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt, hipStream_t st);
+hipError_t launch_sample_on_object(const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
+                                   const float* uv, uint32_t n, float* pt, hipStream_t st);
+uint32_t* traversal_error_word();
+} // namespace snch

@@ -1,0 +1,647 @@
+// Query kernels for triangle scenes on gfx950: closest primitive, closest
+// silhouette, ray (closest hit / any hit) and SampleTriangleInSphere.
+//
+// One lane = one query (the reference: one thread per query through
+// query_device, query.cuh:79-423, and sample_object_in_sphere,
+// sample.cuh:23-92).  Differences in HOW, not WHAT:
+//  * Node layout: one 64-B record per internal node holds both children's
+//    boxes and references; the silhouette query reads a 48-B cone record from
+//    a separate stream, so the closest-point and ray queries never touch it.
+//  * Stack: a short per-lane stack in LDS (16 entries of {node, key}, column
+//    per lane so a wave's 64 lanes hit distinct banks) backed by a private
+//    overflow array; the tree depth is at most 64, so the stack never loses
+//    an entry.
+//  * Order: the nearer child is descended first and the farther is pushed,
+//    and a child is skipped when its lower bound exceeds the best answer so
+//    far.  The reference's own pruning predicates (minmaxdist for closest
+//    point, cone overlap for silhouettes, slab tests for rays) are evaluated
+//    unchanged; the added bound checks are conservative.
+//  * Ties: among candidates at exactly the same distance the smallest index
+//    wins (the reference keeps the last visited one for closest point and
+//    silhouette, query.cuh:286/:383, and the first for rays, :115).  The
+//    answer is then a function of the candidate set alone, independent of
+//    visit order, block size or a seeded search radius.
+//  * Blocks are remapped so that each of the 8 XCDs processes one contiguous
+//    range of queries (blocks are dispatched round-robin over XCDs).
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace snch {
+
+namespace {
+
+constexpr int SS = 16;  // LDS stack entries per lane
+constexpr int OVF = 64; // private overflow entries per lane
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    const uint32_t X = 8, q = nb / X, r = nb % X, x = b % X, k = b / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
+template <int BS>
+struct LaneStack {
+    uint2* lds; // this lane's column: entry s at lds[s * BS]
+    int sp;
+    uint32_t ovn[OVF];
+    float ovd[OVF];
+    bool overflow;
+
+    __device__ __forceinline__ void init(uint2* base)
+    {
+        lds = base;
+        sp = 0;
+        overflow = false;
+    }
+    __device__ __forceinline__ void push(uint32_t n, float d)
+    {
+        const int s = sp & (SS - 1);
+        if (sp >= SS) {
+            const int o = sp - SS;
+            if (o < OVF) {
+                const uint2 old = lds[s * BS];
+                ovn[o] = old.x;
+                ovd[o] = __uint_as_float(old.y);
+            } else {
+                overflow = true;
+                return;
+            }
+        }
+        lds[s * BS] = make_uint2(n, __float_as_uint(d));
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(uint32_t& n, float& d)
+    {
+        --sp;
+        const int s = sp & (SS - 1);
+        const uint2 e = lds[s * BS];
+        n = e.x;
+        d = __uint_as_float(e.y);
+        if (sp >= SS) {
+            const int o = sp - SS;
+            lds[s * BS] = make_uint2(ovn[o], __float_as_uint(ovd[o]));
+        }
+    }
+};
+
+// per-query visit counters, summed over the wave before one atomic per wave
+struct Counts {
+    uint32_t box = 0, leaf = 0, edge = 0, node = 0;
+};
+__device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long* stats)
+{
+    unsigned long long v[4] = {c.box, c.leaf, c.edge, c.node};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            v[k] += __shfl_xor(v[k], o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            atomicAdd(stats + k, v[k]);
+    }
+}
+
+struct NodeRec {
+    box3 L, R;
+    uint32_t lref, rref;
+};
+
+__device__ __forceinline__ NodeRec load_node(const float4* __restrict__ tnode, uint32_t i)
+{
+    const float4 a = tnode[4 * i], b = tnode[4 * i + 1], c = tnode[4 * i + 2], r = tnode[4 * i + 3];
+    NodeRec n;
+    n.L.lo = mk(a.x, a.y, a.z);
+    n.L.hi = mk(a.w, b.x, b.y);
+    n.R.lo = mk(b.z, b.w, c.x);
+    n.R.hi = mk(c.y, c.z, c.w);
+    n.lref = __float_as_uint(r.x);
+    n.rref = __float_as_uint(r.y);
+    return n;
+}
+
+__device__ __forceinline__ box3 ldbox2(const float4* nb, uint32_t n)
+{
+    const float4 a = nb[2 * n], c = nb[2 * n + 1];
+    box3 b;
+    b.lo = mk(a.x, a.y, a.z);
+    b.hi = mk(c.x, c.y, c.z);
+    return b;
+}
+
+// ------------------------------------------------------------ closest point
+__device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, float& best, uint32_t& best_idx)
+{
+    if (j >= t.n_leaves)
+        return;
+    const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], c = t.tleaf[3 * j + 2];
+    float d = closest_point_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), p);
+    d *= d; // query.cuh:284-285
+    const uint32_t o = __float_as_uint(a.w);
+    if (d < best || (d == best && o < best_idx)) {
+        best = d;
+        best_idx = o;
+    }
+}
+
+template <int BS, bool STATS>
+__global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* __restrict__ q, uint32_t n,
+                                                      uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
+                                                      uint32_t* __restrict__ err, unsigned long long* stats)
+{
+    __shared__ uint2 stk[SS * BS];
+    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    Counts cnt;
+    if (qi >= n) {
+        if (STATS)
+            flush_counts(cnt, stats);
+        return;
+    }
+    const v3 p = mk(q[3 * qi], q[3 * qi + 1], q[3 * qi + 2]);
+    float best = F_INF;
+    uint32_t best_idx = INVALID;
+    if (t.n_leaves == 1) {
+        cp_leaf(t, 0, p, best, best_idx);
+    } else {
+        LaneStack<BS> st;
+        st.init(stk + threadIdx.x);
+        uint32_t node = 0;
+        // each internal node is visited at most once; the cap and the range
+        // checks only guard against a malformed tree
+        const uint32_t cap_iters = 2 * t.n_leaves + 64;
+        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+            const NodeRec nr = load_node(t.tnode, node);
+            const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
+            const float Lmm = box_minmaxdist(nr.L, p), Rmm = box_minmaxdist(nr.R, p);
+            const bool cL = Lmd <= Rmm, cR = Rmd <= Lmm; // query.cuh:279, :297
+            const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
+            if (STATS) {
+                cnt.node += 1;
+                cnt.box += 2;
+                cnt.leaf += (cL && lLeaf && Lmd <= best) + (cR && rLeaf && Rmd <= best);
+            }
+            if (cL && lLeaf && Lmd <= best)
+                cp_leaf(t, nr.lref & ~LEAF_BIT, p, best, best_idx);
+            if (cR && rLeaf && Rmd <= best)
+                cp_leaf(t, nr.rref & ~LEAF_BIT, p, best, best_idx);
+            const bool gL = cL && !lLeaf && Lmd <= best;
+            const bool gR = cR && !rLeaf && Rmd <= best;
+            if (gL && gR) {
+                const bool lfirst = Lmd <= Rmd;
+                st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
+                node = lfirst ? nr.lref : nr.rref;
+                continue;
+            }
+            if (gL || gR) {
+                node = gL ? nr.lref : nr.rref;
+                continue;
+            }
+            bool found = false;
+            while (st.sp > 0) {
+                uint32_t nn;
+                float dd;
+                st.pop(nn, dd);
+                if (dd <= best) { // query.cuh:259
+                    node = nn;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found)
+                break;
+        }
+        if (st.overflow)
+            if (err)
+                atomicOr(err, 1u);
+    }
+    out_idx[qi] = best_idx;
+    out_dist[qi] = dm::sqrt_(best);
+    if (STATS)
+        flush_counts(cnt, stats);
+}
+
+// ------------------------------------------------------------ silhouette
+// one leaf: the owned edges (scene.cuh:978-1003), each tested as in
+// find_closest_silhouette_point (scene.cuh:788-824)
+__device__ __forceinline__ void sil_leaf(const TreeView& t, uint32_t j, v3 p, bool flip, float& best,
+                                         uint32_t& best_e, Counts& cnt)
+{
+    if (j >= t.n_leaves)
+        return;
+    const uint32_t es = __float_as_uint(t.tleaf[3 * j + 1].w);
+    const uint32_t ec = __float_as_uint(t.tleaf[3 * j + 2].w);
+    cnt.leaf += 1;
+    cnt.edge += ec;
+    for (uint32_t k = 0; k < ec; ++k) {
+        const uint32_t e = es + k;
+        const float mr2 = best * best;
+        if (!(0.0f < mr2))
+            break; // min_radius_squared (0) >= max_radius_squared
+        const float4 A = t.tedge[4 * e], B = t.tedge[4 * e + 1];
+        const v3 pa = mk(A.x, A.y, A.z), pb = mk(B.x, B.y, B.z);
+        v3 cp;
+        const float d = closest_point_segment(pa, pb, p, cp);
+        if (d * d > mr2)
+            continue;
+        const uint32_t fl = __float_as_uint(B.w);
+        bool sil = (fl & 3u) != 3u; // a boundary edge is always a silhouette
+        if (!sil) {
+            const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
+            sil = is_silhouette_edge(pa, pb, mk(N0.x, N0.y, N0.z), mk(N1.x, N1.y, N1.z),
+                                     mk(p.x - cp.x, p.y - cp.y, p.z - cp.z), d, flip);
+        }
+        if (sil && (d < best || (d == best && e < best_e))) {
+            best = d;
+            best_e = e;
+        }
+    }
+}
+
+template <int BS, bool STATS>
+__global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
+                                                           int flip, const float* __restrict__ seed,
+                                                           uint32_t* __restrict__ out_e, float* __restrict__ out_d,
+                                                           uint32_t* __restrict__ err, unsigned long long* stats)
+{
+    __shared__ uint2 stk[SS * BS];
+    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    Counts cnt;
+    if (qi >= n) {
+        if (STATS)
+            flush_counts(cnt, stats);
+        return;
+    }
+    const v3 p = mk(q[3 * qi], q[3 * qi + 1], q[3 * qi + 2]);
+    float best = F_INF;
+    uint32_t best_e = INVALID;
+    if (t.n_leaves == 1) {
+        const box3 b = ldbox2(t.nbox, 0);
+        const float4 c0 = t.ncone[0], c1 = t.ncone[1];
+        if (cone_valid(c0.w) && cone_overlap(mk(c0.x, c0.y, c0.z), c0.w, c1.x, p, b, box_mindist(b, p)))
+            sil_leaf(t, 0, p, flip != 0, best, best_e, cnt);
+    } else {
+        // A seeded radius r0 caps the search: nodes farther than r0 are
+        // skipped.  A silhouette found at distance <= r0 is the global answer,
+        // because every node that could hold one at least as close was
+        // visited; otherwise the search reruns without the cap, keeping what
+        // it found as the initial bound.  With index tie-breaking the result
+        // is bit-identical to the unseeded search.
+        float cap = seed ? seed[qi] : F_INF;
+        if (!(cap >= 0.0f))
+            cap = F_INF;
+        LaneStack<BS> st;
+        for (int pass = 0; pass < 2; ++pass) {
+            st.init(stk + threadIdx.x);
+            const float cap2 = cap * cap;
+            uint32_t node = 0;
+            const uint32_t cap_iters = 2 * t.n_leaves + 64;
+            for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+                const NodeRec nr = load_node(t.tnode, node);
+                const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
+                const float b2 = best * best;
+                bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
+                if (STATS) {
+                    cnt.node += 1;
+                    cnt.box += 2;
+                }
+                if (hL || hR) {
+                    const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
+                    // query.cuh:366-367
+                    hL = hL && cone_valid(CL.w) && cone_overlap(mk(CL.x, CL.y, CL.z), CL.w, Cr.x, p, nr.L, Lmd);
+                    hR = hR && cone_valid(CR.w) && cone_overlap(mk(CR.x, CR.y, CR.z), CR.w, Cr.y, p, nr.R, Rmd);
+                }
+                const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
+                if (hL && lLeaf)
+                    sil_leaf(t, nr.lref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
+                if (hR && rLeaf)
+                    sil_leaf(t, nr.rref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
+                const float nb2 = best * best;
+                const bool gL = hL && !lLeaf && Lmd <= nb2;
+                const bool gR = hR && !rLeaf && Rmd <= nb2;
+                if (gL && gR) {
+                    const bool lfirst = Lmd <= Rmd;
+                    st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
+                    node = lfirst ? nr.lref : nr.rref;
+                    continue;
+                }
+                if (gL || gR) {
+                    node = gL ? nr.lref : nr.rref;
+                    continue;
+                }
+                bool found = false;
+                while (st.sp > 0) {
+                    uint32_t nn;
+                    float dd;
+                    st.pop(nn, dd);
+                    if (dd <= best * best) { // query.cuh:347
+                        node = nn;
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found)
+                    break;
+            }
+            if (st.overflow)
+                if (err)
+                atomicOr(err, 1u);
+            // every candidate within `best` was examined when best <= cap
+            if (cap == F_INF || (best_e != INVALID && best <= cap))
+                break;
+            cap = F_INF; // nothing within the seeded radius: exact fallback
+        }
+    }
+    if (out_e)
+        out_e[qi] = best_e;
+    out_d[qi] = best;
+    if (STATS)
+        flush_counts(cnt, stats);
+}
+
+// ------------------------------------------------------------ ray
+template <int BS, bool STATS>
+__global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict__ o, const float* __restrict__ d,
+                                            const float* __restrict__ tmax, uint32_t n, int any_hit,
+                                            uint8_t* __restrict__ out_hit, float* __restrict__ out_t,
+                                            float* __restrict__ out_uv, uint32_t* __restrict__ out_idx,
+                                            uint32_t* __restrict__ err, unsigned long long* stats)
+{
+    __shared__ uint2 stk[SS * BS];
+    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    Counts cnt;
+    if (qi >= n) {
+        if (STATS)
+            flush_counts(cnt, stats);
+        return;
+    }
+    const v3 ro = mk(o[3 * qi], o[3 * qi + 1], o[3 * qi + 2]);
+    const v3 rd = mk(d[3 * qi], d[3 * qi + 1], d[3 * qi + 2]);
+    const v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z); // aabb.cuh:308-311
+    const float max_dist = tmax ? tmax[qi] : F_INF;
+    float tbest = F_INF, bu = 0.0f, bv = 0.0f;
+    uint32_t bidx = INVALID;
+    bool found = false;
+    auto leaf = [&](uint32_t j) {
+        if (j >= t.n_leaves)
+            return;
+        if (STATS)
+            cnt.leaf += 1;
+        const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], c = t.tleaf[3 * j + 2];
+        float th, uh, vh;
+        const uint32_t o = __float_as_uint(a.w);
+        if (ray_triangle(ro, rd, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), th, uh, vh) &&
+            th < max_dist && (th < tbest || (th == tbest && o < bidx))) { // query.cuh:115
+            tbest = th;
+            bu = uh;
+            bv = vh;
+            bidx = o;
+            found = true;
+        }
+    };
+    if (t.n_leaves == 1) {
+        leaf(0);
+    } else {
+        LaneStack<BS> st;
+        st.init(stk + threadIdx.x);
+        uint32_t node = 0;
+        const uint32_t cap_iters = 2 * t.n_leaves + 64;
+        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+            const NodeRec nr = load_node(t.tnode, node);
+            float Ld, Rd;
+            if (STATS) {
+                cnt.node += 1;
+                cnt.box += 2;
+            }
+            bool Lh = box_ray(nr.L, ro, inv, max_dist, Ld) && Ld <= tbest;
+            bool Rh = box_ray(nr.R, ro, inv, max_dist, Rd) && Rd <= tbest;
+            const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
+            // nearer child first, as query.cuh:136-149
+            const bool rfirst = Lh && Rh && Rd < Ld;
+            if (rfirst) {
+                if (Rh && rLeaf) leaf(nr.rref & ~LEAF_BIT);
+                if (Lh && lLeaf && Ld <= tbest) leaf(nr.lref & ~LEAF_BIT);
+            } else {
+                if (Lh && lLeaf) leaf(nr.lref & ~LEAF_BIT);
+                if (Rh && rLeaf && Rd <= tbest) leaf(nr.rref & ~LEAF_BIT);
+            }
+            if (any_hit && found)
+                break;
+            const bool gL = Lh && !lLeaf && Ld <= tbest;
+            const bool gR = Rh && !rLeaf && Rd <= tbest;
+            if (gL && gR) {
+                const bool lnear = !(Rd < Ld);
+                st.push(lnear ? nr.rref : nr.lref, lnear ? Rd : Ld);
+                node = lnear ? nr.lref : nr.rref;
+                continue;
+            }
+            if (gL || gR) {
+                node = gL ? nr.lref : nr.rref;
+                continue;
+            }
+            bool more = false;
+            while (st.sp > 0) {
+                uint32_t nn;
+                float dd;
+                st.pop(nn, dd);
+                if (dd <= tbest) { // query.cuh:106
+                    node = nn;
+                    more = true;
+                    break;
+                }
+            }
+            if (!more)
+                break;
+        }
+        if (st.overflow)
+            if (err)
+                atomicOr(err, 1u);
+    }
+    if (out_hit)
+        out_hit[qi] = found ? 1 : 0;
+    if (STATS)
+        flush_counts(cnt, stats);
+    if (any_hit)
+        return;
+    if (out_t)
+        out_t[qi] = tbest;
+    if (out_uv) {
+        out_uv[2 * qi] = bu;
+        out_uv[2 * qi + 1] = bv;
+    }
+    if (out_idx)
+        out_idx[qi] = bidx;
+}
+
+// ------------------------------------------------------------ sampling
+// sample.cuh:23-92: a single root-to-leaf descent, bit-identical to the
+// reference's FP32 arithmetic (u rescaled at each level).
+__global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __restrict__ V,
+                                                const int32_t* __restrict__ F, const float* __restrict__ cc,
+                                                const float* __restrict__ rr, const float* __restrict__ uu,
+                                                uint32_t n, int32_t* __restrict__ out_idx,
+                                                float* __restrict__ out_pdf, float* __restrict__ out_pt)
+{
+    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (qi >= n)
+        return;
+    const v3 c = mk(cc[3 * qi], cc[3 * qi + 1], cc[3 * qi + 2]);
+    const float r = rr[qi];
+    float u = uu[3 * qi];
+    float npdf = 1.0f;
+    int32_t obj = -1;
+    float pdf = 0.0f;
+    uint32_t ref = t.n_leaves == 1 ? LEAF_BIT : 0u;
+    for (int depth = 0; depth < 96; ++depth) { // a single descent: at most the tree depth
+        if (ref & LEAF_BIT) {
+            const uint32_t j = ref & ~LEAF_BIT;
+            if (j >= t.n_leaves)
+                break;
+            const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], d = t.tleaf[3 * j + 2];
+            const v3 pa = mk(a.x, a.y, a.z), pb = mk(b.x, b.y, b.z), pc = mk(d.x, d.y, d.z);
+            if (tri_sphere(pa, pb, pc, c, r)) {
+                obj = (int32_t)__float_as_uint(a.w);
+                pdf = npdf;
+                pdf /= tri_area(pa, pb, pc);
+            }
+            break;
+        }
+        if (ref >= t.n_leaves - 1)
+            break;
+        const NodeRec nr = load_node(t.tnode, ref);
+        const float Lw = box_sphere(nr.L, c, r) ? green_weight(c, box_centroid(nr.L)) : 0.0f;
+        const float Rw = box_sphere(nr.R, c, r) ? green_weight(c, box_centroid(nr.R)) : 0.0f;
+        const float tw = Lw + Rw;
+        if (!(tw > 0.0f))
+            break;
+        const float Lp = Lw / tw;
+        if (u < Lp) {
+            u /= Lp;
+            npdf = Lp * npdf;
+            ref = nr.lref;
+        } else {
+            const float Rp = 1.0f - Lp;
+            u = (u - Lp) / Rp;
+            npdf = Rp * npdf;
+            ref = nr.rref;
+        }
+    }
+    out_idx[qi] = obj;
+    out_pdf[qi] = obj >= 0 ? pdf : 0.0f;
+    if (out_pt) {
+        v3 pt = mk(0.0f, 0.0f, 0.0f);
+        if (obj >= 0) {
+            const float4 a = V[F[3 * obj]], b = V[F[3 * obj + 1]], d = V[F[3 * obj + 2]];
+            pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uu[3 * qi + 1],
+                                 uu[3 * qi + 2]);
+        }
+        out_pt[3 * qi] = pt.x;
+        out_pt[3 * qi + 1] = pt.y;
+        out_pt[3 * qi + 2] = pt.z;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sample_on_object(const float4* __restrict__ V, const int32_t* __restrict__ F,
+                                                          uint32_t nF, const int32_t* __restrict__ idx,
+                                                          const float* __restrict__ uv, uint32_t n,
+                                                          float* __restrict__ out_pt)
+{
+    const uint32_t qi = blockIdx.x * 256 + threadIdx.x;
+    if (qi >= n)
+        return;
+    const int32_t o = idx[qi];
+    v3 pt = mk(0.0f, 0.0f, 0.0f);
+    if (o >= 0 && (uint32_t)o < nF) {
+        const float4 a = V[F[3 * o]], b = V[F[3 * o + 1]], d = V[F[3 * o + 2]];
+        pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uv[2 * qi], uv[2 * qi + 1]);
+    }
+    out_pt[3 * qi] = pt.x;
+    out_pt[3 * qi + 1] = pt.y;
+    out_pt[3 * qi + 2] = pt.z;
+}
+
+inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
+
+} // namespace
+
+// error word for traversal overflow, per device (tiny, allocated once)
+uint32_t* traversal_error_word();
+
+hipError_t launch_closest_point(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
+                                uint64_t* stats, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    auto* sp = reinterpret_cast<unsigned long long*>(stats);
+    if (stats)
+        hipLaunchKernelGGL((k_closest_point<256, true>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n, idx, dist,
+                           traversal_error_word(), sp);
+    else
+        hipLaunchKernelGGL((k_closest_point<256, false>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n, idx,
+                           dist, traversal_error_word(), sp);
+    return hipGetLastError();
+}
+
+template <int BS>
+static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
+                          float* dist, unsigned long long* stats, hipStream_t st)
+{
+    if (stats)
+        hipLaunchKernelGGL((k_closest_silhouette<BS, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n, flip,
+                           seed, sil, dist, traversal_error_word(), stats);
+    else
+        hipLaunchKernelGGL((k_closest_silhouette<BS, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n, flip,
+                           seed, sil, dist, traversal_error_word(), stats);
+}
+
+hipError_t launch_closest_silhouette(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
+                                     uint32_t* sil, float* dist, uint32_t block, uint64_t* stats, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    auto* sp = reinterpret_cast<unsigned long long*>(stats);
+    switch (block) {
+    case 64: launch_sil_bs<64>(t, q, n, flip, seed, sil, dist, sp, st); break;
+    case 128: launch_sil_bs<128>(t, q, n, flip, seed, sil, dist, sp, st); break;
+    case 512: launch_sil_bs<512>(t, q, n, flip, seed, sil, dist, sp, st); break;
+    default: launch_sil_bs<256>(t, q, n, flip, seed, sil, dist, sp, st); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ray(const TreeView& t, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
+                      uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    auto* sp = reinterpret_cast<unsigned long long*>(stats);
+    if (stats)
+        hipLaunchKernelGGL((k_ray<256, true>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n, any_hit, hit,
+                           tout, uv, idx, traversal_error_word(), sp);
+    else
+        hipLaunchKernelGGL((k_ray<256, false>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n, any_hit,
+                           hit, tout, uv, idx, traversal_error_word(), sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const TreeView& t, const float4* V, const int32_t* F, const float* c, const float* r,
+                         const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_sample, dim3(nblocks(n, 256)), dim3(256), 0, st, t, V, F, c, r, u, n, idx, pdf, pt);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_on_object(const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
+                                   const float* uv, uint32_t n, float* pt, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_sample_on_object, dim3(nblocks(n, 256)), dim3(256), 0, st, V, F, nF, idx, uv, n, pt);
+    return hipGetLastError();
+}
+
+} // namespace snch

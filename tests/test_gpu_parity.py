@@ -1,0 +1,202 @@
+"""GPU parity: the HIP kernels behind the C ABI against the CPU oracle.
+
+Bars: build (Morton codes, sorted order, topology, AABBs, cones) bit-exact;
+sampler bit-exact; closest point / silhouette / ray: distances bit-exact and
+indices identical except where the oracle's FP32 distance of both candidates
+is exactly tied (the library breaks ties by smallest index, the reference by
+visit order)."""
+import numpy as np
+import pytest
+
+from snch_lbvh_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import snch_lbvh_amd as S
+
+    S.lib()
+    return S
+
+
+def _soup(seed=9, nv=300, nf=500):
+    rng = np.random.default_rng(seed)
+    return rng.random((nv, 3)).astype(np.float32), rng.integers(0, nv, size=(nf, 3)).astype(np.int32)
+
+
+MESHES = {
+    "ico5": lambda: W.icosphere(5, 0.02, seed=1),
+    "ico2_clean": lambda: W.icosphere(2, 0.0),
+    "torus": lambda: W.cut_torus(96, 48),
+    "soup": _soup,
+    "one": lambda: (np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.int32([[0, 1, 2]])),
+    "two": lambda: (np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.5]]), np.int32([[0, 1, 2], [1, 3, 2]])),
+    "three": lambda: (np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.5], [2, 0, 1]]),
+                      np.int32([[0, 1, 2], [1, 3, 2], [1, 4, 3]])),
+}
+
+
+def _queries(V, n, seed):
+    lo, hi = W.padded_box(V, 0.25)
+    return W.uniform_points(n, lo, hi, seed)
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+@pytest.mark.parametrize("reference_radius", [False, True])
+def test_build_bit_exact(S, oracle, name, reference_radius):
+    V, F = MESHES[name]()
+    sc = S.Scene(V, F).build_bvh(reference_radius=reference_radius)
+    g = sc.export()
+    o = oracle.build_tree(V, F, conservative_radius=not reference_radius)
+    slots, owned = sc.silhouettes()
+    _, oslots, oowned = oracle.edge_table(F)
+    np.testing.assert_array_equal(slots, oslots)
+    np.testing.assert_array_equal(owned, oowned)
+    np.testing.assert_array_equal(g["sorted_codes"], o.sorted_codes)
+    np.testing.assert_array_equal(g["sorted_idx"], o.sorted_idx)
+    np.testing.assert_array_equal(g["nodes"], o.nodes)
+    np.testing.assert_array_equal(g["boxes"], o.boxes)
+    np.testing.assert_array_equal(g["cones"], o.cones)
+
+
+def test_build_deterministic(S):
+    V, F = W.cut_torus(128, 64)
+    a = S.Scene(V, F).build_bvh().export()
+    b = S.Scene(V, F).build_bvh().export()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def _check_ties(oracle, V, F, q, gi, oi, gd, od, f32dist):
+    np.testing.assert_array_equal(gd, od)
+    diff = gi != oi
+    if diff.any():
+        dg = f32dist(gi[diff], q[diff])
+        do = f32dist(oi[diff], q[diff])
+        np.testing.assert_array_equal(dg, do)
+    return float(np.mean(diff))
+
+
+@pytest.mark.parametrize("name", ["ico5", "torus", "soup", "one", "two", "three"])
+def test_closest_point_parity(S, oracle, name):
+    V, F = MESHES[name]()
+    q = _queries(V, 20000, 2)
+    sc = S.Scene(V, F).build_bvh()
+    gi, gd = (t.cpu().numpy() for t in sc.closest_point(q))
+    o = oracle.build_tree(V, F)
+    oi, od = oracle.closest_point(o, q)
+    sq = lambda tri, qq: oracle.tri_dist(V, F, tri, qq) ** 2  # noqa: E731
+    _check_ties(oracle, V, F, q, gi.view(np.uint32), oi, gd, od, sq)
+
+
+@pytest.mark.parametrize("name", ["ico5", "torus", "soup", "one", "two", "three"])
+@pytest.mark.parametrize("flip", [False, True])
+def test_closest_silhouette_parity(S, oracle, name, flip):
+    V, F = MESHES[name]()
+    q = _queries(V, 20000, 3)
+    sc = S.Scene(V, F).build_bvh()
+    ge, gd = (t.cpu().numpy() for t in sc.closest_silhouette(q, flip=flip))
+    o = oracle.build_tree(V, F)
+    oe, od = oracle.closest_silhouette(o, q, flip=flip)
+    f = lambda e, qq: oracle.edge_sil_dist(V, o.slots, e, qq, flip=flip)  # noqa: E731
+    _check_ties(oracle, V, F, q, ge.view(np.uint32), oe, gd, od, f)
+
+
+@pytest.mark.parametrize("name", ["ico5", "torus", "soup", "two"])
+def test_ray_parity(S, oracle, name):
+    V, F = MESHES[name]()
+    n = 20000
+    o_ = _queries(V, n, 4)
+    d = W.unit_directions(n, 5)
+    tmax = np.where(np.arange(n) % 3 == 0, np.float32(0.3), np.float32(np.inf)).astype(np.float32)
+    sc = S.Scene(V, F).build_bvh()
+    hit, t, uv, idx = (x.cpu().numpy() for x in sc.ray_intersect(o_, d, tmax))
+    tr = oracle.build_tree(V, F)
+    ohit, ot, ouv, oidx = oracle.ray(tr, o_, d, tmax)
+    np.testing.assert_array_equal(hit, ohit)
+    np.testing.assert_array_equal(t[hit], ot[hit])
+    same = idx.view(np.uint32) == oidx
+    np.testing.assert_array_equal(uv[hit & same], ouv[hit & same])
+    ahit = sc.ray_intersect(o_, d, tmax, any_hit=True)[0].cpu().numpy()
+    np.testing.assert_array_equal(ahit, ohit)
+
+
+@pytest.mark.parametrize("name", ["ico5", "torus", "soup", "one", "two"])
+def test_sample_bit_exact(S, oracle, name):
+    V, F = MESHES[name]()
+    n = 20000
+    c = _queries(V, n, 6)
+    r = W.log_uniform(n, 1e-3, 0.5, 7)
+    u = np.random.default_rng(8).random((n, 3)).astype(np.float32)
+    sc = S.Scene(V, F).build_bvh()
+    gi, gp, gpt = (x.cpu().numpy() for x in sc.sample_in_sphere(c, r, u))
+    tr = oracle.build_tree(V, F)
+    oi, op, opt = oracle.sample_in_sphere(tr, c, r, u)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gpt, opt)
+    ok = oi >= 0
+    pt2 = sc.sample_on_object(gi[ok], u[ok, 1:]).cpu().numpy()
+    np.testing.assert_array_equal(pt2, opt[ok])
+
+
+def test_seeded_silhouette_identical(S):
+    V, F = W.cut_torus(96, 48)
+    q = _queries(V, 30000, 9)
+    sc = S.Scene(V, F).build_bvh()
+    e0, d0 = (t.cpu().numpy() for t in sc.closest_silhouette(q))
+    rng = np.random.default_rng(1)
+    for seed in (np.zeros(len(q)), d0 * 0.5, d0, d0 * 1.0001 + 1e-3, rng.random(len(q)) * 0.3,
+                 np.full(len(q), -1.0), np.full(len(q), np.nan)):
+        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=seed.astype(np.float32)))
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_array_equal(d1, d0)
+    for bs in (64, 128, 512):
+        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, block_size=bs))
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_array_equal(d1, d0)
+
+
+def test_silhouette_vs_fp64_bruteforce(S, oracle):
+    V, F = W.cut_torus(64, 32)
+    q = _queries(V, 4096, 10)
+    sc = S.Scene(V, F).build_bvh()
+    ge, gd = (t.cpu().numpy() for t in sc.closest_silhouette(q))
+    slots, _ = sc.silhouettes()
+    be, bd = oracle.bf_closest_silhouette(V, slots, q)
+    fin = np.isfinite(bd)
+    assert np.array_equal(np.isfinite(gd), fin)
+    d64 = oracle.edge_sil_dist64(V, slots, ge.view(np.uint32)[fin], q[fin])
+    assert np.all(d64 - bd[fin] <= 1e-6)
+
+
+def test_error_paths(S):
+    import ctypes
+
+    L = S.lib()
+    V, F = MESHES["two"]()
+    with pytest.raises(S.SnchError) as e:
+        S.Scene(V, np.zeros((0, 3), np.int32))
+    assert e.value.code == 2
+    bad = V.copy()
+    bad[0, 0] = np.nan
+    with pytest.raises(S.SnchError) as e:
+        S.Scene(bad, F)
+    assert e.value.code == 1
+    with pytest.raises(S.SnchError) as e:
+        S.Scene(V, np.int32([[0, 1, 7]]))
+    assert e.value.code == 1
+    sc = S.Scene(V, F)
+    with pytest.raises(S.SnchError) as e:
+        sc.closest_point(np.zeros((4, 3), np.float32))
+    assert e.value.code == 4 and "not built" in str(e.value)
+    sc.build_bvh()
+    idx, d = sc.closest_point(np.zeros((0, 3), np.float32))
+    assert idx.numel() == 0
+    assert L.snch_closest_point(sc._h, None, 5, None, None, None) == 1
