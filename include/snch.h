@@ -136,6 +136,33 @@ int snch_scene_export(const snch_scene *scene, uint32_t *nodes, float *boxes, fl
  * triangle, 2 per segment, -1 padded). */
 int snch_scene_silhouettes(const snch_scene *scene, int32_t *slots, int32_t *owned);
 
+/* ---- learned distance bound (Fourier-feature MLP on BF16 MFMA) ----------
+ * No counterpart in the reference.  3 -> 64 Fourier features (sigma) -> 4 x 128
+ * SiLU -> 1, BF16 weights and activations, FP32 accumulation.  It regresses a
+ * query's closest-silhouette distance; snch_mlp_predict(mode = 1) turns the
+ * prediction into a seed radius for snch_closest_silhouette, whose exact
+ * verification and fallback keep results identical to the unseeded call. */
+typedef struct snch_mlp snch_mlp;
+uint32_t snch_mlp_num_params(void);
+/* lo/hi: the box the inputs live in (normalised to [-1, 1]^3) */
+int snch_mlp_create(snch_context *ctx, const float *lo, const float *hi, float sigma, uint64_t seed, snch_mlp **out);
+int snch_mlp_destroy(snch_mlp *mlp);
+/* host copies: params_out/params_in (snch_mlp_num_params() floats, FP32 master
+ * weights), fourier_out (32 x 3, already scaled by 2 pi), frame_out (8 floats:
+ * centre xyz, 1/half-extent xyz, 1/D, D).  Any may be NULL. */
+int snch_mlp_params(snch_mlp *mlp, float *params_out, const float *params_in, float *fourier_out, float *frame_out);
+/* Adam training on device points (n x 3) and exact distances (n); batch must be
+ * a multiple of 64; epoch_loss (host, epochs floats, may be NULL) gets the mean
+ * loss per epoch.  Synchronises the stream. */
+int snch_mlp_train(snch_mlp *mlp, const float *pts, const float *dist, uint32_t n, uint32_t batch, uint32_t epochs,
+                   float lr, float *epoch_loss, void *stream);
+/* Gradient of the mean loss over n points (host output, FP32), for checking. */
+int snch_mlp_gradient(snch_mlp *mlp, const float *pts, const float *dist, uint32_t n, float *grad_out,
+                      float *loss_out, void *stream);
+/* mode 0: predicted distance; mode 1: seed radius (pred * (1 + rel) + abs * D) */
+int snch_mlp_predict(snch_mlp *mlp, const float *pts, uint32_t n, int mode, float rel, float abs_margin, float *out,
+                     void *stream);
+
 /* ---- device memory helpers for callers without their own allocator ------- */
 int snch_device_malloc(void **ptr, uint64_t bytes);
 int snch_device_free(void *ptr);

@@ -294,3 +294,129 @@ def edge_sil_dist64(V, slots, edge, q, flip=False):
     L.orc_edge_sil64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
     L.orc_edge_sil64(_p(V), _p(slots), _p(edge), _p(q), edge.size, int(bool(flip)), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- 2D (scene<2>)
+class OrcTree2D(ctypes.Structure):
+    _fields_ = OrcTree._fields_
+
+
+def silhouettes2d(S, nV):
+    """Silhouette-vertex slots (prev, v, next, -1) per vertex and the 2 owned
+    vertices per segment (scene.cuh:631-672)."""
+    S = _c(S, I32).reshape(-1, 2)
+    slots = np.empty((nV, 4), I32)
+    owned = np.empty((S.shape[0], 2), I32)
+    L = lib()
+    L.orc_silhouettes2d.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_silhouettes2d(_p(S), S.shape[0], nV, _p(slots), _p(owned))
+    return slots, owned
+
+
+class Tree2D(Tree):
+    def __init__(self, V, S, slots, owned, nodes, boxes, cones, codes, sidx, collision):
+        self.V, self.F, self.slots, self.owned = V, S, slots, owned
+        self.nodes, self.boxes, self.cones = nodes, boxes, cones
+        self.sorted_codes, self.sorted_idx, self.collision = codes, sidx, collision
+        self.n_obj = S.shape[0]
+        self._s = OrcTree2D(self.n_obj, _p(nodes), _p(boxes), _p(cones), _p(V), _p(S), _p(slots), _p(owned))
+
+
+def build_tree2d(V, S, conservative_radius=True):
+    V = _c(V, F32).reshape(-1, 2)
+    S = _c(S, I32).reshape(-1, 2)
+    slots, owned = silhouettes2d(S, V.shape[0])
+    N = S.shape[0]
+    NN = 2 * N - 1
+    nodes = np.empty((NN, 4), U32)
+    boxes = np.empty((NN, 6), F32)
+    cones = np.empty((NN, 5), F32)
+    codes = np.empty(N, U32)
+    sidx = np.empty(N, U32)
+    L = lib()
+    L.orc_build2d.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int32] + [ctypes.c_void_p] * 7 + [ctypes.c_int]
+    col = L.orc_build2d(_p(V), _p(S), N, _p(slots), _p(owned), _p(nodes), _p(boxes), _p(cones), _p(codes), _p(sidx),
+                        1 if conservative_radius else 0)
+    if col < 0:
+        raise ValueError("empty mesh")
+    return Tree2D(V, S, slots, owned, nodes, boxes, cones, codes, sidx, bool(col))
+
+
+def closest_point2d(tree, q):
+    q = _c(q, F32).reshape(-1, 2)
+    n = q.shape[0]
+    idx = np.empty(n, U32)
+    dist = np.empty(n, F32)
+    L = lib()
+    L.orc_closest_point2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_closest_point2d(tree.ref, _p(q), n, _p(idx), _p(dist))
+    return idx, dist
+
+
+def closest_silhouette2d(tree, q, flip=False):
+    q = _c(q, F32).reshape(-1, 2)
+    n = q.shape[0]
+    sil = np.empty(n, U32)
+    dist = np.empty(n, F32)
+    L = lib()
+    L.orc_closest_silhouette2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_closest_silhouette2d(tree.ref, _p(q), n, int(bool(flip)), _p(sil), _p(dist))
+    return sil, dist
+
+
+def ray2d(tree, o, d, tmax=None):
+    o = _c(o, F32).reshape(-1, 2)
+    d = _c(d, F32).reshape(-1, 2)
+    n = o.shape[0]
+    tmax = _c(np.full(n, np.inf) if tmax is None else tmax, F32)
+    idx = np.empty(n, U32)
+    t = np.empty(n, F32)
+    s = np.empty(n, F32)
+    hit = np.empty(n, np.uint8)
+    L = lib()
+    L.orc_ray2d.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 4
+    L.orc_ray2d(tree.ref, _p(o), _p(d), _p(tmax), n, _p(idx), _p(t), _p(s), _p(hit))
+    return hit.astype(bool), t, s, idx
+
+
+def sample2d(tree, c, r, u):
+    c = _c(c, F32).reshape(-1, 2)
+    n = c.shape[0]
+    r = _c(r, F32).ravel()
+    u = _c(u, F32).reshape(n, 3)
+    idx = np.empty(n, I32)
+    pdf = np.empty(n, F32)
+    pt = np.empty((n, 2), F32)
+    L = lib()
+    L.orc_sample2d.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 3
+    L.orc_sample2d(tree.ref, _p(c), _p(r), _p(u), n, _p(idx), _p(pdf), _p(pt))
+    return idx, pdf, pt
+
+
+def bf_closest_point2d(V, S, q):
+    V = _c(V, F32).reshape(-1, 2)
+    S = _c(S, I32).reshape(-1, 2)
+    q = _c(q, F32).reshape(-1, 2)
+    n = q.shape[0]
+    idx = np.empty(n, U32)
+    dist = np.empty(n, np.float64)
+    L = lib()
+    L.orc_bf_closest_point2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_bf_closest_point2d(_p(V), _p(S), S.shape[0], _p(q), n, _p(idx), _p(dist))
+    return idx, dist
+
+
+def bf_closest_silhouette2d(V, slots, q, flip=False):
+    V = _c(V, F32).reshape(-1, 2)
+    slots = _c(slots, I32).reshape(-1, 4)
+    q = _c(q, F32).reshape(-1, 2)
+    n = q.shape[0]
+    sil = np.empty(n, U32)
+    dist = np.empty(n, np.float64)
+    L = lib()
+    L.orc_bf_closest_silhouette2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                              ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_bf_closest_silhouette2d(_p(V), _p(slots), slots.shape[0], _p(q), n, int(bool(flip)), _p(sil), _p(dist))
+    return sil, dist

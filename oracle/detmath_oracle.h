@@ -183,4 +183,30 @@ static inline void odm_sincosf(float x, float *sn, float *cs)
     }
 }
 
+/* log(x) for positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+ * log(m) = 2 atanh(s) = 2 s + s^3 P(s^2), s = (m - 1) / (m + 1) */
+#define ODM_LN2_HI 0.6931471824645996f
+#define ODM_LN2_LO (-1.904654323148236e-09f)
+static inline float odm_logf(float x)
+{
+    uint32_t u;
+    int e;
+    float m, s, z, p;
+    memcpy(&u, &x, 4);
+    e = (int)((u >> 23) & 0xFFu) - 127;
+    u = (u & 0x7FFFFFu) | 0x3F800000u;
+    memcpy(&m, &u, 4);
+    if (m > 1.41421356237f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    s = (m - 1.0f) / (m + 1.0f);
+    z = s * s;
+    p = 0.2333136796951294f;
+    p = p * z + 0.28550779819488525f;
+    p = p * z + 0.4000012278556824f;
+    p = p * z + 0.6666666865348816f;
+    return (float)e * ODM_LN2_HI + ((2.0f * s + (s * z) * p) + (float)e * ODM_LN2_LO);
+}
+
 #endif

@@ -1562,3 +1562,765 @@ void orc_edge_sil64(const float *V, const int32_t *slots, const uint32_t *edge, 
         out[i] = d;
     }
 }
+
+/* ================================================================== */
+/* scene<2>: line segments with silhouette vertices (scene.cuh:287-703) */
+/* Boxes and cones use the 3D records with z = 0.                     */
+/* ================================================================== */
+typedef struct { float x, y; } f2;
+static inline f2 mk2(float x, float y) { f2 r = {x, y}; return r; }
+static inline f2 vget2(const float *V, int i) { return mk2(V[2 * i], V[2 * i + 1]); }
+static inline float dot2(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+static inline float len2(f2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+static inline f2 normalize2(f2 v) { float n = len2(v); return mk2(v.x / n, v.y / n); }
+#define BVH_OFFSET 1e-3f /* scene.cuh:12 */
+
+/* compute_silhouettes (scene.cuh:631-652) and build_bvh ownership (:654-672).
+ * slots: 4 per vertex (prev, v, next, -1); owned: 2 per segment. */
+void orc_silhouettes2d(const int32_t *S, int32_t nS, int32_t nV, int32_t *slots, int32_t *owned)
+{
+    int32_t i;
+    unsigned char *seen = (unsigned char *)calloc((size_t)(nV > 0 ? nV : 1), 1);
+    for (i = 0; i < 4 * nV; ++i) slots[i] = -1;
+    for (i = 0; i < nS; ++i) {
+        int32_t a = S[2 * i], b = S[2 * i + 1];
+        slots[4 * a + 1] = a;
+        slots[4 * a + 2] = b;
+        slots[4 * b + 0] = a;
+        slots[4 * b + 1] = b;
+    }
+    for (i = 0; i < nS; ++i) {
+        int k, ptr = 0;
+        owned[2 * i] = owned[2 * i + 1] = -1;
+        for (k = 0; k < 2; ++k) {
+            int32_t v = S[2 * i + k];
+            if (!seen[v]) {
+                seen[v] = 1;
+                owned[2 * i + ptr++] = v;
+            }
+        }
+    }
+    free(seen);
+}
+
+static inline int sv_has_face(const int32_t *s, int f) { return f == 0 ? s[2] != -1 : s[0] != -1; } /* :322-325 */
+static f2 sv_face_normal(const float *V, const int32_t *s, int f, int do_norm) /* :326-336 */
+{
+    int i = f == 0 ? 1 : 0;
+    f2 pa = vget2(V, s[i]), pb = vget2(V, s[i + 1]);
+    f2 d = mk2(pb.x - pa.x, pb.y - pa.y);
+    f2 n = mk2(d.y, -d.x);
+    return do_norm ? normalize2(n) : n;
+}
+static f2 sv_normal(const float *V, const int32_t *s) /* :337-353 */
+{
+    f2 n = mk2(0.0f, 0.0f);
+    if (sv_has_face(s, 0)) { f2 a = sv_face_normal(V, s, 0, 0); n = mk2(n.x + a.x, n.y + a.y); }
+    if (sv_has_face(s, 1)) { f2 b = sv_face_normal(V, s, 1, 0); n = mk2(n.x + b.x, n.y + b.y); }
+    return normalize2(n);
+}
+
+static box3 leaf_box2d(const float *V, const int32_t *S, int32_t i) /* :420-433 */
+{
+    f2 p0 = vget2(V, S[2 * i]), p1 = vget2(V, S[2 * i + 1]);
+    box3 b;
+    b.hi = mk(fmaxf(p0.x, p1.x) + BVH_OFFSET, fmaxf(p0.y, p1.y) + BVH_OFFSET, 0.0f);
+    b.lo = mk(fminf(p0.x, p1.x) - BVH_OFFSET, fminf(p0.y, p1.y) - BVH_OFFSET, 0.0f);
+    return b;
+}
+
+static cone3 leaf_cone2d(const float *V, const int32_t *S, const int32_t *slots, const int32_t *owned, int32_t i)
+{ /* :435-502 */
+    box3 b = leaf_box2d(V, S, i);
+    f3 c3 = box_centroid(b);
+    f2 c = mk2(c3.x, c3.y), ax = mk2(0.0f, 0.0f);
+    cone3 r;
+    int any = 0, all_two = 1, k;
+    r.half_angle = F_PI;
+    r.radius = 0.0f;
+    for (k = 0; k < 2; ++k) {
+        int32_t v = owned[2 * i + k];
+        if (v != -1) {
+            const int32_t *s = slots + 4 * v;
+            f2 n = sv_normal(V, s), p = vget2(V, s[1]);
+            ax.x += n.x;
+            ax.y += n.y;
+            r.radius = smax(r.radius, len2(mk2(p.x - c.x, p.y - c.y)));
+            all_two = all_two && sv_has_face(s, 0) && sv_has_face(s, 1);
+            any = 1;
+        }
+    }
+    if (!any) {
+        r.half_angle = -F_PI;
+    } else if (!all_two) {
+        r.half_angle = F_PI;
+    } else {
+        float an = len2(ax);
+        if (an > F_EPS) {
+            ax.x /= an;
+            ax.y /= an;
+            r.half_angle = 0.0f;
+            for (k = 0; k < 2; ++k) {
+                int32_t v = owned[2 * i + k];
+                if (v != -1) {
+                    const int32_t *s = slots + 4 * v;
+                    f2 nf[2];
+                    int jj;
+                    nf[0] = sv_has_face(s, 0) ? sv_face_normal(V, s, 0, 1) : mk2(0.0f, 0.0f);
+                    nf[1] = sv_has_face(s, 1) ? sv_face_normal(V, s, 1, 1) : mk2(0.0f, 0.0f);
+                    for (jj = 0; jj < 2; ++jj)
+                        r.half_angle = smax(r.half_angle, odm_acosf(clamp11(dot2(ax, nf[jj]))));
+                }
+            }
+        }
+    }
+    r.axis = mk(ax.x, ax.y, 0.0f);
+    return r;
+}
+
+static inline uint32_t morton2(f2 p) /* morton_code.cuh:85-92 */
+{
+    const float res = 1024.0f;
+    float x = fminf(fmaxf(p.x * res, 0.0f), res - 1.0f);
+    float y = fminf(fmaxf(p.y * res, 0.0f), res - 1.0f);
+    return expand_bits((uint32_t)x) * 2u + expand_bits((uint32_t)y);
+}
+
+/* cone.cuh:78-121 */
+static int cone_overlap2d(cone3 bc, f2 o, box3 b, float dist_to_box)
+{
+    f2 c, v, e, bb;
+    float l, d_axis, d, s, vha, sum;
+    if (bc.half_angle >= F_PI_2 || dist_to_box < F_EPS)
+        return 1;
+    c = mk2((b.hi.x + b.lo.x) * 0.5f, (b.hi.y + b.lo.y) * 0.5f);
+    v = mk2(c.x - o.x, c.y - o.y);
+    l = len2(v);
+    v.x /= l;
+    v.y /= l;
+    d_axis = odm_acosf(clamp11(dot2(mk2(bc.axis.x, bc.axis.y), v)));
+    if (inrange(F_PI_2, d_axis - bc.half_angle, d_axis + bc.half_angle))
+        return 1;
+    if (l > bc.radius) {
+        vha = odm_asinf(bc.radius / l);
+        sum = bc.half_angle + vha;
+        return sum >= F_PI_2 ? 1 : inrange(F_PI_2, d_axis - sum, d_axis + sum);
+    }
+    e = mk2(b.hi.x - c.x, b.hi.y - c.y);
+    d = dot2(e, mk2(fabsf(v.x), fabsf(v.y)));
+    s = l - d;
+    if (s <= 0.0f)
+        return 1;
+    bb = mk2(-v.y, v.x); /* project_to_plane, cone.cuh:44-49 */
+    d = fabsf(dot2(e, mk2(fabsf(bb.x), fabsf(bb.y))));
+    vha = odm_atan2f(d, s);
+    sum = bc.half_angle + vha;
+    return sum >= F_PI_2 ? 1 : inrange(F_PI_2, d_axis - sum, d_axis + sum);
+}
+
+/* cone.cuh:266-275 and :320-372 (Q1 as in 3D) */
+static cone3 cone_merge2d(cone3 a, cone3 b, f3 oa, f3 ob, f3 on)
+{
+    cone3 r;
+    memset(&r, 0, sizeof(r));
+    if (cone_valid(a) && cone_valid(b)) {
+        f2 axa = mk2(a.axis.x, a.axis.y), axb = mk2(b.axis.x, b.axis.y), t2;
+        float ha = a.half_angle, hb = b.half_angle, t, theta, o_theta;
+        f2 da = mk2(on.x - oa.x, on.y - oa.y), db = mk2(on.x - ob.x, on.y - ob.y);
+        if (g_conservative_radius)
+            r.radius = smax(a.radius + len2(da), b.radius + len2(db));
+        else
+            r.radius = sqrtf(smax(a.radius * a.radius + (da.x * da.x + da.y * da.y),
+                                  b.radius * b.radius + (db.x * db.x + db.y * db.y)));
+        if (hb > ha) {
+            t2 = axa; axa = axb; axb = t2;
+            t = ha; ha = hb; hb = t;
+        }
+        theta = odm_acosf(clamp11(dot2(axa, axb)));
+        if (smin(theta + hb, F_PI) <= ha) {
+            r.axis = mk(axa.x, axa.y, 0.0f);
+            r.half_angle = ha;
+            return r;
+        }
+        o_theta = (ha + theta + hb) / 2.0f;
+        if (o_theta >= F_PI) {
+            r.axis = mk(axa.x, axa.y, 0.0f);
+            r.half_angle = F_PI;
+            return r;
+        }
+        {
+            float rt = o_theta - ha, det = axa.x * axb.y - axa.y * axb.x, st, ct;
+            rt *= copysignf(1.0f, det);
+            odm_sincosf(rt, &st, &ct);
+            r.axis = mk(ct * axa.x - st * axa.y, st * axa.x + ct * axa.y, 0.0f);
+        }
+        r.half_angle = o_theta;
+    } else if (cone_valid(a)) {
+        r = a;
+    } else if (cone_valid(b)) {
+        r = b;
+    } else {
+        r.half_angle = -F_PI;
+    }
+    return r;
+}
+
+int orc_build2d(const float *V, const int32_t *S, int32_t nS, const int32_t *slots, const int32_t *owned,
+                uint32_t *nodes, float *boxes, float *cones, uint32_t *sorted_codes, uint32_t *sorted_idx,
+                int flags_opt)
+{
+    uint32_t N = (uint32_t)nS, NI, NN, i;
+    box3 *bx, whole;
+    cone3 *cn;
+    code_idx *ci;
+    uint32_t *codes;
+    uint64_t *codes64 = NULL;
+    int collision = 0, *flags;
+    keyset ks;
+    if (nS <= 0) return -1;
+    g_conservative_radius = flags_opt & 1;
+    NI = N - 1;
+    NN = 2 * N - 1;
+    bx = (box3 *)malloc(sizeof(box3) * NN);
+    cn = (cone3 *)malloc(sizeof(cone3) * NN);
+    ci = (code_idx *)malloc(sizeof(code_idx) * N);
+    codes = (uint32_t *)malloc(sizeof(uint32_t) * N);
+    for (i = 0; i < NN; ++i) {
+        bx[i].hi = mk(-INFINITY, -INFINITY, 0.0f);
+        bx[i].lo = mk(INFINITY, INFINITY, 0.0f);
+        cn[i].axis = mk(0.0f, 0.0f, 0.0f);
+        cn[i].half_angle = F_PI;
+        cn[i].radius = 0.0f;
+    }
+    for (i = 0; i < N; ++i) {
+        bx[NI + i] = leaf_box2d(V, S, (int32_t)i);
+        cn[NI + i] = leaf_cone2d(V, S, slots, owned, (int32_t)i);
+    }
+    whole.hi = mk(-INFINITY, -INFINITY, 0.0f);
+    whole.lo = mk(INFINITY, INFINITY, 0.0f);
+    for (i = 0; i < N; ++i) whole = box_merge(whole, bx[NI + i]);
+    for (i = 0; i < N; ++i) { /* bvh.cuh:269-277 */
+        f3 c = box_centroid(bx[NI + i]);
+        f2 p = mk2(c.x, c.y);
+        p.x -= whole.lo.x;
+        p.y -= whole.lo.y;
+        p.x /= (whole.hi.x - whole.lo.x);
+        p.y /= (whole.hi.y - whole.lo.y);
+        ci[i].code = morton2(p);
+        ci[i].idx = i;
+    }
+    qsort(ci, N, sizeof(code_idx), cmp_code_idx);
+    {
+        box3 *tb = (box3 *)malloc(sizeof(box3) * N);
+        cone3 *tc = (cone3 *)malloc(sizeof(cone3) * N);
+        for (i = 0; i < N; ++i) { tb[i] = bx[NI + ci[i].idx]; tc[i] = cn[NI + ci[i].idx]; }
+        for (i = 0; i < N; ++i) { bx[NI + i] = tb[i]; cn[NI + i] = tc[i]; }
+        free(tb); free(tc);
+    }
+    for (i = 0; i < N; ++i) {
+        codes[i] = ci[i].code;
+        if (sorted_codes) sorted_codes[i] = ci[i].code;
+        if (sorted_idx) sorted_idx[i] = ci[i].idx;
+    }
+    for (i = 1; i < N; ++i)
+        if (codes[i] == codes[i - 1]) { collision = 1; break; }
+    if (collision) {
+        codes64 = (uint64_t *)malloc(sizeof(uint64_t) * N);
+        for (i = 0; i < N; ++i) codes64[i] = ((uint64_t)codes[i] << 32) | ci[i].idx;
+    }
+    ks.c32 = codes;
+    ks.c64 = codes64;
+    for (i = 0; i < NN; ++i)
+        nodes[4 * i] = nodes[4 * i + 1] = nodes[4 * i + 2] = nodes[4 * i + 3] = INVALID;
+    for (i = 0; i < N; ++i) nodes[4 * (NI + i) + 3] = ci[i].idx;
+    for (i = 0; i < NI; ++i) {
+        uint32_t f, l, g, left, right;
+        determine_range(ks, N, i, &f, &l);
+        g = find_split(ks, f, l);
+        left = g;
+        right = g + 1;
+        if ((f < l ? f : l) == g) left += NI;
+        if ((f > l ? f : l) == g + 1) right += NI;
+        nodes[4 * i + 1] = left;
+        nodes[4 * i + 2] = right;
+        nodes[4 * left] = i;
+        nodes[4 * right] = i;
+    }
+    flags = (int *)calloc(NI > 0 ? NI : 1, sizeof(int));
+    for (i = NI; i < NN; ++i) {
+        uint32_t p = nodes[4 * i];
+        while (p != INVALID) {
+            if (flags[p] == 0) { flags[p] = 1; break; }
+            bx[p] = box_merge(bx[nodes[4 * p + 1]], bx[nodes[4 * p + 2]]);
+            p = nodes[4 * p];
+        }
+    }
+    memset(flags, 0, sizeof(int) * (NI > 0 ? NI : 1));
+    for (i = NI; i < NN; ++i) {
+        uint32_t p = nodes[4 * i];
+        while (p != INVALID) {
+            uint32_t l, r;
+            if (flags[p] == 0) { flags[p] = 1; break; }
+            l = nodes[4 * p + 1];
+            r = nodes[4 * p + 2];
+            cn[p] = cone_merge2d(cn[l], cn[r], box_centroid(bx[l]), box_centroid(bx[r]), box_centroid(bx[p]));
+            p = nodes[4 * p];
+        }
+    }
+    for (i = 0; i < NN; ++i) {
+        boxes[6 * i + 0] = bx[i].lo.x; boxes[6 * i + 1] = bx[i].lo.y; boxes[6 * i + 2] = bx[i].lo.z;
+        boxes[6 * i + 3] = bx[i].hi.x; boxes[6 * i + 4] = bx[i].hi.y; boxes[6 * i + 5] = bx[i].hi.z;
+        cones[5 * i + 0] = cn[i].axis.x; cones[5 * i + 1] = cn[i].axis.y; cones[5 * i + 2] = cn[i].axis.z;
+        cones[5 * i + 3] = cn[i].half_angle; cones[5 * i + 4] = cn[i].radius;
+    }
+    free(flags); free(bx); free(cn); free(ci); free(codes); free(codes64);
+    return collision;
+}
+
+/* ---- 2D primitive tests ---- */
+static float closest_point_segment2d(f2 pa, f2 pb, f2 x) /* scene.cuh:176-201 */
+{
+    f2 u = mk2(pb.x - pa.x, pb.y - pa.y), v = mk2(x.x - pa.x, x.y - pa.y), pt;
+    float c1 = dot2(u, v), c2, t;
+    if (c1 <= 0.0f) return len2(mk2(x.x - pa.x, x.y - pa.y));
+    c2 = dot2(u, u);
+    if (c2 <= c1) return len2(mk2(x.x - pb.x, x.y - pb.y));
+    t = c1 / c2;
+    pt = mk2(pa.x + u.x * t, pa.y + u.y * t);
+    return len2(mk2(x.x - pt.x, x.y - pt.y));
+}
+
+static int is_silhouette_vertex(f2 n0, f2 n1, f2 view, float d, int flip) /* scene.cuh:112-141 */
+{
+    const float precision = 1e-3f;
+    float sign = flip ? 1.0f : -1.0f, dot0, dot1;
+    f2 vu;
+    if (d <= precision) {
+        float det = n0.x * n1.y - n0.y * n1.x;
+        return sign * det > precision;
+    }
+    vu = mk2(view.x / d, view.y / d);
+    dot0 = dot2(vu, n0);
+    dot1 = dot2(vu, n1);
+    if (fabsf(dot0) <= precision) return sign * dot1 > precision;
+    if (fabsf(dot1) <= precision) return sign * dot0 > precision;
+    return dot0 * dot1 < 0.0f;
+}
+
+static int vertex_closest_silhouette(const float *V, const int32_t *s, f2 o, float max_r2, float *dist, int flip)
+{ /* scene.cuh:354-386 */
+    f2 p, view;
+    float d;
+    int sil;
+    if (0.0f >= max_r2) return 0;
+    p = vget2(V, s[1]);
+    view = mk2(o.x - p.x, o.y - p.y);
+    d = len2(view);
+    if (d * d > max_r2) return 0;
+    sil = !sv_has_face(s, 0) || !sv_has_face(s, 1);
+    if (!sil) sil = is_silhouette_vertex(sv_face_normal(V, s, 0, 1), sv_face_normal(V, s, 1, 1), view, d, flip);
+    if (sil && d * d <= max_r2) {
+        *dist = d;
+        return 1;
+    }
+    return 0;
+}
+
+static int ray_segment(f2 o, f2 dir, f2 p0, f2 p1, float *t_out, float *s_out) /* scene.cuh:545-576 */
+{
+    f2 sd = mk2(p1.x - p0.x, p1.y - p0.y);
+    float D = dir.x * (-sd.y) + dir.y * sd.x, inv, t, s;
+    if (fabsf(D) < F_EPS) return 0;
+    inv = 1.0f / D;
+    t = ((p0.x - o.x) * (-sd.y) - (p0.y - o.y) * (-sd.x)) * inv;
+    s = (dir.x * (p0.y - o.y) - dir.y * (p0.x - o.x)) * inv;
+    if (s >= -1e-3f && s <= 1.0f + 1e-3f && t >= 0.0f) {
+        *t_out = t;
+        *s_out = s;
+        return 1;
+    }
+    return 0;
+}
+
+static int sphere_segment(f2 p1, f2 p2, f2 c, float radius) /* scene.cuh:581-602 */
+{
+    f2 d = mk2(p2.x - p1.x, p2.y - p1.y), cp;
+    float len_sq = d.x * d.x + d.y * d.y;
+    float t = ((c.x - p1.x) * d.x + (c.y - p1.y) * d.y) / len_sq, dx, dy;
+    t = smax(0.0f, smin(1.0f, t));
+    cp = mk2(p1.x + t * d.x, p1.y + t * d.y);
+    dx = cp.x - c.x;
+    dy = cp.y - c.y;
+    return dx * dx + dy * dy <= (radius * radius);
+}
+
+static float green_weight2d(f2 x, f2 y) /* scene.cuh:605-612 */
+{
+    float r = fmaxf(len2(mk2(x.x - y.x, x.y - y.y)), 1e-2f);
+    return fabsf(odm_logf(r) / (F_PI * 2.0f));
+}
+
+/* ---- 2D queries in the reference's visit order ---- */
+typedef struct {
+    uint32_t n_obj;
+    const uint32_t *nodes;
+    const float *boxes;
+    const float *cones;
+    const float *V; /* 2 floats per vertex */
+    const int32_t *S;
+    const int32_t *slots;
+    const int32_t *owned;
+} orc_tree2d;
+
+static inline box3 tbox2(const orc_tree2d *t, uint32_t i)
+{
+    box3 b;
+    const float *p = t->boxes + 6 * (size_t)i;
+    b.lo = mk(p[0], p[1], p[2]);
+    b.hi = mk(p[3], p[4], p[5]);
+    return b;
+}
+static inline cone3 tcone2(const orc_tree2d *t, uint32_t i)
+{
+    cone3 c;
+    const float *p = t->cones + 5 * (size_t)i;
+    c.axis = mk(p[0], p[1], p[2]);
+    c.half_angle = p[3];
+    c.radius = p[4];
+    return c;
+}
+static float seg_dist(const orc_tree2d *t, uint32_t obj, f2 q)
+{
+    return closest_point_segment2d(vget2(t->V, t->S[2 * obj]), vget2(t->V, t->S[2 * obj + 1]), q);
+}
+/* box mindist / minmaxdist in 2D (aabb.cuh:130-135, 160-180) */
+static inline float mindist2(box3 b, f2 p)
+{
+    float dx = fminf(b.hi.x, fmaxf(b.lo.x, p.x)) - p.x;
+    float dy = fminf(b.hi.y, fmaxf(b.lo.y, p.y)) - p.y;
+    return dx * dx + dy * dy;
+}
+static inline float minmaxdist2(box3 b, f2 p)
+{
+    float lx = b.lo.x - p.x, ly = b.lo.y - p.y, ux = b.hi.x - p.x, uy = b.hi.y - p.y;
+    float rmx = lx * lx, rmy = ly * ly, rMx = ux * ux, rMy = uy * uy, t;
+    if ((b.hi.x + b.lo.x) * 0.5f < p.x) { t = rmx; rmx = rMx; rMx = t; }
+    if ((b.hi.y + b.lo.y) * 0.5f < p.y) { t = rmy; rmy = rMy; rMy = t; }
+    return fminf(rmx + rMy, rMx + rmy);
+}
+
+void orc_closest_point2d(const orc_tree2d *t, const float *q, int64_t n, uint32_t *idx_out, float *dist_out)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (qi = 0; qi < n; ++qi) {
+        f2 p = mk2(q[2 * qi], q[2 * qi + 1]);
+        uint32_t sn[128];
+        float sd[128];
+        int sp = 0;
+        uint32_t nearest = INVALID;
+        float best = INFINITY;
+        if (t->n_obj == 1) {
+            float d = seg_dist(t, t->nodes[3], p);
+            d *= d;
+            idx_out[qi] = t->nodes[3];
+            dist_out[qi] = sqrtf(d);
+            continue;
+        }
+        sn[sp] = 0; sd[sp] = mindist2(tbox2(t, 0), p); ++sp;
+        do {
+            uint32_t node, L, R, o;
+            float nd, Lmd, Rmd, Lmm, Rmm;
+            --sp;
+            node = sn[sp];
+            nd = sd[sp];
+            if (nd > best) continue;
+            L = t->nodes[4 * node + 1];
+            R = t->nodes[4 * node + 2];
+            Lmd = mindist2(tbox2(t, L), p);
+            Rmd = mindist2(tbox2(t, R), p);
+            Lmm = minmaxdist2(tbox2(t, L), p);
+            Rmm = minmaxdist2(tbox2(t, R), p);
+            if (Lmd <= Rmm) {
+                o = t->nodes[4 * L + 3];
+                if (o != INVALID) {
+                    float d = seg_dist(t, o, p);
+                    d *= d;
+                    if (d <= best) { best = d; nearest = o; }
+                } else { sn[sp] = L; sd[sp] = Lmd; ++sp; }
+            }
+            if (Rmd <= Lmm) {
+                o = t->nodes[4 * R + 3];
+                if (o != INVALID) {
+                    float d = seg_dist(t, o, p);
+                    d *= d;
+                    if (d <= best) { best = d; nearest = o; }
+                } else { sn[sp] = R; sd[sp] = Rmd; ++sp; }
+            }
+        } while (sp > 0);
+        idx_out[qi] = nearest;
+        dist_out[qi] = sqrtf(best);
+    }
+}
+
+static int sil_leaf2d(const orc_tree2d *t, uint32_t obj, f2 q, float max_r2, float *dist, uint32_t *sil, int flip)
+{ /* scene.cuh:517-541 */
+    int k, ret = 0;
+    float mr2 = max_r2;
+    for (k = 0; k < 2; ++k) {
+        int32_t v = t->owned[2 * (size_t)obj + k];
+        if (v != -1 && vertex_closest_silhouette(t->V, t->slots + 4 * (size_t)v, q, mr2, dist, flip)) {
+            ret = 1;
+            mr2 = (*dist) * (*dist);
+            *sil = (uint32_t)v;
+        }
+    }
+    return ret;
+}
+
+void orc_closest_silhouette2d(const orc_tree2d *t, const float *q, int64_t n, int flip, uint32_t *sil_out,
+                              float *dist_out)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (qi = 0; qi < n; ++qi) {
+        f2 p = mk2(q[2 * qi], q[2 * qi + 1]);
+        uint32_t sn[128], best_s = INVALID;
+        float sd[128], best = INFINITY;
+        int sp = 0;
+        if (t->n_obj == 1) {
+            float d = INFINITY;
+            uint32_t s = INVALID;
+            if (cone_valid(tcone2(t, 0)) && cone_overlap2d(tcone2(t, 0), p, tbox2(t, 0), mindist2(tbox2(t, 0), p)) &&
+                sil_leaf2d(t, t->nodes[3], p, best * best, &d, &s, flip) && d <= best) {
+                best = d;
+                best_s = s;
+            }
+            sil_out[qi] = best_s;
+            dist_out[qi] = best;
+            continue;
+        }
+        sn[sp] = 0; sd[sp] = mindist2(tbox2(t, 0), p); ++sp;
+        do {
+            uint32_t node, c[2];
+            float nd, md[2];
+            int hit[2], s;
+            box3 b[2];
+            --sp;
+            node = sn[sp];
+            nd = sd[sp];
+            if (nd > best * best) continue;
+            c[0] = t->nodes[4 * node + 1];
+            c[1] = t->nodes[4 * node + 2];
+            for (s = 0; s < 2; ++s) {
+                b[s] = tbox2(t, c[s]);
+                md[s] = mindist2(b[s], p);
+                hit[s] = cone_valid(tcone2(t, c[s])) && cone_overlap2d(tcone2(t, c[s]), p, b[s], md[s]);
+            }
+            for (s = 0; s < 2; ++s) {
+                uint32_t o;
+                if (!hit[s]) continue;
+                o = t->nodes[4 * c[s] + 3];
+                if (o != INVALID) {
+                    float d = INFINITY;
+                    uint32_t sv = INVALID;
+                    if (sil_leaf2d(t, o, p, best * best, &d, &sv, flip) && d <= best) {
+                        best = d;
+                        best_s = sv;
+                    }
+                } else { sn[sp] = c[s]; sd[sp] = md[s]; ++sp; }
+            }
+        } while (sp > 0);
+        sil_out[qi] = best_s;
+        dist_out[qi] = best;
+    }
+}
+
+static inline int box_ray2(box3 b, f2 o, f2 inv, float max_dist, float *dist) /* aabb.cuh:366-395 */
+{
+    float t1 = (b.lo.x - o.x) * inv.x, t2 = (b.hi.x - o.x) * inv.x;
+    float tmin = fminf(t1, t2), tmax = fmaxf(t1, t2);
+    t1 = (b.lo.y - o.y) * inv.y;
+    t2 = (b.hi.y - o.y) * inv.y;
+    tmin = fmaxf(tmin, fminf(t1, t2));
+    tmax = fminf(tmax, fmaxf(t1, t2));
+    if (tmax >= tmin && tmax >= 0.0f && tmin <= max_dist) {
+        *dist = tmin >= 0.0f ? tmin : 0.0f;
+        return 1;
+    }
+    return 0;
+}
+
+void orc_ray2d(const orc_tree2d *t, const float *o, const float *d, const float *tmax, int64_t n, uint32_t *idx_out,
+               float *t_out, float *s_out, uint8_t *hit_out)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (qi = 0; qi < n; ++qi) {
+        f2 ro = mk2(o[2 * qi], o[2 * qi + 1]), rd = mk2(d[2 * qi], d[2 * qi + 1]);
+        f2 inv = mk2(1.0f / rd.x, 1.0f / rd.y);
+        float max_dist = tmax[qi], min_dist = INFINITY, ss = 0.0f;
+        uint32_t sn[128], nearest = INVALID;
+        float sd[128];
+        int sp = 0, found = 0;
+        sn[sp] = 0; sd[sp] = INFINITY; ++sp;
+        do {
+            uint32_t node, obj;
+            float nt;
+            --sp;
+            node = sn[sp];
+            nt = sd[sp];
+            if (nt > min_dist) continue;
+            obj = t->nodes[4 * node + 3];
+            if (obj != INVALID) {
+                float th, sh;
+                if (ray_segment(ro, rd, vget2(t->V, t->S[2 * obj]), vget2(t->V, t->S[2 * obj + 1]), &th, &sh) &&
+                    th < max_dist && th < min_dist) {
+                    min_dist = th;
+                    found = 1;
+                    ss = sh;
+                    nearest = obj;
+                }
+            } else {
+                uint32_t L = t->nodes[4 * node + 1], R = t->nodes[4 * node + 2], cl = L, ot = R, tmp;
+                float Ld, Rd, tf;
+                int Lh = box_ray2(tbox2(t, L), ro, inv, max_dist, &Ld);
+                int Rh = box_ray2(tbox2(t, R), ro, inv, max_dist, &Rd);
+                if (Lh && Rh) {
+                    if (Rd < Ld) { tf = Ld; Ld = Rd; Rd = tf; tmp = cl; cl = ot; ot = tmp; }
+                    sn[sp] = ot; sd[sp] = Rd; ++sp;
+                    sn[sp] = cl; sd[sp] = Ld; ++sp;
+                } else if (Lh) { sn[sp] = L; sd[sp] = Ld; ++sp; }
+                else if (Rh) { sn[sp] = R; sd[sp] = Rd; ++sp; }
+            }
+        } while (sp > 0);
+        hit_out[qi] = (uint8_t)found;
+        idx_out[qi] = nearest;
+        t_out[qi] = min_dist;
+        s_out[qi] = ss;
+    }
+}
+
+void orc_sample2d(const orc_tree2d *t, const float *c, const float *r, const float *u, int64_t n, int32_t *idx_out,
+                  float *pdf_out, float *pt_out)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(static)
+    for (qi = 0; qi < n; ++qi) {
+        f2 cc = mk2(c[2 * qi], c[2 * qi + 1]);
+        float rr = r[qi], uu = u[3 * qi], npdf = 1.0f, pdf = 0.0f;
+        uint32_t node = 0;
+        int32_t obj_index = -1;
+        for (;;) {
+            uint32_t obj = t->nodes[4 * node + 3];
+            if (obj != INVALID) {
+                f2 p0 = vget2(t->V, t->S[2 * obj]), p1 = vget2(t->V, t->S[2 * obj + 1]);
+                if (sphere_segment(p0, p1, cc, rr)) {
+                    obj_index = (int32_t)obj;
+                    pdf = npdf;
+                    pdf /= len2(mk2(p0.x - p1.x, p0.y - p1.y)); /* scene.cuh:409-418 */
+                }
+                break;
+            } else {
+                uint32_t L = t->nodes[4 * node + 1], R = t->nodes[4 * node + 2];
+                box3 Lb = tbox2(t, L), Rb = tbox2(t, R);
+                f3 cl3 = box_centroid(Lb), cr3 = box_centroid(Rb);
+                float Lw = box_sphere(Lb, mk(cc.x, cc.y, 0.0f), rr) ? green_weight2d(cc, mk2(cl3.x, cl3.y)) : 0.0f;
+                float Rw = box_sphere(Rb, mk(cc.x, cc.y, 0.0f), rr) ? green_weight2d(cc, mk2(cr3.x, cr3.y)) : 0.0f;
+                float tw = Lw + Rw;
+                if (tw > 0.0f) {
+                    float Lp = Lw / tw;
+                    if (uu < Lp) {
+                        uu /= Lp;
+                        npdf = Lp * npdf;
+                        node = L;
+                    } else {
+                        float Rp = 1.0f - Lp;
+                        uu = (uu - Lp) / Rp;
+                        npdf = Rp * npdf;
+                        node = R;
+                    }
+                } else break;
+            }
+        }
+        idx_out[qi] = obj_index;
+        pdf_out[qi] = obj_index >= 0 ? pdf : 0.0f;
+        if (pt_out) {
+            float px = 0.0f, py = 0.0f;
+            if (obj_index >= 0) { /* sample_line, scene.cuh:29-32 */
+                f2 pa = vget2(t->V, t->S[2 * obj_index]), pb = vget2(t->V, t->S[2 * obj_index + 1]);
+                float uv = u[3 * qi + 1];
+                px = pa.x + uv * (pb.x - pa.x);
+                py = pa.y + uv * (pb.y - pa.y);
+            }
+            pt_out[2 * qi] = px;
+            pt_out[2 * qi + 1] = py;
+        }
+    }
+}
+
+/* FP64 brute force, 2D */
+void orc_bf_closest_point2d(const float *V, const int32_t *S, int32_t nS, const float *q, int64_t n, uint32_t *idx,
+                            double *dist)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (qi = 0; qi < n; ++qi) {
+        double px = q[2 * qi], py = q[2 * qi + 1], best = INFINITY;
+        uint32_t bi = INVALID;
+        int32_t s;
+        for (s = 0; s < nS; ++s) {
+            double ax = V[2 * S[2 * s]], ay = V[2 * S[2 * s] + 1], bx = V[2 * S[2 * s + 1]], by = V[2 * S[2 * s + 1] + 1];
+            double ux = bx - ax, uy = by - ay, c1 = ux * (px - ax) + uy * (py - ay), c2 = ux * ux + uy * uy, t, dx, dy, d;
+            t = c1 <= 0 ? 0.0 : (c2 <= c1 ? 1.0 : c1 / c2);
+            dx = px - (ax + t * ux);
+            dy = py - (ay + t * uy);
+            d = sqrt(dx * dx + dy * dy);
+            if (d < best) { best = d; bi = (uint32_t)s; }
+        }
+        idx[qi] = bi;
+        dist[qi] = best;
+    }
+}
+
+void orc_bf_closest_silhouette2d(const float *V, const int32_t *slots, int32_t nV, const float *q, int64_t n, int flip,
+                                 uint32_t *sil, double *dist)
+{
+    int64_t qi;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (qi = 0; qi < n; ++qi) {
+        double px = q[2 * qi], py = q[2 * qi + 1], best = INFINITY, sign = flip ? 1.0 : -1.0;
+        uint32_t bs = INVALID;
+        int32_t v;
+        for (v = 0; v < nV; ++v) {
+            const int32_t *s = slots + 4 * v;
+            double vx, vy, d, n0x, n0y, n1x, n1y, l0, l1;
+            int is;
+            if (s[1] == -1) continue;
+            vx = px - V[2 * s[1]];
+            vy = py - V[2 * s[1] + 1];
+            d = sqrt(vx * vx + vy * vy);
+            if (d >= best) continue;
+            if (s[0] == -1 || s[2] == -1) is = 1;
+            else {
+                n0x = (double)V[2 * s[2] + 1] - V[2 * s[1] + 1];
+                n0y = -((double)V[2 * s[2]] - V[2 * s[1]]);
+                n1x = (double)V[2 * s[1] + 1] - V[2 * s[0] + 1];
+                n1y = -((double)V[2 * s[1]] - V[2 * s[0]]);
+                l0 = sqrt(n0x * n0x + n0y * n0y);
+                l1 = sqrt(n1x * n1x + n1y * n1y);
+                n0x /= l0; n0y /= l0; n1x /= l1; n1y /= l1;
+                if (d <= 1e-3) is = sign * (n0x * n1y - n0y * n1x) > 1e-3;
+                else {
+                    double d0 = (vx * n0x + vy * n0y) / d, d1 = (vx * n1x + vy * n1y) / d;
+                    if (fabs(d0) <= 1e-3) is = sign * d1 > 1e-3;
+                    else if (fabs(d1) <= 1e-3) is = sign * d0 > 1e-3;
+                    else is = d0 * d1 < 0;
+                }
+            }
+            if (is) { best = d; bs = (uint32_t)v; }
+        }
+        sil[qi] = bs;
+        dist[qi] = best;
+    }
+}

@@ -62,6 +62,8 @@ EXPORTED = [
     "snch_ray", "snch_ray_ex",
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
+    "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
+    "snch_mlp_gradient", "snch_mlp_predict",
 ]
 
 
@@ -86,9 +88,16 @@ def _declare(L):
     L.snch_sample_on_object.argtypes = [vp, vp, vp, u32, vp, vp]
     L.snch_scene_export.argtypes = [vp, vp, vp, vp, vp, vp]
     L.snch_scene_silhouettes.argtypes = [vp, vp, vp]
+    L.snch_mlp_create.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.snch_mlp_destroy.argtypes = [vp]
+    L.snch_mlp_params.argtypes = [vp, vp, vp, vp, vp]
+    L.snch_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, ctypes.c_float, vp, vp]
+    L.snch_mlp_gradient.argtypes = [vp, vp, vp, u32, vp, vp, vp]
+    L.snch_mlp_predict.argtypes = [vp, vp, u32, c_int, ctypes.c_float, ctypes.c_float, vp, vp]
     for name in EXPORTED:
         if name != "snch_last_error":
             getattr(L, name).restype = i32
+    L.snch_mlp_num_params.restype = u32
 
 
 def check(rc):
@@ -129,6 +138,8 @@ def _dev_f32(x, cols, device):
 class Scene:
     """``lbvh::scene<3>`` on one GPU (scene.cuh:705-1268)."""
 
+    dim = 3
+
     def __init__(self, vertices, indices, device=0):
         import torch
 
@@ -136,12 +147,13 @@ class Scene:
             raise RuntimeError("snch_lbvh_amd needs a ROCm GPU; none is visible")
         self.device = device
         torch.cuda.set_device(device)
-        V = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, 3))
-        F = np.ascontiguousarray(np.asarray(indices, np.int32).reshape(-1, 3))
+        d = self.dim
+        V = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, d))
+        F = np.ascontiguousarray(np.asarray(indices, np.int32).reshape(-1, d))
         self.V, self.F = V, F
         h = ctypes.c_void_p()
-        check(lib().snch_scene_create_tri(context(device), V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0],
-                                          ctypes.byref(h)))
+        create = lib().snch_scene_create_tri if d == 3 else lib().snch_scene_create_seg
+        check(create(context(device), V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0], ctypes.byref(h)))
         self._h = h
         dim, n_prims, n_sil = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().snch_scene_info(h, ctypes.byref(dim), ctypes.byref(n_prims), ctypes.byref(n_sil)))
@@ -175,7 +187,7 @@ class Scene:
         evaluations, edge tests, nodes visited), accumulated."""
         import torch
 
-        q = _dev_f32(q, 3, self.device)
+        q = _dev_f32(q, self.dim, self.device)
         n = q.shape[0]
         idx = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
@@ -186,7 +198,7 @@ class Scene:
     def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None):
         import torch
 
-        q = _dev_f32(q, 3, self.device)
+        q = _dev_f32(q, self.dim, self.device)
         n = q.shape[0]
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
@@ -199,13 +211,13 @@ class Scene:
     def ray_intersect(self, o, d, tmax=None, any_hit=False, stats=None):
         import torch
 
-        o = _dev_f32(o, 3, self.device)
-        d = _dev_f32(d, 3, self.device)
+        o = _dev_f32(o, self.dim, self.device)
+        d = _dev_f32(d, self.dim, self.device)
         n = o.shape[0]
         tm = _dev_f32(tmax, 0, self.device).reshape(-1) if tmax is not None else None
         hit = torch.empty(n, dtype=torch.uint8, device=o.device)
         t = torch.empty(n, dtype=torch.float32, device=o.device)
-        uv = torch.empty((n, 2), dtype=torch.float32, device=o.device)
+        uv = torch.empty((n, 2) if self.dim == 3 else (n,), dtype=torch.float32, device=o.device)
         idx = torch.empty(n, dtype=torch.int32, device=o.device)
         opts = QueryOpts(0, 0, stats.data_ptr() if stats is not None else None)
         check(lib().snch_ray_ex(self._h, _ptr(o), _ptr(d), _ptr(tm), n, int(bool(any_hit)), _ptr(hit), _ptr(t),
@@ -215,13 +227,13 @@ class Scene:
     def sample_in_sphere(self, c, r, u, with_point=True):
         import torch
 
-        c = _dev_f32(c, 3, self.device)
+        c = _dev_f32(c, self.dim, self.device)
         n = c.shape[0]
         r = _dev_f32(r, 0, self.device).reshape(-1)
         u = _dev_f32(u, 3, self.device)
         idx = torch.empty(n, dtype=torch.int32, device=c.device)
         pdf = torch.empty(n, dtype=torch.float32, device=c.device)
-        pt = torch.empty((n, 3), dtype=torch.float32, device=c.device) if with_point else None
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=c.device) if with_point else None
         check(lib().snch_sample_in_sphere(self._h, _ptr(c), _ptr(r), _ptr(u), n, _ptr(idx), _ptr(pdf), _ptr(pt),
                                           _stream()))
         return idx, pdf, pt
@@ -231,7 +243,7 @@ class Scene:
 
         idx = torch.as_tensor(idx, dtype=torch.int32, device=f"cuda:{self.device}").contiguous()
         uv = _dev_f32(uv, 2, self.device)
-        pt = torch.empty((idx.shape[0], 3), dtype=torch.float32, device=idx.device)
+        pt = torch.empty((idx.shape[0], self.dim), dtype=torch.float32, device=idx.device)
         check(lib().snch_sample_on_object(self._h, _ptr(idx), _ptr(uv), idx.shape[0], _ptr(pt), _stream()))
         return pt
 
@@ -250,6 +262,80 @@ class Scene:
 
     def silhouettes(self):
         slots = np.empty((self.n_silhouettes, 4), np.int32)
-        owned = np.empty((self.n_prims, 3), np.int32)
+        owned = np.empty((self.n_prims, self.dim), np.int32)
         check(lib().snch_scene_silhouettes(self._h, slots.ctypes.data, owned.ctypes.data))
         return slots, owned
+
+
+class Scene2D(Scene):
+    """``lbvh::scene<2>``: line segments with silhouette vertices (scene.cuh:287-703)."""
+
+    dim = 2
+
+
+class LearnedBound:
+    """Fourier-feature MLP (BF16 MFMA) that predicts a closest-silhouette
+    distance and turns it into a seed radius for Scene.closest_silhouette.
+    No counterpart in the reference; it never changes a query result."""
+
+    def __init__(self, lo, hi, sigma=4.0, seed=0, device=0):
+        self.device = device
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        h = ctypes.c_void_p()
+        check(lib().snch_mlp_create(context(device), lo.ctypes.data, hi.ctypes.data, float(sigma), int(seed),
+                                    ctypes.byref(h)))
+        self._h = h
+        self.n_params = lib().snch_mlp_num_params()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.snch_mlp_destroy(h)
+            self._h = None
+
+    def params(self):
+        th = np.empty(self.n_params, np.float32)
+        four = np.empty(96, np.float32)
+        frame = np.empty(8, np.float32)
+        check(lib().snch_mlp_params(self._h, th.ctypes.data, None, four.ctypes.data, frame.ctypes.data))
+        return th, four, frame
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == self.n_params
+        check(lib().snch_mlp_params(self._h, None, th.ctypes.data, None, None))
+
+    def train(self, pts, dist, batch=16384, epochs=1, lr=1e-3):
+        p = _dev_f32(pts, 3, self.device)
+        d = _dev_f32(dist, 0, self.device).reshape(-1)
+        losses = np.zeros(epochs, np.float32)
+        check(lib().snch_mlp_train(self._h, _ptr(p), _ptr(d), p.shape[0], batch, epochs, float(lr),
+                                   losses.ctypes.data, _stream()))
+        return losses
+
+    def gradient(self, pts, dist):
+        p = _dev_f32(pts, 3, self.device)
+        d = _dev_f32(dist, 0, self.device).reshape(-1)
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        check(lib().snch_mlp_gradient(self._h, _ptr(p), _ptr(d), p.shape[0], g.ctypes.data, ctypes.byref(loss),
+                                      _stream()))
+        return g, loss.value
+
+    def predict(self, pts, out=None):
+        import torch
+
+        p = _dev_f32(pts, 3, self.device)
+        out = torch.empty(p.shape[0], dtype=torch.float32, device=p.device) if out is None else out
+        check(lib().snch_mlp_predict(self._h, _ptr(p), p.shape[0], 0, 0.0, 0.0, _ptr(out), _stream()))
+        return out
+
+    def seed_radius(self, pts, rel=0.1, abs_margin=0.002, out=None):
+        import torch
+
+        p = _dev_f32(pts, 3, self.device)
+        out = torch.empty(p.shape[0], dtype=torch.float32, device=p.device) if out is None else out
+        check(lib().snch_mlp_predict(self._h, _ptr(p), p.shape[0], 1, float(rel), float(abs_margin), _ptr(out),
+                                     _stream()))
+        return out

@@ -190,6 +190,139 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
     }
 }
 
+// scene<2>: leaf box (scene.cuh:420-433) and leaf cone from the owned
+// silhouette vertices (scene.cuh:435-502); slots are per owned silhouette
+// (prev, v, next, -1).  Also writes the packed silhouette-vertex records.
+__global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, const int32_t* __restrict__ S,
+                                                  const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
+                                                  uint32_t nS, float4* __restrict__ leafbox,
+                                                  float4* __restrict__ leafcone, float4* __restrict__ tedge,
+                                                  int* __restrict__ scene_bits)
+{
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    box3 b;
+    b.lo = mk(F_INF, F_INF, 0.0f);
+    b.hi = mk(-F_INF, -F_INF, 0.0f);
+    if (i < nS) {
+        const v3 p0 = ld3(V, S[2 * i]), p1 = ld3(V, S[2 * i + 1]);
+        b = seg_box(p0, p1);
+        const v3 c = box_centroid(b);
+        float ax = 0.0f, ay = 0.0f, ha = F_PI, rad = 0.0f;
+        bool any = false, all_two = true;
+        const uint32_t e0 = estart[i], e1 = estart[i + 1];
+        float nn[2][2][2];
+        for (uint32_t k = 0; k < 2; ++k) {
+            if (e0 + k >= e1)
+                break;
+            const uint32_t e = e0 + k;
+            const int4 s = slots[e];
+            const bool has0 = s.z != -1, has1 = s.x != -1; // scene.cuh:322-325
+            const v3 pv = ld3(V, s.y);
+            float r0x = 0.0f, r0y = 0.0f, r1x = 0.0f, r1y = 0.0f, nx = 0.0f, ny = 0.0f;
+            if (has0) { // normal(0): segment v -> next (scene.cuh:326-336)
+                const v3 q = ld3(V, s.z);
+                r0x = q.y - pv.y;
+                r0y = -(q.x - pv.x);
+                nx = nx + r0x;
+                ny = ny + r0y;
+            }
+            if (has1) { // normal(1): segment prev -> v
+                const v3 q = ld3(V, s.x);
+                r1x = pv.y - q.y;
+                r1y = -(pv.x - q.x);
+                nx = nx + r1x;
+                ny = ny + r1y;
+            }
+            float l = len2(nx, ny);
+            nx = nx / l;
+            ny = ny / l;
+            ax += nx;
+            ay += ny;
+            rad = smax(rad, len2(pv.x - c.x, pv.y - c.y));
+            all_two = all_two && has0 && has1;
+            any = true;
+            l = len2(r0x, r0y);
+            nn[k][0][0] = has0 ? r0x / l : 0.0f;
+            nn[k][0][1] = has0 ? r0y / l : 0.0f;
+            l = len2(r1x, r1y);
+            nn[k][1][0] = has1 ? r1x / l : 0.0f;
+            nn[k][1][1] = has1 ? r1y / l : 0.0f;
+            tedge[4 * e + 0] = make_float4(pv.x, pv.y, 0.0f, __int_as_float(s.y));
+            tedge[4 * e + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((has0 ? 1u : 0u) | (has1 ? 2u : 0u)));
+            tedge[4 * e + 2] = make_float4(nn[k][0][0], nn[k][0][1], 0.0f, 0.0f);
+            tedge[4 * e + 3] = make_float4(nn[k][1][0], nn[k][1][1], 0.0f, 0.0f);
+        }
+        if (!any) {
+            ha = -F_PI;
+        } else if (!all_two) {
+            ha = F_PI;
+        } else {
+            const float an = len2(ax, ay);
+            if (an > F_EPS) {
+                ax /= an;
+                ay /= an;
+                ha = 0.0f;
+                for (uint32_t k = 0; k < 2; ++k) {
+                    if (e0 + k >= e1)
+                        break;
+                    for (int jj = 0; jj < 2; ++jj)
+                        ha = smax(ha, dm::acosf_(clamp11(dot2(ax, ay, nn[k][jj][0], nn[k][jj][1]))));
+                }
+            }
+        }
+        leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, 0.0f, 0.0f);
+        leafbox[2 * i + 1] = make_float4(b.hi.x, b.hi.y, 0.0f, 0.0f);
+        leafcone[2 * i] = make_float4(ax, ay, 0.0f, ha);
+        leafcone[2 * i + 1] = make_float4(rad, 0.0f, 0.0f, 0.0f);
+    }
+    float v[4] = {b.lo.x, b.lo.y, b.hi.x, b.hi.y};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v[0] = fmn(v[0], __shfl_xor(v[0], o, 64));
+        v[1] = fmn(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = fmx(v[2], __shfl_xor(v[2], o, 64));
+        v[3] = fmx(v[3], __shfl_xor(v[3], o, 64));
+    }
+    __shared__ float red[BT / 64][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            red[w][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        float acc = red[0][k];
+        for (int q = 1; q < BT / 64; ++q)
+            acc = k < 2 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
+        if (k < 2)
+            atomicMin(scene_bits + k, f2ord(acc));
+        else
+            atomicMax(scene_bits + 3 + (k - 2), f2ord(acc));
+    }
+}
+
+// bvh.cuh:269-277 and morton_code.cuh:85-92
+__global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafbox,
+                                                 const int* __restrict__ scene_bits, uint32_t n,
+                                                 uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n)
+        return;
+    const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]);
+    const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]);
+    const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
+    float px = (c.x + a.x) * 0.5f, py = (c.y + a.y) * 0.5f;
+    px -= lox;
+    py -= loy;
+    px /= (hix - lox);
+    py /= (hiy - loy);
+    codes[i] = morton2(px, py);
+    idx[i] = i;
+}
+
 // bvh.cuh:292-302 and morton_code.cuh:61-70
 __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ scene_bits,
                                                uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
@@ -305,6 +438,7 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
     return r;
 }
 
+template <int DIM>
 __global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
                                               const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
@@ -326,11 +460,18 @@ __global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, cons
     ncone[2 * node] = leafcone[2 * orig];
     ncone[2 * node + 1] = leafcone[2 * orig + 1];
     {
-        const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], c = V[F[3 * orig + 2]];
         const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
-        tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
-        tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
-        tleaf[3 * j + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(ec));
+        if (DIM == 3) {
+            const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], c = V[F[3 * orig + 2]];
+            tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
+            tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
+            tleaf[3 * j + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(ec));
+        } else {
+            const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
+            tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(orig));
+            tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(es));
+            tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
+        }
     }
     uint32_t p = parent[node];
     // the climb is at most the tree depth (<= 64 key bits); the cap only
@@ -343,7 +484,10 @@ __global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, cons
         const box3 lb = ldbox(nbox, ch.x), rb = ldbox(nbox, ch.y);
         const cone3 lc = ldcone(ncone, ch.x), rc = ldcone(ncone, ch.y);
         const box3 pb = box_merge(lb, rb);
-        const cone3 pc = cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0);
+        const cone3 pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
+                                               conservative != 0)
+                                  : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
+                                                 conservative != 0);
         nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
         nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
         ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
@@ -367,9 +511,9 @@ inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
 
 } // namespace
 
-hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st)
+hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
 {
-    const uint32_t N = s->nF, NN = 2 * N - 1, E = s->nE;
+    const uint32_t N = s->nF, NN = 2 * N - 1, E = s->dim == 3 ? s->nE : s->n_internal_sil;
     const int conservative = (flags & SNCH_BUILD_REFERENCE_RADIUS) ? 0 : 1;
     BuildScratch& w = s->scratch;
     hipError_t e;
@@ -396,14 +540,22 @@ hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st)
     SNCH_TRY(s->ncone.ensure(sizeof(float4) * 2 * NN));
 
     hipLaunchKernelGGL(k_init_scene_box, dim3(1), dim3(64), 0, st, w.scene_bits.as<int>());
-    hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
-                       s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
-                       w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
-    hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(), w.scene_bits.as<int>(),
-                       N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+    if (s->dim == 3) {
+        hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                           s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
+        hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+    } else {
+        hipLaunchKernelGGL(k_leafgen2d, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                           s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, w.leafbox.as<float4>(),
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
+        hipLaunchKernelGGL(k_morton2d, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+    }
     bool in_alt = false;
     SNCH_TRY(radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
-                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt));
+                              w.idx_alt.as<uint32_t>(), N, s->dim == 3 ? 30 : 20, w.hist, st, &in_alt));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
     if (N > 1)
@@ -411,11 +563,18 @@ hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st)
                            w.child.as<uint2>(), w.flags.as<uint32_t>());
     else
         hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>());
-    hipLaunchKernelGGL(k_refit, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
-                       s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
-                       w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
-                       s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(),
-                       s->tleaf.as<float4>());
+    if (s->dim == 3)
+        hipLaunchKernelGGL(k_refit<3>, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                           s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
+                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
+                           s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
+                           s->tcone.as<float4>(), s->tleaf.as<float4>());
+    else
+        hipLaunchKernelGGL(k_refit<2>, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
+                           s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
+                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
+                           s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
+                           s->tcone.as<float4>(), s->tleaf.as<float4>());
     s->sorted_in_alt = in_alt;
 #undef SNCH_TRY
     return hipGetLastError();

@@ -249,15 +249,79 @@ int snch_scene_create_tri(snch_context* ctx, const float* V, uint32_t nV, const 
 int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const int32_t* S, uint32_t nS,
                           snch_scene** out)
 {
-    (void)ctx;
-    (void)V;
-    (void)nV;
-    (void)S;
-    (void)nS;
-    if (out)
-        *out = nullptr;
-    set_error("snch_scene_create_seg: 2D scenes are not available in this build");
-    return SNCH_ERR_INVALID;
+    SNCH_CHECK_ARG(ctx && out, "snch_scene_create_seg: null context or output");
+    *out = nullptr;
+    if (nS == 0) {
+        set_error("snch_scene_create_seg: empty mesh");
+        return SNCH_ERR_EMPTY;
+    }
+    SNCH_CHECK_ARG(V && S && nV > 0, "snch_scene_create_seg: null vertex or index array");
+    SNCH_CHECK_ARG(nS < 0x40000000u, "snch_scene_create_seg: too many segments (limit 2^30)");
+    for (size_t k = 0; k < 2 * (size_t)nV; ++k)
+        SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_seg: non-finite vertex coordinate");
+    for (size_t k = 0; k < 2 * (size_t)nS; ++k)
+        SNCH_CHECK_ARG(S[k] >= 0 && (uint32_t)S[k] < nV, "snch_scene_create_seg: vertex index out of range");
+    SNCH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto* s = new snch_scene();
+    s->ctx = ctx;
+    s->dim = 2;
+    s->nV = nV;
+    s->nF = nS;
+    s->V.assign(V, V + 2 * (size_t)nV);
+    s->F.assign(S, S + 2 * (size_t)nS);
+    // silhouette vertices (scene.cuh:631-652): (prev, v, next, -1) per vertex
+    s->slots.assign(4 * (size_t)nV, -1);
+    for (uint32_t i = 0; i < nS; ++i) {
+        const int32_t a = S[2 * i], b = S[2 * i + 1];
+        s->slots[4 * (size_t)a + 1] = a;
+        s->slots[4 * (size_t)a + 2] = b;
+        s->slots[4 * (size_t)b + 0] = a;
+        s->slots[4 * (size_t)b + 1] = b;
+    }
+    // ownership (scene.cuh:654-672); internal ids follow first ownership so a
+    // segment's owned vertices are a consecutive run
+    std::vector<unsigned char> seen(nV, 0);
+    std::vector<int32_t> internal_slots;
+    s->owned.assign(2 * (size_t)nS, -1);
+    s->estart.assign((size_t)nS + 1, 0);
+    uint32_t next = 0;
+    for (uint32_t i = 0; i < nS; ++i) {
+        int ptr = 0;
+        for (int k = 0; k < 2; ++k) {
+            const int32_t v = S[2 * i + k];
+            if (!seen[v]) {
+                seen[v] = 1;
+                s->owned[2 * (size_t)i + ptr++] = v;
+                internal_slots.insert(internal_slots.end(), &s->slots[4 * (size_t)v], &s->slots[4 * (size_t)v] + 4);
+                ++next;
+            }
+        }
+        s->estart[i + 1] = next;
+    }
+    s->nE = nV;
+    std::vector<float> V4(4 * (size_t)nV, 0.0f);
+    for (size_t k = 0; k < nV; ++k) {
+        V4[4 * k] = V[2 * k];
+        V4[4 * k + 1] = V[2 * k + 1];
+    }
+    hipError_t e;
+    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dF.ensure(sizeof(int32_t) * s->F.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dF.p, s->F.data(), sizeof(int32_t) * s->F.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(internal_slots.size(), 4))) != hipSuccess ||
+        (internal_slots.size() &&
+         (e = hipMemcpy(s->dSlots.p, internal_slots.data(), sizeof(int32_t) * internal_slots.size(),
+                        hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = s->dEstart.ensure(sizeof(uint32_t) * s->estart.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dEstart.p, s->estart.data(), sizeof(uint32_t) * s->estart.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        delete s;
+        return hip_fail(e, "snch_scene_create_seg: upload");
+    }
+    s->n_internal_sil = next;
+    *out = s;
+    return SNCH_OK;
 }
 
 int snch_scene_destroy(snch_scene* scene)
@@ -293,7 +357,7 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
     SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
     (void)hipEventRecord(e0, st);
-    hipError_t e = build_tree_3d(s, flags, st);
+    hipError_t e = build_tree(s, flags, st);
     (void)hipEventRecord(e1, st);
     hipError_t es = hipStreamSynchronize(st);
     float ms = 0.0f;
@@ -331,7 +395,7 @@ int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint3
 {
     SNCH_NEED_BUILT(s, "snch_closest_point");
     SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_closest_point: null pointer");
-    SNCH_HIP(launch_closest_point(s->view(), q, n, out_idx, out_dist, opts ? opts->stats : nullptr,
+    SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, opts ? opts->stats : nullptr,
                                   static_cast<hipStream_t>(stream)),
              "snch_closest_point");
     return SNCH_OK;
@@ -351,7 +415,7 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
-    SNCH_HIP(launch_closest_silhouette(s->view(), q, n, flip, seed, out_sil, out_dist, block,
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block,
                                        opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
              "snch_closest_silhouette");
     return SNCH_OK;
@@ -369,7 +433,7 @@ int snch_ray_ex(const snch_scene* s, const float* o, const float* d, const float
 {
     SNCH_NEED_BUILT(s, "snch_ray");
     SNCH_CHECK_ARG(n == 0 || (o && d && out_hit), "snch_ray: null pointer");
-    SNCH_HIP(launch_ray(s->view(), o, d, tmax, n, any_hit, out_hit, out_t, out_uv, out_idx,
+    SNCH_HIP(launch_ray(s->view(), (int)s->dim, o, d, tmax, n, any_hit, out_hit, out_t, out_uv, out_idx,
                         opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
              "snch_ray");
     return SNCH_OK;
@@ -386,7 +450,7 @@ int snch_sample_in_sphere(const snch_scene* s, const float* c, const float* r, c
 {
     SNCH_NEED_BUILT(s, "snch_sample_in_sphere");
     SNCH_CHECK_ARG(n == 0 || (c && r && u && out_idx && out_pdf), "snch_sample_in_sphere: null pointer");
-    SNCH_HIP(launch_sample(s->view(), s->dV.as<float4>(), s->dF.as<int32_t>(), c, r, u, n, out_idx, out_pdf, out_pt,
+    SNCH_HIP(launch_sample(s->view(), (int)s->dim, s->dV.as<float4>(), s->dF.as<int32_t>(), c, r, u, n, out_idx, out_pdf, out_pt,
                            static_cast<hipStream_t>(stream)),
              "snch_sample_in_sphere");
     return SNCH_OK;
@@ -397,7 +461,7 @@ int snch_sample_on_object(const snch_scene* s, const int32_t* idx, const float* 
 {
     SNCH_CHECK_ARG(s, "snch_sample_on_object: null scene");
     SNCH_CHECK_ARG(n == 0 || (idx && uv && out_pt), "snch_sample_on_object: null pointer");
-    SNCH_HIP(launch_sample_on_object(s->dV.as<float4>(), s->dF.as<int32_t>(), s->nF, idx, uv, n, out_pt,
+    SNCH_HIP(launch_sample_on_object((int)s->dim, s->dV.as<float4>(), s->dF.as<int32_t>(), s->nF, idx, uv, n, out_pt,
                                      static_cast<hipStream_t>(stream)),
              "snch_sample_on_object");
     return SNCH_OK;

@@ -174,5 +174,27 @@ SNCH_HD void sincosf_(float x, float* sn, float* cs)
     }
 }
 
+// log(x) for positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log(m) = 2 atanh(s) = 2 s + s^3 P(s^2), s = (m - 1) / (m + 1)
+constexpr float LN2_HI = 0.6931471824645996f;
+constexpr float LN2_LO = -1.904654323148236e-09f;
+SNCH_HD float logf_(float x)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    int e = (int)((u >> 23) & 0xFFu) - 127;
+    float m = __builtin_bit_cast(float, (u & 0x7FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356237f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float z = s * s;
+    float p = 0.2333136796951294f;
+    p = p * z + 0.28550779819488525f;
+    p = p * z + 0.4000012278556824f;
+    p = p * z + 0.6666666865348816f;
+    return (float)e * LN2_HI + ((2.0f * s + (s * z) * p) + (float)e * LN2_LO);
+}
+
 } // namespace dm
 } // namespace snch

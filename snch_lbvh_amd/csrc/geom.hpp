@@ -459,4 +459,174 @@ SNCH_HD uint32_t morton3(v3 p)
     return expand_bits((uint32_t)x) * 4u + expand_bits((uint32_t)y) * 2u + expand_bits((uint32_t)z);
 }
 
+// ---------------------------------------------------------------- 2D twins
+// scene<2> (scene.cuh:287-703).  2D boxes and cones reuse the 3D records with
+// z = 0; the 3D box tests with z = 0 give bit-identical results to the 2D
+// ones (every z term is an exact zero, and a 0 * inf slab term is a NaN that
+// fminf/fmaxf drop).
+constexpr float BVH_OFFSET = 1e-3f; // scene.cuh:12
+
+SNCH_HD float dot2(float ax, float ay, float bx, float by) { return ax * bx + ay * by; }
+SNCH_HD float len2(float x, float y) { return dm::sqrt_(x * x + y * y); }
+
+// 2D leaf box of a segment (scene.cuh:420-433)
+SNCH_HD box3 seg_box(v3 p0, v3 p1)
+{
+    box3 b;
+    b.hi = mk(fmx(p0.x, p1.x) + BVH_OFFSET, fmx(p0.y, p1.y) + BVH_OFFSET, 0.0f);
+    b.lo = mk(fmn(p0.x, p1.x) - BVH_OFFSET, fmn(p0.y, p1.y) - BVH_OFFSET, 0.0f);
+    return b;
+}
+
+// 2D Morton code (morton_code.cuh:85-92)
+SNCH_HD uint32_t morton2(float px, float py)
+{
+    const float res = 1024.0f;
+    float x = fmn(fmx(px * res, 0.0f), res - 1.0f);
+    float y = fmn(fmx(py * res, 0.0f), res - 1.0f);
+    return expand_bits((uint32_t)x) * 2u + expand_bits((uint32_t)y);
+}
+
+// 2D cone-view overlap (cone.cuh:78-121)
+SNCH_HD bool cone_overlap2d(v3 axis, float half_angle, float radius, v3 o, const box3& b, float dist_to_box)
+{
+    if (half_angle >= F_PI_2 || dist_to_box < F_EPS)
+        return true;
+    const float cx = (b.hi.x + b.lo.x) * 0.5f, cy = (b.hi.y + b.lo.y) * 0.5f;
+    float vx = cx - o.x, vy = cy - o.y;
+    const float l = len2(vx, vy);
+    vx /= l;
+    vy /= l;
+    const float d_axis = dm::acosf_(clamp11(dot2(axis.x, axis.y, vx, vy)));
+    if (inrange(F_PI_2, d_axis - half_angle, d_axis + half_angle))
+        return true;
+    if (l > radius) {
+        const float sum = half_angle + dm::asinf_(radius / l);
+        return sum >= F_PI_2 ? true : inrange(F_PI_2, d_axis - sum, d_axis + sum);
+    }
+    const float ex = b.hi.x - cx, ey = b.hi.y - cy;
+    float d = dot2(ex, ey, dm::fabs_(vx), dm::fabs_(vy));
+    const float s = l - d;
+    if (s <= 0.0f)
+        return true;
+    d = dm::fabs_(dot2(ex, ey, dm::fabs_(-vy), dm::fabs_(vx))); // project_to_plane (cone.cuh:44-49)
+    const float sum = half_angle + dm::atan2f_(d, s);
+    return sum >= F_PI_2 ? true : inrange(F_PI_2, d_axis - sum, d_axis + sum);
+}
+
+// 2D cone merge (cone.cuh:266-275, 320-372), same fixes as the 3D merge
+SNCH_HD cone3 cone_merge2d(const cone3& a, const cone3& b, v3 oa, v3 ob, v3 on, bool conservative)
+{
+    cone3 r{mk(0.0f, 0.0f, 0.0f), 0.0f, 0.0f};
+    const bool va = cone_valid(a.half_angle), vb = cone_valid(b.half_angle);
+    if (va && vb) {
+        float ax = a.axis.x, ay = a.axis.y, bx = b.axis.x, by = b.axis.y;
+        float ha = a.half_angle, hb = b.half_angle;
+        const float dax = on.x - oa.x, day = on.y - oa.y, dbx = on.x - ob.x, dby = on.y - ob.y;
+        if (conservative)
+            r.radius = smax(a.radius + len2(dax, day), b.radius + len2(dbx, dby));
+        else
+            r.radius = dm::sqrt_(smax(a.radius * a.radius + (dax * dax + day * day),
+                                      b.radius * b.radius + (dbx * dbx + dby * dby)));
+        if (hb > ha) {
+            float t = ax; ax = bx; bx = t;
+            t = ay; ay = by; by = t;
+            t = ha; ha = hb; hb = t;
+        }
+        const float theta = dm::acosf_(clamp11(dot2(ax, ay, bx, by)));
+        if (smin(theta + hb, F_PI) <= ha) {
+            r.axis = mk(ax, ay, 0.0f);
+            r.half_angle = ha;
+            return r;
+        }
+        const float o_theta = (ha + theta + hb) / 2.0f;
+        if (o_theta >= F_PI) {
+            r.axis = mk(ax, ay, 0.0f);
+            r.half_angle = F_PI;
+            return r;
+        }
+        float rt = o_theta - ha;
+        const float det = ax * by - ay * bx;
+        rt *= dm::signbit_(det) ? -1.0f : 1.0f; // copysign(1, det)
+        float st, ct;
+        dm::sincosf_(rt, &st, &ct);
+        r.axis = mk(ct * ax - st * ay, st * ax + ct * ay, 0.0f);
+        r.half_angle = o_theta;
+    } else if (va) {
+        r = a;
+    } else if (vb) {
+        r = b;
+    } else {
+        r.half_angle = -F_PI;
+    }
+    return r;
+}
+
+// closest point on a 2D segment, distance only (scene.cuh:176-201)
+SNCH_HD float closest_point_segment2d(v3 pa, v3 pb, v3 x)
+{
+    const float ux = pb.x - pa.x, uy = pb.y - pa.y;
+    const float c1 = dot2(ux, uy, x.x - pa.x, x.y - pa.y);
+    if (c1 <= 0.0f)
+        return len2(x.x - pa.x, x.y - pa.y);
+    const float c2 = dot2(ux, uy, ux, uy);
+    if (c2 <= c1)
+        return len2(x.x - pb.x, x.y - pb.y);
+    const float t = c1 / c2;
+    const float px = pa.x + ux * t, py = pa.y + uy * t;
+    return len2(x.x - px, x.y - py);
+}
+
+// silhouette predicate of a vertex (scene.cuh:112-141); 2D normalises the view
+SNCH_HD bool is_silhouette_vertex(v3 n0, v3 n1, float vx, float vy, float d, bool flip)
+{
+    const float precision = 1e-3f;
+    const float sign = flip ? 1.0f : -1.0f;
+    if (d <= precision)
+        return sign * (n0.x * n1.y - n0.y * n1.x) > precision;
+    const float ux = vx / d, uy = vy / d;
+    const float dot0 = dot2(ux, uy, n0.x, n0.y), dot1 = dot2(ux, uy, n1.x, n1.y);
+    if (dm::fabs_(dot0) <= precision)
+        return sign * dot1 > precision;
+    if (dm::fabs_(dot1) <= precision)
+        return sign * dot0 > precision;
+    return dot0 * dot1 < 0.0f;
+}
+
+// ray-segment test (scene.cuh:545-576)
+SNCH_HD bool ray_segment(v3 o, v3 dir, v3 p0, v3 p1, float& t_out, float& s_out)
+{
+    const float sx = p1.x - p0.x, sy = p1.y - p0.y;
+    const float D = dir.x * (-sy) + dir.y * sx;
+    if (dm::fabs_(D) < F_EPS)
+        return false;
+    const float inv = 1.0f / D;
+    const float t = ((p0.x - o.x) * (-sy) - (p0.y - o.y) * (-sx)) * inv;
+    const float s = (dir.x * (p0.y - o.y) - dir.y * (p0.x - o.x)) * inv;
+    if (s >= -1e-3f && s <= 1.0f + 1e-3f && t >= 0.0f) {
+        t_out = t;
+        s_out = s;
+        return true;
+    }
+    return false;
+}
+
+// segment-sphere test (scene.cuh:581-602)
+SNCH_HD bool sphere_segment(v3 p1, v3 p2, v3 c, float radius)
+{
+    const float dx = p2.x - p1.x, dy = p2.y - p1.y;
+    const float len_sq = dx * dx + dy * dy;
+    float t = ((c.x - p1.x) * dx + (c.y - p1.y) * dy) / len_sq;
+    t = smax(0.0f, smin(1.0f, t));
+    const float ex = p1.x + t * dx - c.x, ey = p1.y + t * dy - c.y;
+    return ex * ex + ey * ey <= (radius * radius);
+}
+
+// 2D Green's function weight (scene.cuh:605-612)
+SNCH_HD float green_weight2d(v3 x, v3 y)
+{
+    const float r = fmx(len2(x.x - y.x, x.y - y.y), 1e-2f);
+    return dm::fabs_(dm::logf_(r) / (F_PI * 2.0f));
+}
+
 } // namespace snch

@@ -73,7 +73,8 @@ struct snch_context {
 struct snch_scene {
     snch_context* ctx = nullptr;
     uint32_t dim = 3;
-    uint32_t nV = 0, nF = 0, nE = 0;
+    uint32_t nV = 0, nF = 0, nE = 0; // nE: silhouette table size (edges in 3D, vertices in 2D)
+    uint32_t n_internal_sil = 0;     // packed silhouette records (owned silhouettes)
     // host copies of the inputs and of the silhouette table
     std::vector<float> V;
     std::vector<int32_t> F;
@@ -97,20 +98,23 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
 // build.hip
-hipError_t build_tree_3d(snch_scene* s, uint32_t flags, hipStream_t st);
+hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st);
 // sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt);
-// query.hip
-hipError_t launch_closest_point(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
+// query.hip (dim = 2 or 3)
+hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
                                 uint64_t* stats, hipStream_t st);
-hipError_t launch_closest_silhouette(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
-                                     uint32_t* sil, float* dist, uint32_t block, uint64_t* stats, hipStream_t st);
-hipError_t launch_ray(const TreeView& t, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
-                      uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats, hipStream_t st);
-hipError_t launch_sample(const TreeView& t, const float4* V, const int32_t* F, const float* c, const float* r,
-                         const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt, hipStream_t st);
-hipError_t launch_sample_on_object(const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
+hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint64_t* stats,
+                                     hipStream_t st);
+hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
+                      int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
+                      hipStream_t st);
+hipError_t launch_sample(const TreeView& t, int dim, const float4* V, const int32_t* F, const float* c,
+                         const float* r, const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt,
+                         hipStream_t st);
+hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
                                    const float* uv, uint32_t n, float* pt, hipStream_t st);
 uint32_t* traversal_error_word();
 } // namespace snch

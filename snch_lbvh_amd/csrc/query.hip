@@ -133,12 +133,25 @@ __device__ __forceinline__ box3 ldbox2(const float4* nb, uint32_t n)
 }
 
 // ------------------------------------------------------------ closest point
+template <int DIM>
+__device__ __forceinline__ v3 load_point(const float* __restrict__ q, uint32_t i)
+{
+    return DIM == 3 ? mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]) : mk(q[2 * i], q[2 * i + 1], 0.0f);
+}
+
+template <int DIM>
 __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, float& best, uint32_t& best_idx)
 {
     if (j >= t.n_leaves)
         return;
-    const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], c = t.tleaf[3 * j + 2];
-    float d = closest_point_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), p);
+    const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1];
+    float d;
+    if (DIM == 3) {
+        const float4 c = t.tleaf[3 * j + 2];
+        d = closest_point_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), p);
+    } else {
+        d = closest_point_segment2d(mk(a.x, a.y, 0.0f), mk(b.x, b.y, 0.0f), p); // scene.cuh:504-515
+    }
     d *= d; // query.cuh:284-285
     const uint32_t o = __float_as_uint(a.w);
     if (d < best || (d == best && o < best_idx)) {
@@ -147,7 +160,7 @@ __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, flo
     }
 }
 
-template <int BS, bool STATS>
+template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* __restrict__ q, uint32_t n,
                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
                                                       uint32_t* __restrict__ err, unsigned long long* stats)
@@ -160,11 +173,11 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
             flush_counts(cnt, stats);
         return;
     }
-    const v3 p = mk(q[3 * qi], q[3 * qi + 1], q[3 * qi + 2]);
+    const v3 p = load_point<DIM>(q, qi);
     float best = F_INF;
     uint32_t best_idx = INVALID;
     if (t.n_leaves == 1) {
-        cp_leaf(t, 0, p, best, best_idx);
+        cp_leaf<DIM>(t, 0, p, best, best_idx);
     } else {
         LaneStack<BS> st;
         st.init(stk + threadIdx.x);
@@ -184,9 +197,9 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
                 cnt.leaf += (cL && lLeaf && Lmd <= best) + (cR && rLeaf && Rmd <= best);
             }
             if (cL && lLeaf && Lmd <= best)
-                cp_leaf(t, nr.lref & ~LEAF_BIT, p, best, best_idx);
+                cp_leaf<DIM>(t, nr.lref & ~LEAF_BIT, p, best, best_idx);
             if (cR && rLeaf && Rmd <= best)
-                cp_leaf(t, nr.rref & ~LEAF_BIT, p, best, best_idx);
+                cp_leaf<DIM>(t, nr.rref & ~LEAF_BIT, p, best, best_idx);
             const bool gL = cL && !lLeaf && Lmd <= best;
             const bool gR = cR && !rLeaf && Rmd <= best;
             if (gL && gR) {
@@ -224,8 +237,10 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
 }
 
 // ------------------------------------------------------------ silhouette
-// one leaf: the owned edges (scene.cuh:978-1003), each tested as in
-// find_closest_silhouette_point (scene.cuh:788-824)
+// one leaf: its owned silhouettes (edges in 3D, scene.cuh:978-1003 and
+// :788-824; vertices in 2D, scene.cuh:517-541 and :354-386).  The record's
+// w holds the silhouette id reported to the caller.
+template <int DIM>
 __device__ __forceinline__ void sil_leaf(const TreeView& t, uint32_t j, v3 p, bool flip, float& best,
                                          uint32_t& best_e, Counts& cnt)
 {
@@ -241,26 +256,46 @@ __device__ __forceinline__ void sil_leaf(const TreeView& t, uint32_t j, v3 p, bo
         if (!(0.0f < mr2))
             break; // min_radius_squared (0) >= max_radius_squared
         const float4 A = t.tedge[4 * e], B = t.tedge[4 * e + 1];
-        const v3 pa = mk(A.x, A.y, A.z), pb = mk(B.x, B.y, B.z);
-        v3 cp;
-        const float d = closest_point_segment(pa, pb, p, cp);
-        if (d * d > mr2)
-            continue;
+        const uint32_t id = __float_as_uint(A.w);
         const uint32_t fl = __float_as_uint(B.w);
-        bool sil = (fl & 3u) != 3u; // a boundary edge is always a silhouette
-        if (!sil) {
-            const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
-            sil = is_silhouette_edge(pa, pb, mk(N0.x, N0.y, N0.z), mk(N1.x, N1.y, N1.z),
-                                     mk(p.x - cp.x, p.y - cp.y, p.z - cp.z), d, flip);
+        float d;
+        bool sil = (fl & 3u) != 3u; // a boundary edge / end vertex is always a silhouette
+        if (DIM == 3) {
+            const v3 pa = mk(A.x, A.y, A.z), pb = mk(B.x, B.y, B.z);
+            v3 cp;
+            d = closest_point_segment(pa, pb, p, cp);
+            if (d * d > mr2)
+                continue;
+            if (!sil) {
+                const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
+                sil = is_silhouette_edge(pa, pb, mk(N0.x, N0.y, N0.z), mk(N1.x, N1.y, N1.z),
+                                         mk(p.x - cp.x, p.y - cp.y, p.z - cp.z), d, flip);
+            }
+        } else {
+            const float vx = p.x - A.x, vy = p.y - A.y;
+            d = len2(vx, vy);
+            if (d * d > mr2)
+                continue;
+            if (!sil) {
+                const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
+                sil = is_silhouette_vertex(mk(N0.x, N0.y, 0.0f), mk(N1.x, N1.y, 0.0f), vx, vy, d, flip);
+            }
         }
-        if (sil && (d < best || (d == best && e < best_e))) {
+        if (sil && (d < best || (d == best && id < best_e))) {
             best = d;
-            best_e = e;
+            best_e = id;
         }
     }
 }
 
-template <int BS, bool STATS>
+template <int DIM>
+__device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p, const box3& b, float md)
+{
+    return DIM == 3 ? cone_overlap(mk(C.x, C.y, C.z), C.w, radius, p, b, md)
+                    : cone_overlap2d(mk(C.x, C.y, 0.0f), C.w, radius, p, b, md);
+}
+
+template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
@@ -274,14 +309,14 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
             flush_counts(cnt, stats);
         return;
     }
-    const v3 p = mk(q[3 * qi], q[3 * qi + 1], q[3 * qi + 2]);
+    const v3 p = load_point<DIM>(q, qi);
     float best = F_INF;
     uint32_t best_e = INVALID;
     if (t.n_leaves == 1) {
         const box3 b = ldbox2(t.nbox, 0);
         const float4 c0 = t.ncone[0], c1 = t.ncone[1];
-        if (cone_valid(c0.w) && cone_overlap(mk(c0.x, c0.y, c0.z), c0.w, c1.x, p, b, box_mindist(b, p)))
-            sil_leaf(t, 0, p, flip != 0, best, best_e, cnt);
+        if (cone_valid(c0.w) && overlap_dim<DIM>(c0, c1.x, p, b, box_mindist(b, p)))
+            sil_leaf<DIM>(t, 0, p, flip != 0, best, best_e, cnt);
     } else {
         // A seeded radius r0 caps the search: nodes farther than r0 are
         // skipped.  A silhouette found at distance <= r0 is the global answer,
@@ -310,14 +345,14 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
                 if (hL || hR) {
                     const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
                     // query.cuh:366-367
-                    hL = hL && cone_valid(CL.w) && cone_overlap(mk(CL.x, CL.y, CL.z), CL.w, Cr.x, p, nr.L, Lmd);
-                    hR = hR && cone_valid(CR.w) && cone_overlap(mk(CR.x, CR.y, CR.z), CR.w, Cr.y, p, nr.R, Rmd);
+                    hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
+                    hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
                 }
                 const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
                 if (hL && lLeaf)
-                    sil_leaf(t, nr.lref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
+                    sil_leaf<DIM>(t, nr.lref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
                 if (hR && rLeaf)
-                    sil_leaf(t, nr.rref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
+                    sil_leaf<DIM>(t, nr.rref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
                 const float nb2 = best * best;
                 const bool gL = hL && !lLeaf && Lmd <= nb2;
                 const bool gR = hR && !rLeaf && Rmd <= nb2;
@@ -362,7 +397,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
 }
 
 // ------------------------------------------------------------ ray
-template <int BS, bool STATS>
+template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict__ o, const float* __restrict__ d,
                                             const float* __restrict__ tmax, uint32_t n, int any_hit,
                                             uint8_t* __restrict__ out_hit, float* __restrict__ out_t,
@@ -377,9 +412,9 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
             flush_counts(cnt, stats);
         return;
     }
-    const v3 ro = mk(o[3 * qi], o[3 * qi + 1], o[3 * qi + 2]);
-    const v3 rd = mk(d[3 * qi], d[3 * qi + 1], d[3 * qi + 2]);
-    const v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z); // aabb.cuh:308-311
+    const v3 ro = load_point<DIM>(o, qi);
+    const v3 rd = load_point<DIM>(d, qi);
+    const v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z); // aabb.cuh:308-311 (2D: z slab inert)
     const float max_dist = tmax ? tmax[qi] : F_INF;
     float tbest = F_INF, bu = 0.0f, bv = 0.0f;
     uint32_t bidx = INVALID;
@@ -389,11 +424,17 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
             return;
         if (STATS)
             cnt.leaf += 1;
-        const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], c = t.tleaf[3 * j + 2];
-        float th, uh, vh;
+        const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1];
+        float th, uh, vh = 0.0f;
         const uint32_t o = __float_as_uint(a.w);
-        if (ray_triangle(ro, rd, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), th, uh, vh) &&
-            th < max_dist && (th < tbest || (th == tbest && o < bidx))) { // query.cuh:115
+        bool h;
+        if (DIM == 3) {
+            const float4 c = t.tleaf[3 * j + 2];
+            h = ray_triangle(ro, rd, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), th, uh, vh);
+        } else {
+            h = ray_segment(ro, rd, mk(a.x, a.y, 0.0f), mk(b.x, b.y, 0.0f), th, uh);
+        }
+        if (h && th < max_dist && (th < tbest || (th == tbest && o < bidx))) { // query.cuh:115
             tbest = th;
             bu = uh;
             bv = vh;
@@ -468,8 +509,12 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
     if (out_t)
         out_t[qi] = tbest;
     if (out_uv) {
-        out_uv[2 * qi] = bu;
-        out_uv[2 * qi + 1] = bv;
+        if (DIM == 3) {
+            out_uv[2 * qi] = bu;
+            out_uv[2 * qi + 1] = bv;
+        } else {
+            out_uv[qi] = bu; // segment parameter s
+        }
     }
     if (out_idx)
         out_idx[qi] = bidx;
@@ -478,6 +523,7 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
 // ------------------------------------------------------------ sampling
 // sample.cuh:23-92: a single root-to-leaf descent, bit-identical to the
 // reference's FP32 arithmetic (u rescaled at each level).
+template <int DIM>
 __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __restrict__ V,
                                                 const int32_t* __restrict__ F, const float* __restrict__ cc,
                                                 const float* __restrict__ rr, const float* __restrict__ uu,
@@ -487,7 +533,7 @@ __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __rest
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (qi >= n)
         return;
-    const v3 c = mk(cc[3 * qi], cc[3 * qi + 1], cc[3 * qi + 2]);
+    const v3 c = load_point<DIM>(cc, qi);
     const float r = rr[qi];
     float u = uu[3 * qi];
     float npdf = 1.0f;
@@ -501,18 +547,22 @@ __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __rest
                 break;
             const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], d = t.tleaf[3 * j + 2];
             const v3 pa = mk(a.x, a.y, a.z), pb = mk(b.x, b.y, b.z), pc = mk(d.x, d.y, d.z);
-            if (tri_sphere(pa, pb, pc, c, r)) {
+            if (DIM == 3 ? tri_sphere(pa, pb, pc, c, r) : sphere_segment(pa, pb, c, r)) {
                 obj = (int32_t)__float_as_uint(a.w);
                 pdf = npdf;
-                pdf /= tri_area(pa, pb, pc);
+                pdf /= DIM == 3 ? tri_area(pa, pb, pc) : len2(pa.x - pb.x, pa.y - pb.y); // scene.cuh:409-418
             }
             break;
         }
         if (ref >= t.n_leaves - 1)
             break;
         const NodeRec nr = load_node(t.tnode, ref);
-        const float Lw = box_sphere(nr.L, c, r) ? green_weight(c, box_centroid(nr.L)) : 0.0f;
-        const float Rw = box_sphere(nr.R, c, r) ? green_weight(c, box_centroid(nr.R)) : 0.0f;
+        const float Lw = box_sphere(nr.L, c, r)
+                             ? (DIM == 3 ? green_weight(c, box_centroid(nr.L)) : green_weight2d(c, box_centroid(nr.L)))
+                             : 0.0f;
+        const float Rw = box_sphere(nr.R, c, r)
+                             ? (DIM == 3 ? green_weight(c, box_centroid(nr.R)) : green_weight2d(c, box_centroid(nr.R)))
+                             : 0.0f;
         const float tw = Lw + Rw;
         if (!(tw > 0.0f))
             break;
@@ -533,16 +583,28 @@ __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __rest
     if (out_pt) {
         v3 pt = mk(0.0f, 0.0f, 0.0f);
         if (obj >= 0) {
-            const float4 a = V[F[3 * obj]], b = V[F[3 * obj + 1]], d = V[F[3 * obj + 2]];
-            pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uu[3 * qi + 1],
-                                 uu[3 * qi + 2]);
+            if (DIM == 3) {
+                const float4 a = V[F[3 * obj]], b = V[F[3 * obj + 1]], d = V[F[3 * obj + 2]];
+                pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uu[3 * qi + 1],
+                                     uu[3 * qi + 2]);
+            } else { // sample_line (scene.cuh:29-32)
+                const float4 a = V[F[2 * obj]], b = V[F[2 * obj + 1]];
+                const float u1 = uu[3 * qi + 1];
+                pt = mk(a.x + u1 * (b.x - a.x), a.y + u1 * (b.y - a.y), 0.0f);
+            }
         }
-        out_pt[3 * qi] = pt.x;
-        out_pt[3 * qi + 1] = pt.y;
-        out_pt[3 * qi + 2] = pt.z;
+        if (DIM == 3) {
+            out_pt[3 * qi] = pt.x;
+            out_pt[3 * qi + 1] = pt.y;
+            out_pt[3 * qi + 2] = pt.z;
+        } else {
+            out_pt[2 * qi] = pt.x;
+            out_pt[2 * qi + 1] = pt.y;
+        }
     }
 }
 
+template <int DIM>
 __global__ __launch_bounds__(256) void k_sample_on_object(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                                           uint32_t nF, const int32_t* __restrict__ idx,
                                                           const float* __restrict__ uv, uint32_t n,
@@ -554,93 +616,124 @@ __global__ __launch_bounds__(256) void k_sample_on_object(const float4* __restri
     const int32_t o = idx[qi];
     v3 pt = mk(0.0f, 0.0f, 0.0f);
     if (o >= 0 && (uint32_t)o < nF) {
-        const float4 a = V[F[3 * o]], b = V[F[3 * o + 1]], d = V[F[3 * o + 2]];
-        pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uv[2 * qi], uv[2 * qi + 1]);
+        if (DIM == 3) {
+            const float4 a = V[F[3 * o]], b = V[F[3 * o + 1]], d = V[F[3 * o + 2]];
+            pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uv[2 * qi],
+                                 uv[2 * qi + 1]);
+        } else {
+            const float4 a = V[F[2 * o]], b = V[F[2 * o + 1]];
+            const float u1 = uv[2 * qi];
+            pt = mk(a.x + u1 * (b.x - a.x), a.y + u1 * (b.y - a.y), 0.0f);
+        }
     }
-    out_pt[3 * qi] = pt.x;
-    out_pt[3 * qi + 1] = pt.y;
-    out_pt[3 * qi + 2] = pt.z;
+    if (DIM == 3) {
+        out_pt[3 * qi] = pt.x;
+        out_pt[3 * qi + 1] = pt.y;
+        out_pt[3 * qi + 2] = pt.z;
+    } else {
+        out_pt[2 * qi] = pt.x;
+        out_pt[2 * qi + 1] = pt.y;
+    }
 }
 
 inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
 
 } // namespace
 
-// error word for traversal overflow, per device (tiny, allocated once)
-uint32_t* traversal_error_word();
+#define SNCH_DISPATCH_DIM(dim, ...) \
+    do {                            \
+        if ((dim) == 3) {           \
+            constexpr int DIM = 3;  \
+            __VA_ARGS__;            \
+        } else {                    \
+            constexpr int DIM = 2;  \
+            __VA_ARGS__;            \
+        }                           \
+    } while (0)
 
-hipError_t launch_closest_point(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
+hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
                                 uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
-    if (stats)
-        hipLaunchKernelGGL((k_closest_point<256, true>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n, idx, dist,
-                           traversal_error_word(), sp);
-    else
-        hipLaunchKernelGGL((k_closest_point<256, false>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n, idx,
-                           dist, traversal_error_word(), sp);
+    SNCH_DISPATCH_DIM(dim, {
+        if (stats)
+            hipLaunchKernelGGL((k_closest_point<256, true, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
+                               idx, dist, traversal_error_word(), sp);
+        else
+            hipLaunchKernelGGL((k_closest_point<256, false, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
+                               idx, dist, traversal_error_word(), sp);
+    });
     return hipGetLastError();
 }
 
-template <int BS>
+template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
                           float* dist, unsigned long long* stats, hipStream_t st)
 {
     if (stats)
-        hipLaunchKernelGGL((k_closest_silhouette<BS, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n, flip,
-                           seed, sil, dist, traversal_error_word(), stats);
+        hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
+                           flip, seed, sil, dist, traversal_error_word(), stats);
     else
-        hipLaunchKernelGGL((k_closest_silhouette<BS, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n, flip,
-                           seed, sil, dist, traversal_error_word(), stats);
+        hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
+                           flip, seed, sil, dist, traversal_error_word(), stats);
 }
 
-hipError_t launch_closest_silhouette(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
-                                     uint32_t* sil, float* dist, uint32_t block, uint64_t* stats, hipStream_t st)
+hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint64_t* stats,
+                                     hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
-    switch (block) {
-    case 64: launch_sil_bs<64>(t, q, n, flip, seed, sil, dist, sp, st); break;
-    case 128: launch_sil_bs<128>(t, q, n, flip, seed, sil, dist, sp, st); break;
-    case 512: launch_sil_bs<512>(t, q, n, flip, seed, sil, dist, sp, st); break;
-    default: launch_sil_bs<256>(t, q, n, flip, seed, sil, dist, sp, st); break;
-    }
+    SNCH_DISPATCH_DIM(dim, {
+        switch (block) {
+        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
+        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
+        }
+    });
     return hipGetLastError();
 }
 
-hipError_t launch_ray(const TreeView& t, const float* o, const float* d, const float* tmax, uint32_t n, int any_hit,
-                      uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats, hipStream_t st)
+hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
+                      int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
+                      hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
-    if (stats)
-        hipLaunchKernelGGL((k_ray<256, true>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n, any_hit, hit,
-                           tout, uv, idx, traversal_error_word(), sp);
-    else
-        hipLaunchKernelGGL((k_ray<256, false>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n, any_hit,
-                           hit, tout, uv, idx, traversal_error_word(), sp);
+    SNCH_DISPATCH_DIM(dim, {
+        if (stats)
+            hipLaunchKernelGGL((k_ray<256, true, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n,
+                               any_hit, hit, tout, uv, idx, traversal_error_word(), sp);
+        else
+            hipLaunchKernelGGL((k_ray<256, false, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, o, d, tmax, n,
+                               any_hit, hit, tout, uv, idx, traversal_error_word(), sp);
+    });
     return hipGetLastError();
 }
 
-hipError_t launch_sample(const TreeView& t, const float4* V, const int32_t* F, const float* c, const float* r,
-                         const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt, hipStream_t st)
+hipError_t launch_sample(const TreeView& t, int dim, const float4* V, const int32_t* F, const float* c,
+                         const float* r, const float* u, uint32_t n, int32_t* idx, float* pdf, float* pt,
+                         hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_sample, dim3(nblocks(n, 256)), dim3(256), 0, st, t, V, F, c, r, u, n, idx, pdf, pt);
+    SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_sample<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, t, V, F, c, r,
+                                              u, n, idx, pdf, pt));
     return hipGetLastError();
 }
 
-hipError_t launch_sample_on_object(const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
+hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
                                    const float* uv, uint32_t n, float* pt, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_sample_on_object, dim3(nblocks(n, 256)), dim3(256), 0, st, V, F, nF, idx, uv, n, pt);
+    SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_sample_on_object<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, V,
+                                              F, nF, idx, uv, n, pt));
     return hipGetLastError();
 }
 

@@ -172,3 +172,29 @@ def config_large(n_query=1 << 22, nu=1024, nv=512):
     lo, hi = padded_box(V)
     q = uniform_points(n_query, lo, hi, seed=4)
     return V, F, q
+
+
+def circle_in_square(n_circle=256, n_side=64, radius=0.5, sigma=0.0, seed=1):
+    """2D scene like the reference's circle_in_square case: a square [-1, 1]^2
+    boundary (counter-clockwise) around a circle polygon (clockwise), both
+    closed loops of line segments."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0, 1, n_side, endpoint=False)
+    sq = np.concatenate([np.stack([-1 + 2 * t, -np.ones_like(t)], 1), np.stack([np.ones_like(t), -1 + 2 * t], 1),
+                         np.stack([1 - 2 * t, np.ones_like(t)], 1), np.stack([-np.ones_like(t), 1 - 2 * t], 1)])
+    a = -2 * np.pi * np.arange(n_circle) / n_circle
+    rr = radius * (1 + (rng.normal(0, sigma, n_circle) if sigma > 0 else 0))
+    ci = np.stack([rr * np.cos(a), rr * np.sin(a)], 1)
+    V = np.concatenate([sq, ci]).astype(np.float32)
+    ns = sq.shape[0]
+    S = np.concatenate([np.stack([np.arange(ns), (np.arange(ns) + 1) % ns], 1),
+                        ns + np.stack([np.arange(n_circle), (np.arange(n_circle) + 1) % n_circle], 1)])
+    return V, S.astype(np.int32)
+
+
+def polyline_soup(n_vert=400, n_seg=600, seed=2):
+    rng = np.random.default_rng(seed)
+    V = rng.uniform(-1, 1, (n_vert, 2)).astype(np.float32)
+    S = rng.integers(0, n_vert, (n_seg, 2)).astype(np.int32)
+    S = S[S[:, 0] != S[:, 1]]
+    return V, S
