@@ -163,6 +163,20 @@ int snch_mlp_gradient(snch_mlp *mlp, const float *pts, const float *dist, uint32
 int snch_mlp_predict(snch_mlp *mlp, const float *pts, uint32_t n, int mode, float rel, float abs_margin, float *out,
                      void *stream);
 
+/* ---- wavefront walk-on-stars (Laplace, Dirichlet g = x*y + zero Neumann) --
+ * No counterpart in the reference: the consumer that drives all four queries
+ * per step over the shrinking set of live walks.  The step loop can be
+ * captured once as a hipGraph and replayed (use_graph = 1). */
+typedef struct snch_wost snch_wost;
+/* neumann may be NULL (pure Dirichlet problem); scenes must be built 3D scenes */
+int snch_wost_create(snch_context *ctx, const snch_scene *dirichlet, const snch_scene *neumann, uint32_t max_walks,
+                     snch_wost **out);
+int snch_wost_destroy(snch_wost *wost);
+/* x0: device n x 3 start points inside the domain; out: device n estimates;
+ * live_after (host, may be NULL): walks still live after max_steps (0). */
+int snch_wost_solve(snch_wost *wost, const float *x0, uint32_t n, uint32_t max_steps, float eps, uint64_t seed,
+                    int use_graph, float *out, uint32_t *live_after, void *stream);
+
 /* ---- device memory helpers for callers without their own allocator ------- */
 int snch_device_malloc(void **ptr, uint64_t bytes);
 int snch_device_free(void *ptr);

@@ -63,7 +63,7 @@ EXPORTED = [
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
     "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
-    "snch_mlp_gradient", "snch_mlp_predict",
+    "snch_mlp_gradient", "snch_mlp_predict", "snch_wost_create", "snch_wost_destroy", "snch_wost_solve",
 ]
 
 
@@ -94,6 +94,9 @@ def _declare(L):
     L.snch_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, ctypes.c_float, vp, vp]
     L.snch_mlp_gradient.argtypes = [vp, vp, vp, u32, vp, vp, vp]
     L.snch_mlp_predict.argtypes = [vp, vp, u32, c_int, ctypes.c_float, ctypes.c_float, vp, vp]
+    L.snch_wost_create.argtypes = [vp, vp, vp, u32, ctypes.POINTER(vp)]
+    L.snch_wost_destroy.argtypes = [vp]
+    L.snch_wost_solve.argtypes = [vp, vp, u32, u32, ctypes.c_float, ctypes.c_uint64, c_int, vp, vp, vp]
     for name in EXPORTED:
         if name != "snch_last_error":
             getattr(L, name).restype = i32
@@ -339,3 +342,37 @@ class LearnedBound:
         check(lib().snch_mlp_predict(self._h, _ptr(p), p.shape[0], 1, float(rel), float(abs_margin), _ptr(out),
                                      _stream()))
         return out
+
+
+class WoSt:
+    """Walk on stars for the Laplace equation: Dirichlet data g = x y on the
+    ``dirichlet`` scene, zero Neumann flux on the ``neumann`` scene (or none).
+    Each step runs closest point, closest silhouette, ray and
+    SampleTriangleInSphere over the live walks; the loop can run as a
+    captured hipGraph."""
+
+    def __init__(self, dirichlet, neumann=None, max_walks=1 << 20):
+        self.device = dirichlet.device
+        self.max_walks = max_walks
+        self._scenes = (dirichlet, neumann)  # keep the scenes alive
+        h = ctypes.c_void_p()
+        check(lib().snch_wost_create(context(self.device), dirichlet._h, neumann._h if neumann else None, max_walks,
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.snch_wost_destroy(h)
+            self._h = None
+
+    def solve(self, x0, max_steps=64, eps=1e-3, seed=0, use_graph=True, out=None):
+        import torch
+
+        x = _dev_f32(x0, 3, self.device)
+        n = x.shape[0]
+        out = torch.empty(n, dtype=torch.float32, device=x.device) if out is None else out
+        live = ctypes.c_uint32()
+        check(lib().snch_wost_solve(self._h, _ptr(x), n, max_steps, float(eps), int(seed), int(bool(use_graph)),
+                                    _ptr(out), ctypes.byref(live), _stream()))
+        return out, live.value

@@ -360,7 +360,7 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
 __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
                                                uint32_t N, uint32_t* __restrict__ parent, uint2* __restrict__ child,
-                                               uint32_t* __restrict__ flags)
+                                               uint2* __restrict__ range, uint32_t* __restrict__ flags)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     if (i >= N - 1)
@@ -415,6 +415,7 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     if (last == g + 1)
         right += N - 1;
     child[i] = make_uint2(left, right);
+    range[i] = make_uint2(first, last);
     parent[left] = i;
     parent[right] = i;
     flags[i] = 0u;
@@ -438,17 +439,31 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
     return r;
 }
 
+// Bottom-up refit.  A workgroup owns RT consecutive leaves (sorted order);
+// an internal node whose leaf range lies inside that window has both
+// subtrees built by the same workgroup, so its arrival counter lives in LDS
+// (indexed by the node id, one end of its range) and ordering needs only
+// workgroup-scope fences.  Only nodes whose range straddles windows use the
+// global counter with agent-scope acquire-release (an L2 write-back and
+// invalidate on this multi-XCD part), which is ~log2(N / RT) levels per
+// window boundary instead of every level of every leaf.
+constexpr int RT = 1024;
+
 template <int DIM>
-__global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
+__global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
                                               const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
                                               const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
-                                              const uint2* __restrict__ child, uint32_t* flags, uint32_t N,
-                                              int conservative, float4* nbox, float4* ncone,
-                                              float4* __restrict__ tnode, float4* __restrict__ tcone,
+                                              const uint2* __restrict__ child, const uint2* __restrict__ range,
+                                              uint32_t* flags, uint32_t N, int conservative, float4* nbox,
+                                              float4* ncone, float4* __restrict__ tnode, float4* __restrict__ tcone,
                                               float4* __restrict__ tleaf)
 {
-    const uint32_t j = blockIdx.x * BT + threadIdx.x;
+    __shared__ uint32_t lflag[RT];
+    const uint32_t base = blockIdx.x * RT;
+    const uint32_t j = base + threadIdx.x;
+    lflag[threadIdx.x] = 0u;
+    __syncthreads();
     if (j >= N)
         return;
     const uint32_t orig = sidx[j];
@@ -477,7 +492,15 @@ __global__ __launch_bounds__(BT) void k_refit(const float4* __restrict__ V, cons
     // the climb is at most the tree depth (<= 64 key bits); the cap only
     // guards against a malformed parent array
     for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
-        const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint2 rg = range[p];
+        uint32_t old;
+        if (rg.x >= base && rg.y < base + RT) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            old = atomicAdd(&lflag[p - base], 1u);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else {
+            old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (old == 0u)
             return; // the sibling subtree is not finished; its last thread continues
         const uint2 ch = child[p];
@@ -531,6 +554,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
     SNCH_TRY(w.idx_alt.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.parent.ensure(sizeof(uint32_t) * NN));
     SNCH_TRY(w.child.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(w.range.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (N > 1 ? N - 1 : 1)));
@@ -555,24 +579,26 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
     }
     bool in_alt = false;
     SNCH_TRY(radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
-                              w.idx_alt.as<uint32_t>(), N, s->dim == 3 ? 30 : 20, w.hist, st, &in_alt));
+                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
     if (N > 1)
         hipLaunchKernelGGL(k_karras, dim3(blocks(N - 1)), dim3(BT), 0, st, codes, sidx, N, w.parent.as<uint32_t>(),
-                           w.child.as<uint2>(), w.flags.as<uint32_t>());
+                           w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>());
     else
         hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>());
     if (s->dim == 3)
-        hipLaunchKernelGGL(k_refit<3>, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
-                           s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
-                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
+        hipLaunchKernelGGL(k_refit<3>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
+                           s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
+                           w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
+                           w.flags.as<uint32_t>(), N, conservative,
                            s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
                            s->tcone.as<float4>(), s->tleaf.as<float4>());
     else
-        hipLaunchKernelGGL(k_refit<2>, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
-                           s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(), w.leafcone.as<float4>(),
-                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative,
+        hipLaunchKernelGGL(k_refit<2>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
+                           s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
+                           w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
+                           w.flags.as<uint32_t>(), N, conservative,
                            s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
                            s->tcone.as<float4>(), s->tleaf.as<float4>());
     s->sorted_in_alt = in_alt;

@@ -48,6 +48,9 @@ struct TreeView {
     const float4* __restrict__ tedge;
     const float4* __restrict__ nbox;  // own box per node, reference numbering (root = 0)
     const float4* __restrict__ ncone; // own cone per node, reference numbering
+    // optional device-side query count (<= the launch's n): lets a captured
+    // graph run over a data-dependent number of queries with a fixed grid
+    const uint32_t* n_dev = nullptr;
 };
 
 struct DeviceBuffer {
@@ -59,7 +62,7 @@ struct DeviceBuffer {
 };
 
 struct BuildScratch {
-    DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, flags,
+    DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
         status;
 };
 

@@ -168,6 +168,8 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
     __shared__ uint2 stk[SS * BS];
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
+    if (t.n_dev)
+        n = min(n, *t.n_dev);
     if (qi >= n) {
         if (STATS)
             flush_counts(cnt, stats);
@@ -304,6 +306,8 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     __shared__ uint2 stk[SS * BS];
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
+    if (t.n_dev)
+        n = min(n, *t.n_dev);
     if (qi >= n) {
         if (STATS)
             flush_counts(cnt, stats);
@@ -407,6 +411,8 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
     __shared__ uint2 stk[SS * BS];
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
+    if (t.n_dev)
+        n = min(n, *t.n_dev);
     if (qi >= n) {
         if (STATS)
             flush_counts(cnt, stats);
@@ -531,6 +537,8 @@ __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __rest
                                                 float* __restrict__ out_pdf, float* __restrict__ out_pt)
 {
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (t.n_dev)
+        n = min(n, *t.n_dev);
     if (qi >= n)
         return;
     const v3 c = load_point<DIM>(cc, qi);

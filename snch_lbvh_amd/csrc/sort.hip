@@ -45,45 +45,51 @@ __global__ __launch_bounds__(ST) void k_upsweep(const uint32_t* __restrict__ key
     hist[tid * ntiles + blockIdx.x] = h[tid];
 }
 
-// exclusive scan of a[0..len) by one block of 1024 threads
+// exclusive scan of a[0..len) (len a multiple of 4) by one block of 1024
+// threads: coalesced 4096-element tiles with a running carry
 __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ a, uint32_t len)
 {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t chunk = (len + 1023) / 1024;
-    const uint32_t start = tid * chunk;
-    const uint32_t end = start + chunk < len ? start + chunk : len;
-    uint32_t s = 0;
-    for (uint32_t i = start; i < end; ++i)
-        s += a[i];
-    uint32_t incl = s;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < len; base += 4096) {
+        const uint32_t idx = base + 4 * tid;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (idx < len)
+            v = *reinterpret_cast<const uint4*>(a + idx);
+        const uint32_t s = v.x + v.y + v.z + v.w;
+        uint32_t incl = s;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t v = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o)
-            incl += v;
-    }
-    if (lane == 63)
-        wsum[w] = incl;
-    __syncthreads();
-    if (w == 0) {
-        uint32_t v = lane < 16 ? wsum[lane] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
             if (lane >= (uint32_t)o)
-                x += y;
+                incl += y;
         }
-        if (lane < 16)
-            wsum[lane] = x - v;
-    }
-    __syncthreads();
-    uint32_t run = incl - s + wsum[w];
-    for (uint32_t i = start; i < end; ++i) {
-        uint32_t t = a[i];
-        a[i] = run;
-        run += t;
+        if (lane == 63)
+            wsum[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t x0 = lane < 16 ? wsum[lane] : 0u;
+            uint32_t x = x0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= (uint32_t)o)
+                    x += y;
+            }
+            if (lane < 16)
+                wsum[lane] = x - x0;
+        }
+        __syncthreads();
+        const uint32_t ex = carry + wsum[w] + incl - s;
+        if (idx < len)
+            *reinterpret_cast<uint4*>(a + idx) = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
+        if (tid == 1023)
+            carry_s = ex + s;
+        __syncthreads();
+        carry = carry_s;
+        __syncthreads();
     }
 }
 

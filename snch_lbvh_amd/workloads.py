@@ -198,3 +198,19 @@ def polyline_soup(n_vert=400, n_seg=600, seed=2):
     S = rng.integers(0, n_vert, (n_seg, 2)).astype(np.int32)
     S = S[S[:, 0] != S[:, 1]]
     return V, S
+
+
+def torus_interior_points(n, R=1.0, r_max=0.2, seed=8):
+    """Points inside the solid torus (tube radius well inside the displaced surface)."""
+    rng = np.random.default_rng(seed)
+    u = 2 * np.pi * rng.random(n)
+    v = 2 * np.pi * rng.random(n)
+    rho = r_max * np.sqrt(rng.random(n))
+    return np.stack([(R + rho * np.cos(v)) * np.cos(u), (R + rho * np.cos(v)) * np.sin(u), rho * np.sin(v)],
+                    1).astype(np.float32)
+
+
+def split_dirichlet_neumann(V, F):
+    """Dirichlet part: triangles with centroid z >= 0; Neumann: the rest."""
+    cz = V[F].mean(1)[:, 2]
+    return F[cz >= 0], F[cz < 0]

@@ -72,7 +72,7 @@ def test_training_and_learned_seed(S):
     pts = W.uniform_points(n_train, lo, hi, 11)
     _, d = sc.closest_silhouette(pts)
     mlp = S.LearnedBound(lo.astype(np.float32), hi.astype(np.float32), sigma=4.0, seed=1)
-    losses = mlp.train(pts, d, batch=4096, epochs=6, lr=2e-3)
+    losses = mlp.train(pts, d, batch=2048, epochs=20, lr=3e-3)
     assert losses[-1] < 0.5 * losses[0], losses
     q = W.uniform_points(1 << 15, lo, hi, 12)
     st0 = torch.zeros(4, dtype=torch.int64, device="cuda")

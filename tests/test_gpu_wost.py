@@ -1,0 +1,55 @@
+"""Walk on stars driven by the library's four queries."""
+import numpy as np
+import pytest
+
+from snch_lbvh_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import snch_lbvh_amd as S
+
+    S.lib()
+    return S
+
+
+def test_dirichlet_harmonic_kat(S):
+    """g = x y is harmonic, so the all-Dirichlet estimator's mean is x0.x * x0.y."""
+    V, F = W.cut_torus(256, 128, amp=0.0)
+    sc = S.Scene(V, F).build_bvh()
+    starts = np.float32([[1.1, 0.2, 0.05], [-0.7, 0.75, -0.1], [0.3, -1.0, 0.0]])
+    k = 1 << 14
+    x0 = np.repeat(starts, k, axis=0)
+    ws = S.WoSt(sc, None, max_walks=len(x0))
+    vals, live = ws.solve(x0, max_steps=64, eps=1e-3, seed=7)
+    assert live == 0
+    v = vals.cpu().numpy().reshape(3, k)
+    for s, row in zip(starts, v):
+        exact = float(s[0]) * float(s[1])
+        se = row.std() / np.sqrt(k)
+        assert abs(row.mean() - exact) < 4 * se + 2e-3, (s, row.mean(), exact, se)
+
+
+def test_mixed_graph_matches_eager(S):
+    V, F = W.cut_torus(256, 128)
+    FD, FN = W.split_dirichlet_neumann(V, F)
+    sd = S.Scene(V, FD).build_bvh()
+    sn = S.Scene(V, FN).build_bvh()
+    x0 = W.torus_interior_points(1 << 15, seed=8)
+    ws = S.WoSt(sd, sn, max_walks=len(x0))
+    a, la = ws.solve(x0, max_steps=64, eps=1e-3, seed=3, use_graph=False)
+    b, lb = ws.solve(x0, max_steps=64, eps=1e-3, seed=3, use_graph=True)
+    c, lc = ws.solve(x0, max_steps=64, eps=1e-3, seed=3, use_graph=True)  # replay
+    assert la == lb == lc == 0
+    a, b, c = (t.cpu().numpy() for t in (a, b, c))
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(b, c)
+    assert np.all(np.isfinite(a))
+    # estimates are values of g on the Dirichlet part (z >= 0 there, |x y| <= ~0.9)
+    assert np.all(np.abs(a) < 1.0)
