@@ -5,9 +5,11 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 One step = one closest-silhouette launch over all 2^22 queries with the tree,
-the queries and the outputs resident in HBM.  Prints ONE JSON line.
+the queries and the outputs resident in HBM.  The learned-bound step adds the
+MLP inference that produces each query's seed radius.  Prints ONE JSON line.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -19,13 +21,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per query of the reference's traversal (query.cuh:325-423):
-# in 13 B (point + flip) + out 8 B (distance + edge id) + C_box * 52 B (child
-# AABB 24 + half a node record 8 + cone 20) + C_leaf * 40 B (triangle record)
-# + C_edge * 80 B (silhouette edge 32 + its 4 vertices' 48).  C_box, C_leaf and
-# C_edge are the reference visit counts measured by the CPU restatement on this
-# workload (oracle/traversal_counts.py -> tests/golden/traversal_counts.json).
-S_BOX, S_LEAF, S_EDGE, S_IN, S_OUT = 52, 40, 80, 13, 8
+# Algorithmic bytes per closest-silhouette query of this traversal, from the
+# kernel's own visit counters (one extra launch with counters on):
+#   in 12 B (point) + out 8 B (distance + silhouette id)
+#   + nodes visited * 64 B (node record: both child boxes and references)
+#   + cone records read * 48 B (both child cones and radii)
+#   + leaves evaluated * 32 B (edge range words of the leaf record)
+#   + silhouette-edge tests * 32 B (endpoints, id, flags; the 32 B of face
+#     normals read only for edges within the current bound are not counted)
+S_NODE, S_CONE, S_LEAF, S_EDGE, S_IN, S_OUT = 64, 48, 32, 32, 12, 8
 
 
 def parse():
@@ -39,15 +43,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 13)
     ap.add_argument("--build-reps", type=int, default=5)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-points", type=int, default=1 << 18)
+    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--seed-rel", type=float, default=0.25)
+    ap.add_argument("--seed-abs", type=float, default=0.005)
     return ap.parse_args()
-
-
-def counts_fixture():
-    p = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f)
-    return None
 
 
 def cpu_baseline(V, F, q, sample, silhouette_ms_gpu):
@@ -110,21 +111,92 @@ def main():
         S.check(S.lib().snch_closest_silhouette(sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d),
                                                 S._stream()))
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    def run_timed(fn, steps, warmup):
+        """Wall time and event time of `steps` calls, after `warmup` calls;
+        the events are recorded on the stream the kernels are launched on."""
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(steps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, ev0.elapsed_time(ev1) / steps
+
+    wall, step_ms = run_timed(step, args.steps, args.warmup)
     ms_per_step = wall * 1000.0 / args.steps
     value = n * args.steps / wall
+
+    # the step = Morton ordering of the queries + the traversal kernel; time
+    # each alone (the kernel with the order supplied by the caller)
+    order = sc.query_order(q)
+    kopts = S.QueryOpts(0, 0, None, order.data_ptr())
+
+    def kstep():
+        S.check(S.lib().snch_closest_silhouette_ex(sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d),
+                                                   ctypes.byref(kopts), S._stream()))
+
+    _, kernel_ms = run_timed(kstep, args.steps, 1)
+    perm_buf = torch.empty(n, dtype=torch.int32, device=dev)
+    _, order_ms = run_timed(lambda: S.check(S.lib().snch_query_order(sc._h, S._ptr(q), n, S._ptr(perm_buf),
+                                                                     S._stream())), args.steps, 1)
+    # input order (no reordering), for comparison
+    iopts = S.QueryOpts(0, 2, None, None)
+    _, input_order_ms = run_timed(lambda: S.check(S.lib().snch_closest_silhouette_ex(
+        sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d), ctypes.byref(iopts), S._stream())), 2, 1)
+
+    # visit counters of the same launch (one extra launch, counters on)
+    stats = torch.zeros(5, dtype=torch.int64, device=dev)
+    e_chk, d_chk = sc.closest_silhouette(q, stats=stats, order=order)
+    assert torch.equal(e_chk, out_e) and torch.equal(d_chk, out_d)
+    box, leaf, edge, node, cone = (v / n for v in stats.tolist())
+    per_q = S_IN + S_OUT + node * S_NODE + cone * S_CONE + leaf * S_LEAF + edge * S_EDGE
+    achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_query": per_q,
+                "kernel": "k_closest_silhouette<256,false,3>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
+                "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms,
+                "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
+
+    # learned-bound form: seed radius from the Fourier MLP (BF16 MFMA), then
+    # the seeded exact traversal; a step is inference + query, results equal
+    # the plain step's bit for bit
+    learned = None
+    if not args.no_learned:
+        lo32, hi32 = lo.astype(np.float32), hi.astype(np.float32)
+        tp = torch.from_numpy(W.uniform_points(args.train_points, lo, hi, seed=21)).to(dev)
+        _, td = sc.closest_silhouette(tp)
+        mlp = S.LearnedBound(lo32, hi32, sigma=4.0, seed=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = mlp.train(tp, td, batch=4096, epochs=args.train_epochs, lr=3e-3)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        seed_r = torch.empty(n, dtype=torch.float32, device=dev)
+        out_e2 = torch.empty_like(out_e)
+        out_d2 = torch.empty_like(out_d)
+
+        def lstep():
+            mlp.seed_radius(q, rel=args.seed_rel, abs_margin=args.seed_abs, out=seed_r)
+            opts = S.QueryOpts(0, 0, None, None)
+            S.check(S.lib().snch_closest_silhouette_ex(sc._h, S._ptr(q), n, 0, S._ptr(seed_r), S._ptr(out_e2),
+                                                       S._ptr(out_d2), ctypes.byref(opts), S._stream()))
+
+        lwall, lms = run_timed(lstep, args.steps, args.warmup)
+        assert torch.equal(out_e2, out_e) and torch.equal(out_d2, out_d), "seeded result differs"
+        st2 = torch.zeros(5, dtype=torch.int64, device=dev)
+        sc.closest_silhouette(q, seed_radius=seed_r, stats=st2)
+        fallback = float(((out_d > seed_r) | ~torch.isfinite(out_d)).float().mean())
+        learned = {"value": n * args.steps / lwall, "ms_per_step": lwall * 1000.0 / args.steps,
+                   "train_s": t_train, "train_points": args.train_points, "train_epochs": args.train_epochs,
+                   "loss_first_last": [float(losses[0]), float(losses[-1])],
+                   "nodes_per_query": st2[3].item() / n, "seed_rel": args.seed_rel, "seed_abs": args.seed_abs,
+                   "fallback_fraction": fallback}
 
     # other queries (reported beside the headline)
     extra = {}
@@ -144,16 +216,6 @@ def main():
     extra["closest_point_qps"] = n / timed(lambda: sc.closest_point(q))
     extra["ray_qps"] = n / timed(lambda: sc.ray_intersect(q, d))
     extra["sample_in_sphere_qps"] = n / timed(lambda: sc.sample_in_sphere(q, r, u))
-
-    cf = counts_fixture()
-    roofline = None
-    if cf is not None:
-        c = cf["silhouette"]
-        per_q = S_IN + S_OUT + c["box_tests"] * S_BOX + c["leaf_evals"] * S_LEAF + c["edge_tests"] * S_EDGE
-        achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "bytes_per_query": per_q, "kernel": "k_closest_silhouette<256>", "kernel_ms": kernel_ms}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -178,6 +240,7 @@ def main():
         "builds_per_s": 1000.0 / build_ms,
         "build_ms": build_ms,
         "scene_create_s": t_scene,
+        "learned_bound": learned,
         **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,

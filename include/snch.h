@@ -50,12 +50,20 @@ typedef struct snch_build_opts {
     uint32_t leaf_size; /* 1 = the reference topology (bvh.cuh:200-229); 0 means 1 */
 } snch_build_opts;
 
+#define SNCH_ORDER_MIN_QUERIES 65536u
+
 /* Traversal tuning (chosen by the autotuner; defaults are safe) */
 typedef struct snch_query_opts {
     uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (256) */
-    uint32_t coherent;   /* 1 = reorder queries by Morton code before traversal */
-    uint64_t *stats;     /* optional DEVICE uint64[4], accumulated: child box tests, leaf
-                            evaluations, silhouette-edge tests, nodes visited */
+    uint32_t coherent;   /* visiting order: 0 = Morton order of the query points when
+                            n >= SNCH_ORDER_MIN_QUERIES, 1 = always Morton order,
+                            2 = input order.  Results are written in input order and
+                            do not depend on the visiting order. */
+    uint64_t *stats;     /* optional DEVICE uint64[5], accumulated: child box tests, leaf
+                            evaluations, primitive / silhouette tests, internal nodes
+                            visited, cone records read (silhouette query only) */
+    const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
+                              snch_query_order): the visiting order; overrides coherent */
 } snch_query_opts;
 
 const char *snch_last_error(void);
@@ -115,6 +123,10 @@ int snch_sample_in_sphere(const snch_scene *scene, const float *c, const float *
 int snch_sample_on_object(const snch_scene *scene, const int32_t *idx, const float *uv, uint32_t n, float *out_pt,
                           void *stream);
 /* Same queries with explicit traversal options. */
+/* Morton visiting order of n query points (dim floats each) in the scene's
+ * root box: out_perm (DEVICE uint32[n]) receives a permutation of [0, n).
+ * No reference counterpart: the reference visits queries in input order. */
+int snch_query_order(const snch_scene *scene, const float *q, uint32_t n, uint32_t *out_perm, void *stream);
 int snch_closest_point_ex(const snch_scene *scene, const float *q, uint32_t n, uint32_t *out_idx, float *out_dist,
                           const snch_query_opts *opts, void *stream);
 int snch_ray_ex(const snch_scene *scene, const float *o, const float *d, const float *tmax, uint32_t n, int any_hit,

@@ -52,14 +52,15 @@ class BuildOpts(ctypes.Structure):
 
 
 class QueryOpts(ctypes.Structure):
-    _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p)]
+    _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p),
+                ("order", ctypes.c_void_p)]
 
 
 EXPORTED = [
     "snch_last_error", "snch_version", "snch_context_create", "snch_context_destroy", "snch_scene_create_tri",
     "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_info", "snch_scene_build", "snch_scene_build_time",
-    "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette", "snch_closest_silhouette_ex",
-    "snch_ray", "snch_ray_ex",
+    "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
+    "snch_closest_silhouette_ex", "snch_ray", "snch_ray_ex",
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
     "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
@@ -78,6 +79,7 @@ def _declare(L):
     L.snch_scene_info.argtypes = [vp, vp, vp, vp]
     L.snch_scene_build.argtypes = [vp, vp, vp]
     L.snch_scene_build_time.argtypes = [vp, vp]
+    L.snch_query_order.argtypes = [vp, vp, u32, vp, vp]
     L.snch_closest_point.argtypes = [vp, vp, u32, vp, vp, vp]
     L.snch_closest_silhouette.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp]
     L.snch_closest_silhouette_ex.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp, vp]
@@ -138,6 +140,22 @@ def _dev_f32(x, cols, device):
     return t.contiguous()
 
 
+def _stats_ptr(stats):
+    if stats is None:
+        return None
+    if stats.dtype.itemsize != 8 or stats.numel() < 5 or not stats.is_cuda or not stats.is_contiguous():
+        raise ValueError("stats must be a contiguous int64 device tensor of at least 5 counters")
+    return stats.data_ptr()
+
+
+def _order_ptr(order, n):
+    if order is None:
+        return None
+    if order.dtype.itemsize != 4 or order.numel() != n or not order.is_cuda or not order.is_contiguous():
+        raise ValueError("order must be a contiguous 32-bit device tensor with one entry per query")
+    return order.data_ptr()
+
+
 class Scene:
     """``lbvh::scene<3>`` on one GPU (scene.cuh:705-1268)."""
 
@@ -185,20 +203,30 @@ class Scene:
         check(lib().snch_scene_build_time(self._h, ctypes.byref(ms)))
         return ms.value
 
-    def closest_point(self, q, stats=None):
-        """stats: optional int64 CUDA tensor of 4 counters (box tests, leaf
-        evaluations, edge tests, nodes visited), accumulated."""
+    def query_order(self, q):
+        """Morton visiting order of the query points (int32 device tensor)."""
+        import torch
+
+        q = _dev_f32(q, self.dim, self.device)
+        perm = torch.empty(q.shape[0], dtype=torch.int32, device=q.device)
+        check(lib().snch_query_order(self._h, _ptr(q), q.shape[0], _ptr(perm), _stream()))
+        return perm
+
+    def closest_point(self, q, stats=None, coherent=0, order=None):
+        """stats: optional int64 CUDA tensor of 5 counters (child box tests,
+        leaf evaluations, primitive tests, internal nodes visited, cone
+        records read), accumulated."""
         import torch
 
         q = _dev_f32(q, self.dim, self.device)
         n = q.shape[0]
         idx = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(0, 0, stats.data_ptr() if stats is not None else None)
+        opts = QueryOpts(0, coherent, _stats_ptr(stats), _order_ptr(order, n))
         check(lib().snch_closest_point_ex(self._h, _ptr(q), n, _ptr(idx), _ptr(dist), ctypes.byref(opts), _stream()))
         return idx, dist
 
-    def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None):
+    def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None, coherent=0, order=None):
         import torch
 
         q = _dev_f32(q, self.dim, self.device)
@@ -206,7 +234,7 @@ class Scene:
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(block_size, 0, stats.data_ptr() if stats is not None else None)
+        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n))
         check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
                                                _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist
@@ -222,7 +250,7 @@ class Scene:
         t = torch.empty(n, dtype=torch.float32, device=o.device)
         uv = torch.empty((n, 2) if self.dim == 3 else (n,), dtype=torch.float32, device=o.device)
         idx = torch.empty(n, dtype=torch.int32, device=o.device)
-        opts = QueryOpts(0, 0, stats.data_ptr() if stats is not None else None)
+        opts = QueryOpts(0, 0, _stats_ptr(stats))
         check(lib().snch_ray_ex(self._h, _ptr(o), _ptr(d), _ptr(tm), n, int(bool(any_hit)), _ptr(hit), _ptr(t),
                                 _ptr(uv), _ptr(idx), ctypes.byref(opts), _stream()))
         return hit.bool(), t, uv, idx

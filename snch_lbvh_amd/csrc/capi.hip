@@ -390,13 +390,42 @@ int snch_scene_build_time(const snch_scene* s, float* ms)
         }                                                        \
     } while (0)
 
+// query visiting order: 0 = Morton order from SNCH_ORDER_MIN_QUERIES queries
+// up, 1 = always Morton order, 2 = input order
+static bool want_order(const snch_query_opts* opts, uint32_t n)
+{
+    const uint32_t mode = opts ? opts->coherent : 0u;
+    return mode == 1u ? n > 1 : (mode == 0u && n >= SNCH_ORDER_MIN_QUERIES);
+}
+
+int snch_query_order(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_perm, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_query_order");
+    SNCH_CHECK_ARG(n == 0 || (q && out_perm), "snch_query_order: null pointer");
+    if (n == 0)
+        return SNCH_OK;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t* perm = nullptr;
+    SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_query_order");
+    if (perm)
+        SNCH_HIP(hipMemcpyAsync(out_perm, perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st),
+                 "snch_query_order: copy");
+    else
+        SNCH_HIP(hipMemsetAsync(out_perm, 0, sizeof(uint32_t), st), "snch_query_order: copy");
+    return SNCH_OK;
+}
+
 int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_idx, float* out_dist,
                           const snch_query_opts* opts, void* stream)
 {
     SNCH_NEED_BUILT(s, "snch_closest_point");
     SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_closest_point: null pointer");
-    SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, opts ? opts->stats : nullptr,
-                                  static_cast<hipStream_t>(stream)),
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t* perm = opts ? opts->order : nullptr;
+    if (!perm && want_order(opts, n))
+        SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_closest_point: ordering");
+    SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, perm,
+                                  opts ? opts->stats : nullptr, st),
              "snch_closest_point");
     return SNCH_OK;
 }
@@ -415,8 +444,13 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block,
-                                       opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t* perm = opts ? opts->order : nullptr;
+    if (!perm && want_order(opts, n))
+        SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
+                 "snch_closest_silhouette: ordering");
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block, perm,
+                                       opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
     return SNCH_OK;
 }

@@ -61,6 +61,11 @@ struct DeviceBuffer {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// per-scene scratch of the query reordering (Morton sort of query points)
+struct QueryScratch {
+    DeviceBuffer codes, idx, codes_alt, idx_alt, hist;
+};
+
 struct BuildScratch {
     DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
         status;
@@ -89,6 +94,7 @@ struct snch_scene {
     // device tree
     snch::DeviceBuffer tnode, tcone, tleaf, tedge, nbox, ncone;
     snch::BuildScratch scratch;
+    mutable snch::QueryScratch qscratch; // used by const query entry points
     bool built = false;
     bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
     uint32_t build_flags = 0;
@@ -105,12 +111,15 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st);
 // sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt);
-// query.hip (dim = 2 or 3)
+// query.hip (dim = 2 or 3).  perm (nullable): slot -> query index, the
+// order in which queries are visited (query_order computes a Morton order)
+hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
+                       const uint32_t** perm);
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
-                                uint64_t* stats, hipStream_t st);
+                                const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint64_t* stats,
-                                     hipStream_t st);
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block,
+                                     const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

@@ -87,20 +87,20 @@ struct LaneStack {
 
 // per-query visit counters, summed over the wave before one atomic per wave
 struct Counts {
-    uint32_t box = 0, leaf = 0, edge = 0, node = 0;
+    uint32_t box = 0, leaf = 0, edge = 0, node = 0, cone = 0;
 };
 __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long* stats)
 {
-    unsigned long long v[4] = {c.box, c.leaf, c.edge, c.node};
+    unsigned long long v[5] = {c.box, c.leaf, c.edge, c.node, c.cone};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 5; ++k) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
             v[k] += __shfl_xor(v[k], o, 64);
     }
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 5; ++k)
             atomicAdd(stats + k, v[k]);
     }
 }
@@ -163,18 +163,20 @@ __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, flo
 template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* __restrict__ q, uint32_t n,
                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
-                                                      uint32_t* __restrict__ err, unsigned long long* stats)
+                                                      const uint32_t* __restrict__ perm, uint32_t* __restrict__ err,
+                                                      unsigned long long* stats)
 {
     __shared__ uint2 stk[SS * BS];
-    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
         n = min(n, *t.n_dev);
-    if (qi >= n) {
+    if (slot >= n) {
         if (STATS)
             flush_counts(cnt, stats);
         return;
     }
+    const uint32_t qi = perm ? min(perm[slot], n - 1) : slot;
     const v3 p = load_point<DIM>(q, qi);
     float best = F_INF;
     uint32_t best_idx = INVALID;
@@ -301,18 +303,20 @@ template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
+                                                           const uint32_t* __restrict__ perm,
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
     __shared__ uint2 stk[SS * BS];
-    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
         n = min(n, *t.n_dev);
-    if (qi >= n) {
+    if (slot >= n) {
         if (STATS)
             flush_counts(cnt, stats);
         return;
     }
+    const uint32_t qi = perm ? min(perm[slot], n - 1) : slot;
     const v3 p = load_point<DIM>(q, qi);
     float best = F_INF;
     uint32_t best_e = INVALID;
@@ -347,6 +351,8 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
                     cnt.box += 2;
                 }
                 if (hL || hR) {
+                    if (STATS)
+                        cnt.cone += 1;
                     const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
                     // query.cuh:366-367
                     hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
@@ -644,6 +650,24 @@ __global__ __launch_bounds__(256) void k_sample_on_object(const float4* __restri
     }
 }
 
+// Morton code of each query point in the root box (clamped to it), used to
+// visit queries in spatial order: neighbouring lanes then walk mostly the
+// same nodes, which keeps a wave's memory accesses and branches together
+template <int DIM>
+__global__ __launch_bounds__(256) void k_query_codes(const float4* __restrict__ nbox, const float* __restrict__ q,
+                                                     uint32_t n, uint32_t* __restrict__ codes,
+                                                     uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 lo = nbox[0], hi = nbox[1];
+    const v3 p = load_point<DIM>(q, i);
+    const float x = (p.x - lo.x) / (hi.x - lo.x), y = (p.y - lo.y) / (hi.y - lo.y);
+    codes[i] = DIM == 3 ? morton3(mk(x, y, (p.z - lo.z) / (hi.z - lo.z))) : morton2(x, y);
+    idx[i] = i;
+}
+
 inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
 
 } // namespace
@@ -659,8 +683,28 @@ inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
         }                           \
     } while (0)
 
+hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
+                       const uint32_t** perm)
+{
+    *perm = nullptr;
+    if (n < 2)
+        return hipSuccess;
+    hipError_t e;
+    if ((e = w.codes.ensure(sizeof(uint32_t) * n)) != hipSuccess || (e = w.idx.ensure(sizeof(uint32_t) * n)) ||
+        (e = w.codes_alt.ensure(sizeof(uint32_t) * n)) || (e = w.idx_alt.ensure(sizeof(uint32_t) * n)))
+        return e;
+    SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_query_codes<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, t.nbox,
+                                              q, n, w.codes.as<uint32_t>(), w.idx.as<uint32_t>()));
+    bool in_alt = false;
+    if ((e = radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
+                              w.idx_alt.as<uint32_t>(), n, 30, w.hist, st, &in_alt)) != hipSuccess)
+        return e;
+    *perm = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
+    return hipGetLastError();
+}
+
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
-                                uint64_t* stats, hipStream_t st)
+                                const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
@@ -668,39 +712,39 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
     SNCH_DISPATCH_DIM(dim, {
         if (stats)
             hipLaunchKernelGGL((k_closest_point<256, true, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
-                               idx, dist, traversal_error_word(), sp);
+                               idx, dist, perm, traversal_error_word(), sp);
         else
             hipLaunchKernelGGL((k_closest_point<256, false, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
-                               idx, dist, traversal_error_word(), sp);
+                               idx, dist, perm, traversal_error_word(), sp);
     });
     return hipGetLastError();
 }
 
 template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
-                          float* dist, unsigned long long* stats, hipStream_t st)
+                          float* dist, const uint32_t* perm, unsigned long long* stats, hipStream_t st)
 {
     if (stats)
         hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
-                           flip, seed, sil, dist, traversal_error_word(), stats);
+                           flip, seed, sil, dist, perm, traversal_error_word(), stats);
     else
         hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
-                           flip, seed, sil, dist, traversal_error_word(), stats);
+                           flip, seed, sil, dist, perm, traversal_error_word(), stats);
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint64_t* stats,
-                                     hipStream_t st)
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block,
+                                     const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
-        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, sp, st); break;
+        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
+        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
         }
     });
     return hipGetLastError();

