@@ -439,15 +439,49 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
     return r;
 }
 
-// Bottom-up refit.  A workgroup owns RT consecutive leaves (sorted order);
-// an internal node whose leaf range lies inside that window has both
-// subtrees built by the same workgroup, so its arrival counter lives in LDS
-// (indexed by the node id, one end of its range) and ordering needs only
-// workgroup-scope fences.  Only nodes whose range straddles windows use the
-// global counter with agent-scope acquire-release (an L2 write-back and
-// invalidate on this multi-XCD part), which is ~log2(N / RT) levels per
-// window boundary instead of every level of every leaf.
+// Bottom-up refit in two launches, with no device-wide fence.
+//
+// k_refit: a workgroup owns RT consecutive leaves (sorted order).  An
+// internal node whose leaf range lies inside that window has both subtrees
+// built by the same workgroup, so its arrival counter lives in LDS (indexed
+// by the node's Karras index, one end of its range) and ordering needs only
+// workgroup-scope fences.  A climb that reaches a node straddling two windows
+// stops there and appends the finished child to a list.
+// k_refit_top: one workgroup climbs from the listed children through the
+// straddling nodes (a few per window boundary) with arrival counters in
+// global memory; all of them are touched by this one workgroup, so
+// workgroup scope orders them too, and the launch boundary publishes the
+// first launch's results.
 constexpr int RT = 1024;
+
+// the second arrival at internal node p merges its children (bvh.cuh:520-604)
+template <int DIM>
+__device__ __forceinline__ void refit_merge(uint32_t p, uint32_t N, const uint2* __restrict__ child,
+                                            int conservative, float4* nbox, float4* ncone,
+                                            float4* __restrict__ tnode, float4* __restrict__ tcone)
+{
+    const uint2 ch = child[p];
+    const box3 lb = ldbox(nbox, ch.x), rb = ldbox(nbox, ch.y);
+    const cone3 lc = ldcone(ncone, ch.x), rc = ldcone(ncone, ch.y);
+    const box3 pb = box_merge(lb, rb);
+    const cone3 pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
+                                           conservative != 0)
+                              : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
+                                             conservative != 0);
+    nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
+    nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
+    ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
+    ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
+    const uint32_t lref = ch.x < N - 1 ? ch.x : (LEAF_BIT | (ch.x - (N - 1)));
+    const uint32_t rref = ch.y < N - 1 ? ch.y : (LEAF_BIT | (ch.y - (N - 1)));
+    tnode[4 * p + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
+    tnode[4 * p + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
+    tnode[4 * p + 2] = make_float4(rb.lo.z, rb.hi.x, rb.hi.y, rb.hi.z);
+    tnode[4 * p + 3] = make_float4(__uint_as_float(lref), __uint_as_float(rref), 0.0f, 0.0f);
+    tcone[3 * p + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
+    tcone[3 * p + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
+    tcone[3 * p + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+}
 
 template <int DIM>
 __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
@@ -455,9 +489,10 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                                               const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
                                               const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
                                               const uint2* __restrict__ child, const uint2* __restrict__ range,
-                                              uint32_t* flags, uint32_t N, int conservative, float4* nbox,
-                                              float4* ncone, float4* __restrict__ tnode, float4* __restrict__ tcone,
-                                              float4* __restrict__ tleaf)
+                                              uint32_t N, int conservative, float4* nbox, float4* ncone,
+                                              float4* __restrict__ tnode, float4* __restrict__ tcone,
+                                              float4* __restrict__ tleaf, uint32_t* __restrict__ top,
+                                              uint32_t top_cap)
 {
     __shared__ uint32_t lflag[RT];
     const uint32_t base = blockIdx.x * RT;
@@ -469,7 +504,7 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
     const uint32_t orig = sidx[j];
     if (orig >= N)
         return; // not a permutation: leave the node unwritten rather than read out of bounds
-    const uint32_t node = N - 1 + j;
+    uint32_t node = N - 1 + j;
     nbox[2 * node] = leafbox[2 * orig];
     nbox[2 * node + 1] = leafbox[2 * orig + 1];
     ncone[2 * node] = leafcone[2 * orig];
@@ -493,44 +528,63 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
     // guards against a malformed parent array
     for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
         const uint2 rg = range[p];
-        uint32_t old;
-        if (rg.x >= base && rg.y < base + RT) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            old = atomicAdd(&lflag[p - base], 1u);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        } else {
-            old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(rg.x >= base && rg.y < base + RT)) {
+            // straddling node: finished later by k_refit_top
+            const uint32_t k = atomicAdd(top + top_cap, 1u);
+            if (k < top_cap)
+                top[k] = node;
+            return;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const uint32_t old = atomicAdd(&lflag[p - base], 1u);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (old == 0u)
             return; // the sibling subtree is not finished; its last thread continues
-        const uint2 ch = child[p];
-        const box3 lb = ldbox(nbox, ch.x), rb = ldbox(nbox, ch.y);
-        const cone3 lc = ldcone(ncone, ch.x), rc = ldcone(ncone, ch.y);
-        const box3 pb = box_merge(lb, rb);
-        const cone3 pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
-                                               conservative != 0)
-                                  : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
-                                                 conservative != 0);
-        nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
-        nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
-        ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
-        ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
-        const uint32_t lref = ch.x < N - 1 ? ch.x : (LEAF_BIT | (ch.x - (N - 1)));
-        const uint32_t rref = ch.y < N - 1 ? ch.y : (LEAF_BIT | (ch.y - (N - 1)));
-        tnode[4 * p + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
-        tnode[4 * p + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
-        tnode[4 * p + 2] = make_float4(rb.lo.z, rb.hi.x, rb.hi.y, rb.hi.z);
-        tnode[4 * p + 3] = make_float4(__uint_as_float(lref), __uint_as_float(rref), 0.0f, 0.0f);
-        tcone[3 * p + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
-        tcone[3 * p + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
-        tcone[3 * p + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+        refit_merge<DIM>(p, N, child, conservative, nbox, ncone, tnode, tcone);
+        node = p;
         p = parent[p];
     }
 }
 
+template <int DIM>
+__global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ child, uint32_t* flags, uint32_t N,
+                                                    int conservative, float4* nbox, float4* ncone,
+                                                    float4* __restrict__ tnode, float4* __restrict__ tcone,
+                                                    const uint32_t* __restrict__ top, uint32_t top_cap)
+{
+    const uint32_t count = min(top[top_cap], top_cap);
+    for (uint32_t k = threadIdx.x; k < count; k += 1024) {
+        uint32_t node = top[k];
+        if (node >= 2 * N - 1)
+            continue;
+        uint32_t p = parent[node];
+        for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
+            const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u)
+                break;
+            refit_merge<DIM>(p, N, child, conservative, nbox, ncone, tnode, tcone);
+            node = p;
+            p = parent[p];
+        }
+    }
+}
+
 __global__ void k_single_root(uint32_t* parent) { parent[0] = INVALID; }
+__global__ void k_zero_word(uint32_t* w) { *w = 0u; }
 
 inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
+
+#define SNCH_DISPATCH_REFIT(dim, ...) \
+    do {                              \
+        if ((dim) == 3) {             \
+            constexpr int DIM = 3;    \
+            __VA_ARGS__;              \
+        } else {                      \
+            constexpr int DIM = 2;    \
+            __VA_ARGS__;              \
+        }                             \
+    } while (0)
 
 } // namespace
 
@@ -556,6 +610,8 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
     SNCH_TRY(w.child.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(w.range.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * (N > 1 ? N - 1 : 1)));
+    const uint32_t top_cap = 2 * N;
+    SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (top_cap + 1)));
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
@@ -587,20 +643,19 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
                            w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>());
     else
         hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>());
-    if (s->dim == 3)
-        hipLaunchKernelGGL(k_refit<3>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
+    uint32_t* top = w.top.as<uint32_t>();
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, st, top + top_cap);
+    SNCH_DISPATCH_REFIT(s->dim, {
+        hipLaunchKernelGGL(k_refit<DIM>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
-                           w.flags.as<uint32_t>(), N, conservative,
-                           s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
-                           s->tcone.as<float4>(), s->tleaf.as<float4>());
-    else
-        hipLaunchKernelGGL(k_refit<2>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
-                           s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
-                           w.flags.as<uint32_t>(), N, conservative,
-                           s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
-                           s->tcone.as<float4>(), s->tleaf.as<float4>());
+                           N, conservative, s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
+                           s->tcone.as<float4>(), s->tleaf.as<float4>(), top, top_cap);
+        if (N > RT)
+            hipLaunchKernelGGL(k_refit_top<DIM>, dim3(1), dim3(1024), 0, st, w.parent.as<uint32_t>(),
+                               w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative, s->nbox.as<float4>(),
+                               s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(), top, top_cap);
+    });
     s->sorted_in_alt = in_alt;
 #undef SNCH_TRY
     return hipGetLastError();

@@ -68,7 +68,7 @@ struct QueryScratch {
 
 struct BuildScratch {
     DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
-        status;
+        top, status;
 };
 
 } // namespace snch
