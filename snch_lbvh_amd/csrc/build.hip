@@ -23,6 +23,8 @@
 //                     data and the packed traversal record.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace snch {
@@ -358,9 +360,19 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const 
 }
 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
+constexpr uint32_t RT = 1024; // leaves of one refit subtree (threads of one refit workgroup)
+
+__device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_t node)
+{
+    const uint32_t k = atomicAdd(cuts + cap, 1u);
+    if (k < cap)
+        cuts[k] = node;
+}
+
 __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
                                                uint32_t N, uint32_t* __restrict__ parent, uint2* __restrict__ child,
-                                               uint2* __restrict__ range, uint32_t* __restrict__ flags)
+                                               uint2* __restrict__ range, uint32_t* __restrict__ flags,
+                                               uint32_t* __restrict__ cuts, uint32_t cut_cap)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     if (i >= N - 1)
@@ -419,6 +431,15 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     parent[left] = i;
     parent[right] = i;
     flags[i] = 0u;
+    // refit subtrees ("cuts"): maximal subtrees of at most RT leaves
+    if (last - first + 1 > RT) {
+        if (g - first + 1 <= RT)
+            append_cut(cuts, cut_cap, left);
+        if (last - g <= RT)
+            append_cut(cuts, cut_cap, right);
+    } else if (i == 0) {
+        append_cut(cuts, cut_cap, 0u);
+    }
 }
 
 __device__ __forceinline__ box3 ldbox(const float4* nb, uint32_t n)
@@ -441,33 +462,28 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
 
 // Bottom-up refit in two launches, with no device-wide fence.
 //
-// k_refit: a workgroup owns RT consecutive leaves (sorted order).  An
-// internal node whose leaf range lies inside that window has both subtrees
-// built by the same workgroup, so its arrival counter lives in LDS (indexed
-// by the node's Karras index, one end of its range) and ordering needs only
-// workgroup-scope fences.  A climb that reaches a node straddling two windows
-// stops there and appends the finished child to a list.
-// k_refit_top: one workgroup climbs from the listed children through the
-// straddling nodes (a few per window boundary) with arrival counters in
-// global memory; all of them are touched by this one workgroup, so
-// workgroup scope orders them too, and the launch boundary publishes the
-// first launch's results.
-constexpr int RT = 1024;
+// k_karras lists the maximal subtrees of at most RT leaves ("cuts"); they are
+// disjoint and cover every leaf.  k_refit gives each cut to one workgroup:
+// thread t initialises leaf first + t, then climbs; the arrival counter of an
+// internal node lives in LDS at (Karras index - first), and the box and cone
+// of every node of the cut are kept in LDS too, so the second arrival merges
+// its children without a trip to memory.  Workgroup-scope fences order it.
+// k_refit_top: one workgroup climbs from the cut roots through the nodes
+// above them (about 2N/RT of them) with arrival counters in global memory;
+// all of them are touched by this one workgroup, so workgroup scope orders
+// them too, and the launch boundary publishes the first launch's results.
 
-// the second arrival at internal node p merges its children (bvh.cuh:520-604)
+// the parent box and cone of two children (bvh.cuh:520-604), and the records
+// of internal node p
 template <int DIM>
-__device__ __forceinline__ void refit_merge(uint32_t p, uint32_t N, const uint2* __restrict__ child,
-                                            int conservative, float4* nbox, float4* ncone,
-                                            float4* __restrict__ tnode, float4* __restrict__ tcone)
+__device__ __forceinline__ void refit_write(uint32_t p, uint32_t N, uint2 ch, const box3& lb, const box3& rb,
+                                            const cone3& lc, const cone3& rc, int conservative, box3& pb,
+                                            cone3& pc, float4* nbox, float4* ncone, float4* __restrict__ tnode,
+                                            float4* __restrict__ tcone)
 {
-    const uint2 ch = child[p];
-    const box3 lb = ldbox(nbox, ch.x), rb = ldbox(nbox, ch.y);
-    const cone3 lc = ldcone(ncone, ch.x), rc = ldcone(ncone, ch.y);
-    const box3 pb = box_merge(lb, rb);
-    const cone3 pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
-                                           conservative != 0)
-                              : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb),
-                                             conservative != 0);
+    pb = box_merge(lb, rb);
+    pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0)
+                  : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0);
     nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
     nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
     ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
@@ -483,6 +499,28 @@ __device__ __forceinline__ void refit_merge(uint32_t p, uint32_t N, const uint2*
     tcone[3 * p + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
 }
 
+// box + cone of one node in LDS, structure of arrays over 2 * RT slots (slot
+// t: leaf first + t of the cut; slot RT + k: internal node of Karras index
+// first + k)
+struct NodeLds {
+    float v[11][2 * RT];
+    __device__ __forceinline__ void put(uint32_t s, const box3& b, const cone3& c)
+    {
+        v[0][s] = b.lo.x; v[1][s] = b.lo.y; v[2][s] = b.lo.z;
+        v[3][s] = b.hi.x; v[4][s] = b.hi.y; v[5][s] = b.hi.z;
+        v[6][s] = c.axis.x; v[7][s] = c.axis.y; v[8][s] = c.axis.z;
+        v[9][s] = c.half_angle; v[10][s] = c.radius;
+    }
+    __device__ __forceinline__ void get(uint32_t s, box3& b, cone3& c) const
+    {
+        b.lo = mk(v[0][s], v[1][s], v[2][s]);
+        b.hi = mk(v[3][s], v[4][s], v[5][s]);
+        c.axis = mk(v[6][s], v[7][s], v[8][s]);
+        c.half_angle = v[9][s];
+        c.radius = v[10][s];
+    }
+};
+
 template <int DIM>
 __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
@@ -491,58 +529,82 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                                               const uint2* __restrict__ child, const uint2* __restrict__ range,
                                               uint32_t N, int conservative, float4* nbox, float4* ncone,
                                               float4* __restrict__ tnode, float4* __restrict__ tcone,
-                                              float4* __restrict__ tleaf, uint32_t* __restrict__ top,
-                                              uint32_t top_cap)
+                                              float4* __restrict__ tleaf, const uint32_t* __restrict__ cuts,
+                                              uint32_t cut_cap)
 {
     __shared__ uint32_t lflag[RT];
-    const uint32_t base = blockIdx.x * RT;
-    const uint32_t j = base + threadIdx.x;
-    lflag[threadIdx.x] = 0u;
-    __syncthreads();
-    if (j >= N)
-        return;
-    const uint32_t orig = sidx[j];
-    if (orig >= N)
-        return; // not a permutation: leave the node unwritten rather than read out of bounds
-    uint32_t node = N - 1 + j;
-    nbox[2 * node] = leafbox[2 * orig];
-    nbox[2 * node + 1] = leafbox[2 * orig + 1];
-    ncone[2 * node] = leafcone[2 * orig];
-    ncone[2 * node + 1] = leafcone[2 * orig + 1];
-    {
-        const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
-        if (DIM == 3) {
-            const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], c = V[F[3 * orig + 2]];
-            tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
-            tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
-            tleaf[3 * j + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(ec));
+    __shared__ NodeLds nd;
+    const uint32_t t = threadIdx.x;
+    const uint32_t count = min(cuts[cut_cap], cut_cap);
+    for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
+        const uint32_t root = cuts[c];
+        uint32_t first, last;
+        if (root >= N - 1) {
+            first = last = root - (N - 1);
         } else {
-            const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
-            tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(orig));
-            tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(es));
-            tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
+            const uint2 rg = range[root];
+            first = rg.x;
+            last = rg.y;
         }
-    }
-    uint32_t p = parent[node];
-    // the climb is at most the tree depth (<= 64 key bits); the cap only
-    // guards against a malformed parent array
-    for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
-        const uint2 rg = range[p];
-        if (!(rg.x >= base && rg.y < base + RT)) {
-            // straddling node: finished later by k_refit_top
-            const uint32_t k = atomicAdd(top + top_cap, 1u);
-            if (k < top_cap)
-                top[k] = node;
-            return;
+        lflag[t] = 0u;
+        __syncthreads();
+        const uint32_t j = first + t;
+        if (last - first < RT && j <= last && last < N) {
+            const uint32_t orig = sidx[j];
+            if (orig < N) { // a permutation always; otherwise the node stays unwritten
+                uint32_t node = N - 1 + j;
+                const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
+                const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
+                nbox[2 * node] = b0;
+                nbox[2 * node + 1] = b1;
+                ncone[2 * node] = c0;
+                ncone[2 * node + 1] = c1;
+                {
+                    box3 b;
+                    b.lo = mk(b0.x, b0.y, b0.z);
+                    b.hi = mk(b1.x, b1.y, b1.z);
+                    cone3 cc;
+                    cc.axis = mk(c0.x, c0.y, c0.z);
+                    cc.half_angle = c0.w;
+                    cc.radius = c1.x;
+                    nd.put(t, b, cc);
+                }
+                const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
+                if (DIM == 3) {
+                    const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], d = V[F[3 * orig + 2]];
+                    tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
+                    tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
+                    tleaf[3 * j + 2] = make_float4(d.x, d.y, d.z, __uint_as_float(ec));
+                } else {
+                    const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
+                    tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(orig));
+                    tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(es));
+                    tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
+                }
+                // climb inside the cut (at most its depth, < RT levels)
+                for (uint32_t guard = 0; node != root && guard < RT; ++guard) {
+                    const uint32_t p = parent[node];
+                    if (p >= N - 1 || p < first || p - first >= RT)
+                        break; // malformed tree: never taken for a Karras tree
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    const uint32_t old = atomicAdd(&lflag[p - first], 1u);
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    if (old == 0u)
+                        break; // the sibling subtree is not finished; its last thread continues
+                    const uint2 ch = child[p];
+                    const uint32_t sl = ch.x < N - 1 ? RT + (ch.x - first) : ch.x - (N - 1) - first;
+                    const uint32_t sr = ch.y < N - 1 ? RT + (ch.y - first) : ch.y - (N - 1) - first;
+                    box3 lb, rb, pb;
+                    cone3 lc, rc, pc;
+                    nd.get(sl, lb, lc);
+                    nd.get(sr, rb, rc);
+                    refit_write<DIM>(p, N, ch, lb, rb, lc, rc, conservative, pb, pc, nbox, ncone, tnode, tcone);
+                    nd.put(RT + (p - first), pb, pc);
+                    node = p;
+                }
+            }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        const uint32_t old = atomicAdd(&lflag[p - base], 1u);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        if (old == 0u)
-            return; // the sibling subtree is not finished; its last thread continues
-        refit_merge<DIM>(p, N, child, conservative, nbox, ncone, tnode, tcone);
-        node = p;
-        p = parent[p];
+        __syncthreads();
     }
 }
 
@@ -551,11 +613,11 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
                                                     const uint2* __restrict__ child, uint32_t* flags, uint32_t N,
                                                     int conservative, float4* nbox, float4* ncone,
                                                     float4* __restrict__ tnode, float4* __restrict__ tcone,
-                                                    const uint32_t* __restrict__ top, uint32_t top_cap)
+                                                    const uint32_t* __restrict__ cuts, uint32_t cut_cap)
 {
-    const uint32_t count = min(top[top_cap], top_cap);
+    const uint32_t count = min(cuts[cut_cap], cut_cap);
     for (uint32_t k = threadIdx.x; k < count; k += 1024) {
-        uint32_t node = top[k];
+        uint32_t node = cuts[k];
         if (node >= 2 * N - 1)
             continue;
         uint32_t p = parent[node];
@@ -563,14 +625,23 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
             const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u)
                 break;
-            refit_merge<DIM>(p, N, child, conservative, nbox, ncone, tnode, tcone);
+            const uint2 ch = child[p];
+            box3 pb;
+            cone3 pc;
+            refit_write<DIM>(p, N, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
+                             ldcone(ncone, ch.y), conservative, pb, pc, nbox, ncone, tnode, tcone);
             node = p;
             p = parent[p];
         }
     }
 }
 
-__global__ void k_single_root(uint32_t* parent) { parent[0] = INVALID; }
+__global__ void k_single_root(uint32_t* parent, uint32_t* cuts, uint32_t cut_cap)
+{
+    parent[0] = INVALID;
+    cuts[0] = 0u; // the single leaf is node 0 and its own cut
+    cuts[cut_cap] = 1u;
+}
 __global__ void k_zero_word(uint32_t* w) { *w = 0u; }
 
 inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
@@ -610,8 +681,9 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
     SNCH_TRY(w.child.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(w.range.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * (N > 1 ? N - 1 : 1)));
-    const uint32_t top_cap = 2 * N;
-    SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (top_cap + 1)));
+    const uint32_t cut_cap = N;
+    SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 1)));
+    uint32_t* cuts = w.top.as<uint32_t>();
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (N > 1 ? N - 1 : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
@@ -638,23 +710,26 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
                               w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, st, cuts + cut_cap);
     if (N > 1)
         hipLaunchKernelGGL(k_karras, dim3(blocks(N - 1)), dim3(BT), 0, st, codes, sidx, N, w.parent.as<uint32_t>(),
-                           w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>());
+                           w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), cuts, cut_cap);
     else
-        hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>());
-    uint32_t* top = w.top.as<uint32_t>();
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, st, top + top_cap);
+        hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>(), cuts, cut_cap);
+    // cuts hold at least N / RT leaves each but one per side of a big node,
+    // so about 2N / RT of them; each workgroup loops over its share
+    const uint32_t refit_blocks = std::max(1u, std::min(2u * (N + RT - 1) / RT + 1u,
+                                                        (uint32_t)std::max(1, s->ctx ? 2 * s->ctx->n_cu : 512)));
     SNCH_DISPATCH_REFIT(s->dim, {
-        hipLaunchKernelGGL(k_refit<DIM>, dim3((N + RT - 1) / RT), dim3(RT), 0, st, s->dV.as<float4>(),
+        hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RT), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
                            N, conservative, s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
-                           s->tcone.as<float4>(), s->tleaf.as<float4>(), top, top_cap);
+                           s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
         if (N > RT)
             hipLaunchKernelGGL(k_refit_top<DIM>, dim3(1), dim3(1024), 0, st, w.parent.as<uint32_t>(),
                                w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative, s->nbox.as<float4>(),
-                               s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(), top, top_cap);
+                               s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
     });
     s->sorted_in_alt = in_alt;
 #undef SNCH_TRY

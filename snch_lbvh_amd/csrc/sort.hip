@@ -93,6 +93,65 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ 
     }
 }
 
+// multi-block exclusive scan for long histograms: per-chunk sums, a scan of
+// the sums, then each chunk scanned with its offset (chunks of 4096)
+__device__ __forceinline__ uint32_t block_sum_1024(uint32_t x, uint32_t* wsum)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        x += __shfl_xor(x, o, 64);
+    if (lane == 0)
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        t += wsum[k];
+    return t;
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_sums(const uint32_t* __restrict__ a, uint32_t len,
+                                                     uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t idx = blockIdx.x * 4096 + 4 * threadIdx.x;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (idx < len)
+        v = *reinterpret_cast<const uint4*>(a + idx);
+    const uint32_t t = block_sum_1024(v.x + v.y + v.z + v.w, wsum);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* __restrict__ a, uint32_t len,
+                                                     const uint32_t* __restrict__ offs)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t idx = blockIdx.x * 4096 + 4 * tid;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (idx < len)
+        v = *reinterpret_cast<const uint4*>(a + idx);
+    const uint32_t s = v.x + v.y + v.z + v.w;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o)
+            incl += y;
+    }
+    if (lane == 63)
+        wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < w; ++k)
+        before += wsum[k];
+    const uint32_t ex = offs[blockIdx.x] + before + incl - s;
+    if (idx < len)
+        *reinterpret_cast<uint4*>(a + idx) = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
+}
+
 __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                    uint32_t n, int shift, uint32_t ntiles,
@@ -177,11 +236,25 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     hipError_t e = hist.ensure(sizeof(uint32_t) * (size_t)RADIX * ntiles);
     if (e != hipSuccess)
         return e;
+    const uint32_t hlen = RADIX * ntiles, nchunks = (hlen + 4095) / 4096;
+    const bool chunked = nchunks > 4;
+    if (chunked) {
+        e = hist.ensure(sizeof(uint32_t) * ((size_t)hlen + ((nchunks + 3) & ~3u)));
+        if (e != hipSuccess)
+            return e;
+    }
     uint32_t* h = hist.as<uint32_t>();
+    uint32_t* sums = h + hlen;
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_upsweep, dim3(ntiles), dim3(ST), 0, st, ki, n, shift, ntiles, h);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, h, (uint32_t)(RADIX * ntiles));
+        if (chunked) {
+            hipLaunchKernelGGL(k_chunk_sums, dim3(nchunks), dim3(1024), 0, st, h, hlen, sums);
+            hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, sums, (nchunks + 3) & ~3u);
+            hipLaunchKernelGGL(k_chunk_scan, dim3(nchunks), dim3(1024), 0, st, h, hlen, sums);
+        } else {
+            hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, h, hlen);
+        }
         hipLaunchKernelGGL(k_downsweep, dim3(ntiles), dim3(ST), 0, st, ki, vi, ko, vo, n, shift, ntiles, h);
         uint32_t* t = ki;
         ki = ko;

@@ -65,6 +65,26 @@ def test_build_bit_exact(S, oracle, name, reference_radius):
     np.testing.assert_array_equal(g["cones"], o.cones)
 
 
+def test_build_bit_exact_large(S, oracle):
+    """327,680 triangles: the radix sort's multi-block histogram scan and the
+    refit's top-level pass both run; the whole tree must equal the oracle's.
+    Then 300,000 queries in Morton order against input order."""
+    V, F = W.cut_torus(512, 320)
+    sc = S.Scene(V, F).build_bvh()
+    g = sc.export()
+    o = oracle.build_tree(V, F)
+    for k, ov in (("sorted_codes", o.sorted_codes), ("sorted_idx", o.sorted_idx), ("nodes", o.nodes),
+                  ("boxes", o.boxes), ("cones", o.cones)):
+        np.testing.assert_array_equal(g[k], ov, err_msg=k)
+    q = _queries(V, 300000, 13)
+    perm = sc.query_order(q).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(len(q)))
+    a = [t.cpu().numpy() for t in sc.closest_silhouette(q, coherent=1)]
+    b = [t.cpu().numpy() for t in sc.closest_silhouette(q, coherent=2)]
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_build_deterministic(S):
     V, F = W.cut_torus(128, 64)
     a = S.Scene(V, F).build_bvh().export()
