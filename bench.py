@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 13)
     ap.add_argument("--build-reps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--input-order", action="store_true", help="also time the kernel over unordered queries")
+    ap.add_argument("--pmc-mode", action="store_true",
+                    help="short run for hardware-counter passes: 2 silhouette launches + a 256 MiB copy")
     ap.add_argument("--train-points", type=int, default=1 << 18)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--seed-rel", type=float, default=0.25)
@@ -76,6 +79,40 @@ def cpu_baseline(V, F, q, sample, silhouette_ms_gpu):
     }
 
 
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_silhouette_torus1M.md")
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the silhouette kernel from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by what the
+    counter reads for a copy of known size); None when absent."""
+    if not os.path.exists(PMC_FILE):
+        return None
+    with open(PMC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def pmc_mode(sc, q, n, S, torch):
+    """A few launches for `rocprofv3 --pmc`: the ordered silhouette kernel
+    twice, then a device copy of 256 MiB (reads and writes 2^28 bytes each)
+    that calibrates the byte counters."""
+    order = sc.query_order(q)
+    out_e = torch.empty(n, dtype=torch.int32, device=q.device)
+    out_d = torch.empty(n, dtype=torch.float32, device=q.device)
+    kopts = S.QueryOpts(0, 0, None, order.data_ptr())
+    for _ in range(2):
+        S.check(S.lib().snch_closest_silhouette_ex(sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d),
+                                                   ctypes.byref(kopts), S._stream()))
+    src = torch.ones(1 << 26, dtype=torch.float32, device=q.device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_mode": True, "silhouette_launches": 2, "copy_bytes": src.numel() * 4}))
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -103,6 +140,9 @@ def main():
 
     q = torch.from_numpy(q_np).to(dev)
     n = q.shape[0]
+    if args.pmc_mode:
+        pmc_mode(sc, q, n, S, torch)
+        return
     out_e = torch.empty(n, dtype=torch.int32, device=dev)
     out_d = torch.empty(n, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
@@ -145,10 +185,11 @@ def main():
     perm_buf = torch.empty(n, dtype=torch.int32, device=dev)
     _, order_ms = run_timed(lambda: S.check(S.lib().snch_query_order(sc._h, S._ptr(q), n, S._ptr(perm_buf),
                                                                      S._stream())), args.steps, 1)
-    # input order (no reordering), for comparison
-    iopts = S.QueryOpts(0, 2, None, None)
-    _, input_order_ms = run_timed(lambda: S.check(S.lib().snch_closest_silhouette_ex(
-        sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d), ctypes.byref(iopts), S._stream())), 2, 1)
+    input_order_ms = None
+    if args.input_order:  # the same kernel over the queries in input order, for comparison
+        iopts = S.QueryOpts(0, 2, None, None)
+        _, input_order_ms = run_timed(lambda: S.check(S.lib().snch_closest_silhouette_ex(
+            sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d), ctypes.byref(iopts), S._stream())), 2, 1)
 
     # visit counters of the same launch (one extra launch, counters on)
     stats = torch.zeros(5, dtype=torch.int64, device=dev)
@@ -158,8 +199,8 @@ def main():
     per_q = S_IN + S_OUT + node * S_NODE + cone * S_CONE + leaf * S_LEAF + edge * S_EDGE
     achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_query": per_q,
-                "kernel": "k_closest_silhouette<256,false,3>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(), "bytes_per_query": per_q,
+                "kernel": "k_closest_silhouette<256,false,3,false>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
                 "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
 

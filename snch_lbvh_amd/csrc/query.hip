@@ -299,7 +299,7 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
                     : cone_overlap2d(mk(C.x, C.y, 0.0f), C.w, radius, p, b, md);
 }
 
-template <int BS, bool STATS, int DIM>
+template <int BS, bool STATS, int DIM, bool SEEDED>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
@@ -332,11 +332,11 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
         // visited; otherwise the search reruns without the cap, keeping what
         // it found as the initial bound.  With index tie-breaking the result
         // is bit-identical to the unseeded search.
-        float cap = seed ? seed[qi] : F_INF;
+        float cap = SEEDED ? seed[qi] : F_INF;
         if (!(cap >= 0.0f))
             cap = F_INF;
         LaneStack<BS> st;
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < (SEEDED ? 2 : 1); ++pass) {
             st.init(stk + threadIdx.x);
             const float cap2 = cap * cap;
             uint32_t node = 0;
@@ -724,12 +724,21 @@ template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
                           float* dist, const uint32_t* perm, unsigned long long* stats, hipStream_t st)
 {
-    if (stats)
-        hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
-                           flip, seed, sil, dist, perm, traversal_error_word(), stats);
-    else
-        hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
-                           flip, seed, sil, dist, perm, traversal_error_word(), stats);
+    if (stats) {
+        if (seed)
+            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t,
+                               q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+        else
+            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
+                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+    } else {
+        if (seed)
+            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
+                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+        else
+            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
+                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+    }
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
