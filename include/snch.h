@@ -64,6 +64,10 @@ typedef struct snch_query_opts {
                             visited, cone records read (silhouette query only) */
     const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
                               snch_query_order): the visiting order; overrides coherent */
+    uint32_t stack_depth;      /* batched queries: LDS stack entries per lane, 16, 24 or 32
+                                  (0 = 16) */
+    uint32_t queries_per_wave; /* batched queries: queries a wavefront takes from the work
+                                  queue at a time, a multiple of 64 up to 4096 (0 = 64) */
 } snch_query_opts;
 
 const char *snch_last_error(void);
@@ -84,12 +88,26 @@ int snch_scene_create_tri(snch_context *ctx, const float *V, uint32_t nV, const 
  * V: nV x 2 floats, S: nS x 2 int32 line segments (host). */
 int snch_scene_create_seg(snch_context *ctx, const float *V, uint32_t nV, const int32_t *S, uint32_t nS,
                           snch_scene **out);
+/* Batch of n_meshes independent triangle meshes in one scene (no reference
+ * counterpart: the reference builds one lbvh::scene<3> per mesh,
+ * scene.cuh:1128-1229).  V: vertices of all meshes, concatenated (v_off[m] ..
+ * v_off[m+1] - 1 belong to mesh m); F: triangles of all meshes, concatenated
+ * (f_off[m] .. f_off[m+1] - 1), with vertex indices LOCAL to the mesh.
+ * v_off and f_off hold n_meshes + 1 entries starting at 0 (host).  Each mesh
+ * gets its own edge table, scene box and tree, identical to those of the
+ * mesh built alone; one build and one query launch cover all meshes. */
+int snch_scene_create_tri_batch(snch_context *ctx, uint32_t n_meshes, const float *V, const uint32_t *v_off,
+                                const int32_t *F, const uint32_t *f_off, snch_scene **out);
 int snch_scene_destroy(snch_scene *scene);
+int snch_scene_mesh_count(const snch_scene *scene, uint32_t *n_meshes);
 /* dim (2|3), primitive count, silhouette count (edges in 3D, vertices in 2D) */
 int snch_scene_info(const snch_scene *scene, uint32_t *dim, uint32_t *n_prims, uint32_t *n_silhouettes);
 
 /* scene<3>::build_bvh() -> bvh::construct() (scene.cuh:1205-1229, bvh.cuh:380-613),
- * entirely on the GPU.  Synchronises `stream` before returning. */
+ * entirely on the GPU, for every mesh of the scene at once.  opts->leaf_size
+ * k in 1..8: k = 1 is the reference's tree; k > 1 keeps the same tree and
+ * lets the traversal queries treat any internal node over <= k primitives as
+ * one leaf (results do not change).  Synchronises `stream` before returning. */
 int snch_scene_build(snch_scene *scene, const snch_build_opts *opts, void *stream);
 /* Milliseconds of GPU time of the last build, measured with HIP events. */
 int snch_scene_build_time(const snch_scene *scene, float *ms);
@@ -136,6 +154,34 @@ int snch_closest_silhouette_ex(const snch_scene *scene, const float *q, uint32_t
                                const float *seed_radius, uint32_t *out_sil, float *out_dist,
                                const snch_query_opts *opts, void *stream);
 
+/* ---- batched queries over every mesh of a batch (no reference counterpart;
+ * the reference launches one kernel per mesh and query set) ---------------
+ * q: all query points (DEVICE, n x 3 floats), grouped by mesh: the queries of
+ * mesh m are q_off[m] .. q_off[m+1] - 1 (q_off: HOST, n_meshes + 1 entries,
+ * q_off[0] = 0).  Outputs are per query, in input order; ids are local to the
+ * query's mesh.  One persistent launch: waves take chunks of
+ * opts->queries_per_wave queries of one mesh from a work queue (meshes with
+ * more primitives first).  Results equal the per-mesh single calls. */
+int snch_batch_closest_point(const snch_scene *scene, const float *q, const uint32_t *q_off, uint32_t *out_idx,
+                             float *out_dist, const snch_query_opts *opts, void *stream);
+int snch_batch_closest_silhouette(const snch_scene *scene, const float *q, const uint32_t *q_off, int flip,
+                                  uint32_t *out_sil, float *out_dist, const snch_query_opts *opts, void *stream);
+
+/* Autotuner for the batched queries (kind 0 = closest point, 1 = closest
+ * silhouette): times every leaf size {1, 2, 4, 8} x block size {64, 128, 256,
+ * 512} x stack depth {16, 24, 32} x queries per wave {64, 128, 256} on the
+ * first min(count, sample_per_mesh) queries of each mesh (HIP events), then
+ * rebuilds the scene with the best leaf size.  Pass the result's block_size,
+ * stack_depth and queries_per_wave in snch_query_opts.  Synchronises. */
+typedef struct snch_tune_result {
+    uint32_t leaf_size, block_size, stack_depth, queries_per_wave;
+    float ms;         /* best time of one batched launch over the sample */
+    float ms_default; /* the same for leaf 1, block 256, stack 16, 64 queries per wave */
+    uint32_t n_tried;
+} snch_tune_result;
+int snch_batch_autotune(snch_scene *scene, uint32_t kind, const float *q, const uint32_t *q_off,
+                        uint32_t sample_per_mesh, int flip, snch_tune_result *out, void *stream);
+
 /* ---- inspection (host outputs) ------------------------------------------ */
 /* The tree in the reference's layout (bvh.cuh:27-33, :625-633): nodes are
  * (parent, left, right, object) per node, internal nodes 0..N-2, leaves
@@ -143,9 +189,13 @@ int snch_closest_silhouette_ex(const snch_scene *scene, const float *q, uint32_t
  * in 3D or (axis xy, half_angle, radius) in 2D.  Any output may be NULL. */
 int snch_scene_export(const snch_scene *scene, uint32_t *nodes, float *boxes, float *cones, uint32_t *sorted_codes,
                       uint32_t *sorted_idx);
+/* The same for mesh `mesh` of a batch (ids local to the mesh). */
+int snch_scene_export_mesh(const snch_scene *scene, uint32_t mesh, uint32_t *nodes, float *boxes, float *cones,
+                           uint32_t *sorted_codes, uint32_t *sorted_idx);
 /* Edge (3D) or vertex (2D) silhouette table: slots (opp0, lo, hi, opp1) per
  * edge / (prev, v, next, -1) per vertex, and per-primitive owned ids (3 per
- * triangle, 2 per segment, -1 padded). */
+ * triangle, 2 per segment, -1 padded).  For a batch: the tables of the meshes
+ * concatenated, with mesh-local vertex and edge ids. */
 int snch_scene_silhouettes(const snch_scene *scene, int32_t *slots, int32_t *owned);
 
 /* ---- learned distance bound (Fourier-feature MLP on BF16 MFMA) ----------

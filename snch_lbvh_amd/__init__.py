@@ -53,12 +53,20 @@ class BuildOpts(ctypes.Structure):
 
 class QueryOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p),
-                ("order", ctypes.c_void_p)]
+                ("order", ctypes.c_void_p), ("stack_depth", ctypes.c_uint32), ("queries_per_wave", ctypes.c_uint32)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("leaf_size", ctypes.c_uint32), ("block_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32),
+                ("queries_per_wave", ctypes.c_uint32), ("ms", ctypes.c_float), ("ms_default", ctypes.c_float),
+                ("n_tried", ctypes.c_uint32)]
 
 
 EXPORTED = [
     "snch_last_error", "snch_version", "snch_context_create", "snch_context_destroy", "snch_scene_create_tri",
-    "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_info", "snch_scene_build", "snch_scene_build_time",
+    "snch_scene_create_tri_batch", "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_mesh_count",
+    "snch_scene_info", "snch_scene_build", "snch_scene_build_time", "snch_batch_closest_point",
+    "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh",
     "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
     "snch_closest_silhouette_ex", "snch_ray", "snch_ray_ex",
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
@@ -75,6 +83,12 @@ def _declare(L):
     L.snch_context_destroy.argtypes = [vp]
     L.snch_scene_create_tri.argtypes = [vp, vp, u32, vp, u32, ctypes.POINTER(vp)]
     L.snch_scene_create_seg.argtypes = [vp, vp, u32, vp, u32, ctypes.POINTER(vp)]
+    L.snch_scene_create_tri_batch.argtypes = [vp, u32, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.snch_scene_mesh_count.argtypes = [vp, vp]
+    L.snch_batch_closest_point.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.snch_batch_closest_silhouette.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp]
+    L.snch_batch_autotune.argtypes = [vp, u32, vp, vp, u32, c_int, vp, vp]
+    L.snch_scene_export_mesh.argtypes = [vp, u32, vp, vp, vp, vp, vp]
     L.snch_scene_destroy.argtypes = [vp]
     L.snch_scene_info.argtypes = [vp, vp, vp, vp]
     L.snch_scene_build.argtypes = [vp, vp, vp]
@@ -222,7 +236,7 @@ class Scene:
         n = q.shape[0]
         idx = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(0, coherent, _stats_ptr(stats), _order_ptr(order, n))
+        opts = QueryOpts(0, coherent, _stats_ptr(stats), _order_ptr(order, n), 0, 0)
         check(lib().snch_closest_point_ex(self._h, _ptr(q), n, _ptr(idx), _ptr(dist), ctypes.byref(opts), _stream()))
         return idx, dist
 
@@ -234,7 +248,7 @@ class Scene:
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n))
+        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), 0, 0)
         check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
                                                _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist
@@ -250,7 +264,7 @@ class Scene:
         t = torch.empty(n, dtype=torch.float32, device=o.device)
         uv = torch.empty((n, 2) if self.dim == 3 else (n,), dtype=torch.float32, device=o.device)
         idx = torch.empty(n, dtype=torch.int32, device=o.device)
-        opts = QueryOpts(0, 0, _stats_ptr(stats))
+        opts = QueryOpts(0, 0, _stats_ptr(stats), None, 0, 0)
         check(lib().snch_ray_ex(self._h, _ptr(o), _ptr(d), _ptr(tm), n, int(bool(any_hit)), _ptr(hit), _ptr(t),
                                 _ptr(uv), _ptr(idx), ctypes.byref(opts), _stream()))
         return hit.bool(), t, uv, idx
@@ -302,6 +316,114 @@ class Scene2D(Scene):
     """``lbvh::scene<2>``: line segments with silhouette vertices (scene.cuh:287-703)."""
 
     dim = 2
+
+
+class BatchScene:
+    """Several independent triangle meshes built and queried together: one
+    build and one persistent query launch cover every mesh.  No counterpart
+    in the reference (one lbvh::scene<3> per mesh there)."""
+
+    dim = 3
+
+    def __init__(self, meshes, device=0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("snch_lbvh_amd needs a ROCm GPU; none is visible")
+        self.device = device
+        torch.cuda.set_device(device)
+        Vs = [np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1, 3)) for v, _ in meshes]
+        Fs = [np.ascontiguousarray(np.asarray(f, np.int32).reshape(-1, 3)) for _, f in meshes]
+        self.meshes = list(zip(Vs, Fs))
+        v_off = np.zeros(len(Vs) + 1, np.uint32)
+        f_off = np.zeros(len(Fs) + 1, np.uint32)
+        v_off[1:] = np.cumsum([v.shape[0] for v in Vs])
+        f_off[1:] = np.cumsum([f.shape[0] for f in Fs])
+        V = np.ascontiguousarray(np.concatenate(Vs)) if Vs else np.zeros((0, 3), np.float32)
+        F = np.ascontiguousarray(np.concatenate(Fs)) if Fs else np.zeros((0, 3), np.int32)
+        self.f_off = f_off
+        h = ctypes.c_void_p()
+        check(lib().snch_scene_create_tri_batch(context(device), len(Vs), V.ctypes.data, v_off.ctypes.data,
+                                                F.ctypes.data, f_off.ctypes.data, ctypes.byref(h)))
+        self._h = h
+        self.n_meshes = len(Vs)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.snch_scene_destroy(h)
+            self._h = None
+
+    def build_bvh(self, reference_radius=False, leaf_size=1):
+        opts = BuildOpts(BUILD_REFERENCE_RADIUS if reference_radius else 0, leaf_size)
+        check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
+        return self
+
+    def build_ms(self):
+        ms = ctypes.c_float()
+        check(lib().snch_scene_build_time(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def _qcat(queries):
+        q_off = np.zeros(len(queries) + 1, np.uint32)
+        q_off[1:] = np.cumsum([len(x) for x in queries])
+        return q_off
+
+    def _opts(self, tune, coherent):
+        t = tune or {}
+        return QueryOpts(t.get("block_size", 0), coherent, None, None, t.get("stack_depth", 0),
+                         t.get("queries_per_wave", 0))
+
+    def closest_point(self, q, q_off, tune=None, coherent=0):
+        """q: (n, 3) points of all meshes, grouped by mesh; q_off: n_meshes + 1
+        offsets.  Returns (index local to the mesh, distance) per query."""
+        import torch
+
+        q = _dev_f32(q, 3, self.device)
+        q_off = np.ascontiguousarray(q_off, np.uint32)
+        n = q.shape[0]
+        idx = torch.empty(n, dtype=torch.int32, device=q.device)
+        dist = torch.empty(n, dtype=torch.float32, device=q.device)
+        opts = self._opts(tune, coherent)
+        check(lib().snch_batch_closest_point(self._h, _ptr(q), q_off.ctypes.data, _ptr(idx), _ptr(dist),
+                                             ctypes.byref(opts), _stream()))
+        return idx, dist
+
+    def closest_silhouette(self, q, q_off, flip=False, tune=None, coherent=0):
+        import torch
+
+        q = _dev_f32(q, 3, self.device)
+        q_off = np.ascontiguousarray(q_off, np.uint32)
+        n = q.shape[0]
+        sil = torch.empty(n, dtype=torch.int32, device=q.device)
+        dist = torch.empty(n, dtype=torch.float32, device=q.device)
+        opts = self._opts(tune, coherent)
+        check(lib().snch_batch_closest_silhouette(self._h, _ptr(q), q_off.ctypes.data, int(bool(flip)), _ptr(sil),
+                                                  _ptr(dist), ctypes.byref(opts), _stream()))
+        return sil, dist
+
+    def autotune(self, q, q_off, kind="silhouette", sample_per_mesh=4096, flip=False):
+        """Sweeps leaf size, block size, stack depth and queries per wave;
+        rebuilds with the best leaf size and returns the choice as a dict."""
+        q = _dev_f32(q, 3, self.device)
+        q_off = np.ascontiguousarray(q_off, np.uint32)
+        r = TuneResult()
+        check(lib().snch_batch_autotune(self._h, 1 if kind == "silhouette" else 0, _ptr(q), q_off.ctypes.data,
+                                        sample_per_mesh, int(bool(flip)), ctypes.byref(r), _stream()))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+    def export(self, mesh):
+        V, F = self.meshes[mesh]
+        N = F.shape[0]
+        nodes = np.empty((2 * N - 1, 4), np.uint32)
+        boxes = np.empty((2 * N - 1, 6), np.float32)
+        cones = np.empty((2 * N - 1, 5), np.float32)
+        codes = np.empty(N, np.uint32)
+        sidx = np.empty(N, np.uint32)
+        check(lib().snch_scene_export_mesh(self._h, mesh, nodes.ctypes.data, boxes.ctypes.data, cones.ctypes.data,
+                                           codes.ctypes.data, sidx.ctypes.data))
+        return {"nodes": nodes, "boxes": boxes, "cones": cones, "sorted_codes": codes, "sorted_idx": sidx}
 
 
 class LearnedBound:

@@ -15,12 +15,16 @@
 //                     duplicate codes it is the reference's 64-bit fallback
 //                     (bvh.cuh:459-476), so no uniqueness pass or host
 //                     round trip is needed.
-//   k_refit           one thread per leaf climbs with an agent-scope
-//                     acquire-release counter per internal node; the second
-//                     arrival merges the children's AABB and then their cone
-//                     (the parent box centroid feeds the cone merge, as in
-//                     bvh.cuh:598) and writes both the reference-layout node
-//                     data and the packed traversal record.
+//   k_refit           bottom-up AABB + cone merge over subtrees of at most
+//                     1024 leaves in LDS, then k_refit_top above them (the
+//                     parent box centroid feeds the cone merge, as in
+//                     bvh.cuh:598); writes the reference-layout node data and
+//                     the packed traversal records.
+//
+// A scene may hold several meshes: every kernel runs over all of them at
+// once.  Each mesh gets its own scene box, Morton codes in that box, and its
+// own tree (the sort orders by (mesh, code, index)); a mesh's tree is
+// bit-identical to the tree of the same mesh built alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,13 +50,99 @@ __device__ __forceinline__ v3 ld3(const float4* V, int i)
     return mk(a.x, a.y, a.z);
 }
 
-__global__ void k_init_scene_box(int* bits)
+// mesh of primitive i (pmesh is null for a single mesh)
+__device__ __forceinline__ uint32_t mesh_of_prim(const uint32_t* __restrict__ pmesh, uint32_t i)
 {
-    int t = threadIdx.x;
-    if (t < 3)
-        bits[t] = f2ord(F_INF);
-    else if (t < 6)
-        bits[t] = f2ord(-F_INF);
+    return pmesh ? pmesh[i] : 0u;
+}
+
+// last mesh whose key (node base or internal-record base) is <= x
+template <bool BY_NODE>
+__device__ __forceinline__ uint32_t mesh_search(const MeshDesc* __restrict__ d, uint32_t M, uint32_t x)
+{
+    uint32_t lo = 0, hi = M; // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t key = BY_NODE ? d[mid].nb : d[mid].off - mid;
+        if (key <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// scene-box accumulators (+inf, -inf) of every mesh, 8 ints per mesh
+__global__ void k_init_scene_box(int* bits, uint32_t M)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * M)
+        return;
+    const uint32_t k = t & 7u;
+    bits[t] = k < 3 ? f2ord(F_INF) : f2ord(-F_INF);
+}
+
+// block reduction of a scene box into the mesh accumulators: wave butterfly,
+// then one atomic per component per block when the whole block holds one
+// mesh (always for a single mesh), else per wave or per lane
+template <int NV>
+__device__ __forceinline__ void reduce_scene_box(float (&v)[NV], uint32_t mesh, bool valid, int* scene_bits,
+                                                 const int* slot_of)
+{
+    __shared__ float red[BT / 64][NV];
+    __shared__ uint32_t redm[BT / 64];
+    __shared__ int redu[BT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t m0 = __shfl(mesh, 0, 64), m63 = __shfl(mesh, 63, 64);
+    const bool uni = m0 == m63;
+    if (uni) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                v[k] = k < NV / 2 ? fmn(v[k], __shfl_xor(v[k], o, 64)) : fmx(v[k], __shfl_xor(v[k], o, 64));
+        }
+    } else if (valid) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (k < NV / 2)
+                atomicMin(scene_bits + 8 * mesh + slot_of[k], f2ord(v[k]));
+            else
+                atomicMax(scene_bits + 8 * mesh + slot_of[k], f2ord(v[k]));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            red[w][k] = v[k];
+        redm[w] = m0;
+        redu[w] = uni ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        bool same = true;
+        for (int q = 0; q < BT / 64; ++q)
+            same = same && redu[q] && redm[q] == redm[0];
+        if (same) {
+            float acc = red[0][k];
+            for (int q = 1; q < BT / 64; ++q)
+                acc = k < NV / 2 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
+            if (k < NV / 2)
+                atomicMin(scene_bits + 8 * redm[0] + slot_of[k], f2ord(acc));
+            else
+                atomicMax(scene_bits + 8 * redm[0] + slot_of[k], f2ord(acc));
+        } else {
+            for (int q = 0; q < BT / 64; ++q) {
+                if (!redu[q])
+                    continue;
+                if (k < NV / 2)
+                    atomicMin(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+                else
+                    atomicMax(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+            }
+        }
+    }
 }
 
 // normal of face f of a silhouette edge (scene.cuh:747-772); slots (opp0, lo, hi, opp1)
@@ -75,9 +165,12 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
                                                 const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
                                                 uint32_t nF, int conservative, float4* __restrict__ leafbox,
                                                 float4* __restrict__ leafcone, float4* __restrict__ tedge,
-                                                int* __restrict__ scene_bits)
+                                                int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh,
+                                                const MeshDesc* __restrict__ desc)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    const uint32_t mesh = mesh_of_prim(pmesh, i < nF ? i : nF - 1);
+    const uint32_t eoff = desc[mesh].eoff;
     box3 b;
     b.lo = mk(F_INF, F_INF, F_INF);
     b.hi = mk(-F_INF, -F_INF, -F_INF);
@@ -128,7 +221,7 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
             nn[k][0] = has0 ? normalize3(raw0) : mk(0.0f, 0.0f, 0.0f);
             nn[k][1] = has1 ? normalize3(raw1) : mk(0.0f, 0.0f, 0.0f);
             // packed silhouette-edge record (each edge has exactly one owner)
-            tedge[4 * e + 0] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(e));
+            tedge[4 * e + 0] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(e - eoff));
             tedge[4 * e + 1] =
                 make_float4(pb.x, pb.y, pb.z, __uint_as_float((has0 ? 1u : 0u) | (has1 ? 2u : 0u)));
             tedge[4 * e + 2] = make_float4(nn[k][0].x, nn[k][0].y, nn[k][0].z, 0.0f);
@@ -161,35 +254,10 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
         leafcone[2 * i] = make_float4(r.axis.x, r.axis.y, r.axis.z, r.half_angle);
         leafcone[2 * i + 1] = make_float4(r.radius, 0.0f, 0.0f, 0.0f);
     }
-    // scene AABB: wave64 butterfly, then LDS across the block's 4 waves
+    // scene AABB of the primitive's mesh
     float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            v[k] = fmn(v[k], __shfl_xor(v[k], o, 64));
-#pragma unroll
-        for (int k = 3; k < 6; ++k)
-            v[k] = fmx(v[k], __shfl_xor(v[k], o, 64));
-    }
-    __shared__ float red[BT / 64][6];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            red[w][k] = v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const int k = threadIdx.x;
-        float acc = red[0][k];
-        for (int q = 1; q < BT / 64; ++q)
-            acc = k < 3 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
-        if (k < 3)
-            atomicMin(scene_bits + k, f2ord(acc));
-        else
-            atomicMax(scene_bits + k, f2ord(acc));
-    }
+    const int slot_of[6] = {0, 1, 2, 3, 4, 5};
+    reduce_scene_box<6>(v, mesh, i < nF, scene_bits, slot_of);
 }
 
 // scene<2>: leaf box (scene.cuh:420-433) and leaf cone from the owned
@@ -199,9 +267,10 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
                                                   const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
                                                   uint32_t nS, float4* __restrict__ leafbox,
                                                   float4* __restrict__ leafcone, float4* __restrict__ tedge,
-                                                  int* __restrict__ scene_bits)
+                                                  int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    const uint32_t mesh = mesh_of_prim(pmesh, i < nS ? i : nS - 1);
     box3 b;
     b.lo = mk(F_INF, F_INF, 0.0f);
     b.hi = mk(-F_INF, -F_INF, 0.0f);
@@ -278,41 +347,19 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
         leafcone[2 * i + 1] = make_float4(rad, 0.0f, 0.0f, 0.0f);
     }
     float v[4] = {b.lo.x, b.lo.y, b.hi.x, b.hi.y};
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        v[0] = fmn(v[0], __shfl_xor(v[0], o, 64));
-        v[1] = fmn(v[1], __shfl_xor(v[1], o, 64));
-        v[2] = fmx(v[2], __shfl_xor(v[2], o, 64));
-        v[3] = fmx(v[3], __shfl_xor(v[3], o, 64));
-    }
-    __shared__ float red[BT / 64][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            red[w][k] = v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const int k = threadIdx.x;
-        float acc = red[0][k];
-        for (int q = 1; q < BT / 64; ++q)
-            acc = k < 2 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
-        if (k < 2)
-            atomicMin(scene_bits + k, f2ord(acc));
-        else
-            atomicMax(scene_bits + 3 + (k - 2), f2ord(acc));
-    }
+    const int slot_of[4] = {0, 1, 3, 4};
+    reduce_scene_box<4>(v, mesh, i < nS, scene_bits, slot_of);
 }
 
 // bvh.cuh:269-277 and morton_code.cuh:85-92
-__global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafbox,
-                                                 const int* __restrict__ scene_bits, uint32_t n,
-                                                 uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
+__global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
+                                                 uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
+                                                 const uint32_t* __restrict__ pmesh)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     if (i >= n)
         return;
+    const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
     const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]);
     const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]);
     const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
@@ -326,12 +373,14 @@ __global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leaf
 }
 
 // bvh.cuh:292-302 and morton_code.cuh:61-70
-__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ scene_bits,
-                                               uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
+__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
+                                               uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
+                                               const uint32_t* __restrict__ pmesh)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     if (i >= n)
         return;
+    const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
     const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]), loz = ord2f(scene_bits[2]);
     const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]), hiz = ord2f(scene_bits[5]);
     const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
@@ -349,17 +398,17 @@ __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbo
     idx[i] = i;
 }
 
-// delta of the virtual 64-bit key (code << 32 | idx)
+// delta of the virtual 64-bit key (code << 32 | local index); codes and idx
+// point at the mesh's first sorted position, idx holds global primitive ids
 __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
-                                      uint32_t a, uint32_t b)
+                                      uint32_t off, uint32_t a, uint32_t b)
 {
     const uint32_t ca = codes[a], cb = codes[b];
     if (ca != cb)
         return __clz(ca ^ cb);
-    return 32 + __clz(idx[a] ^ idx[b]);
+    return 32 + __clz((idx[a] - off) ^ (idx[b] - off));
 }
 
-// bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
 constexpr uint32_t RT = 1024; // leaves of one refit subtree (threads of one refit workgroup)
 
 __device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_t node)
@@ -369,38 +418,50 @@ __device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_
         cuts[k] = node;
 }
 
-__global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
-                                               uint32_t N, uint32_t* __restrict__ parent, uint2* __restrict__ child,
+// bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes).
+// One thread per internal node of every mesh (internal-record slot t);
+// nodes are written with global (mesh-major) ids.
+__global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes_all,
+                                               const uint32_t* __restrict__ idx_all, uint32_t n_internal,
+                                               const MeshDesc* __restrict__ desc, uint32_t M,
+                                               uint32_t* __restrict__ parent, uint2* __restrict__ child,
                                                uint2* __restrict__ range, uint32_t* __restrict__ flags,
                                                uint32_t* __restrict__ cuts, uint32_t cut_cap)
 {
-    const uint32_t i = blockIdx.x * BT + threadIdx.x;
-    if (i >= N - 1)
+    const uint32_t t = blockIdx.x * BT + threadIdx.x;
+    if (t >= n_internal)
         return;
+    const uint32_t m = mesh_search<false>(desc, M, t);
+    const MeshDesc D = desc[m];
+    const uint32_t i = t - (D.off - m), N = D.n, off = D.off, nb = D.nb;
+    if (N < 2 || i >= N - 1)
+        return; // not reached for consistent offsets
+    const uint32_t* codes = codes_all + off;
+    const uint32_t* idx = idx_all + off;
     uint32_t first, last;
     if (i == 0) {
         first = 0;
         last = N - 1;
-        parent[0] = INVALID;
+        parent[nb] = INVALID;
     } else {
-        const int L = kdelta(codes, idx, i, i - 1);
-        const int R = kdelta(codes, idx, i, i + 1);
+        const int L = kdelta(codes, idx, off, i, i - 1);
+        const int R = kdelta(codes, idx, off, i, i + 1);
         const int d = (R > L) ? 1 : -1;
         const int dmin = L < R ? L : R;
         int lmax = 2;
         int itmp = (int)i + d * lmax;
-        int delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+        int delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
         while (delta > dmin) {
             lmax <<= 1;
             itmp = (int)i + d * lmax;
-            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
         }
         int l = 0;
-        for (int t = lmax >> 1; t > 0; t >>= 1) {
-            itmp = (int)i + (l + t) * d;
-            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, i, (uint32_t)itmp) : -1;
+        for (int s = lmax >> 1; s > 0; s >>= 1) {
+            itmp = (int)i + (l + s) * d;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
             if (delta > dmin)
-                l += t;
+                l += s;
         }
         const uint32_t j = (uint32_t)((int)i + l * d);
         first = d < 0 ? j : i;
@@ -409,28 +470,26 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     // find_split; first != last always holds for internal nodes and the
     // virtual 64-bit keys are unique, so the equal-key branch of the
     // reference (bvh.cuh:175-178) cannot trigger
-    const int dnode = kdelta(codes, idx, first, last);
+    const int dnode = kdelta(codes, idx, off, first, last);
     int split = (int)first;
     int stride = (int)(last - first);
     do {
         stride = (stride + 1) >> 1;
         const int mid = split + stride;
         if (mid < (int)last) {
-            if (kdelta(codes, idx, first, (uint32_t)mid) > dnode)
+            if (kdelta(codes, idx, off, first, (uint32_t)mid) > dnode)
                 split = mid;
         }
     } while (stride > 1);
     const uint32_t g = (uint32_t)split;
-    uint32_t left = g, right = g + 1;
-    if (first == g)
-        left += N - 1;
-    if (last == g + 1)
-        right += N - 1;
-    child[i] = make_uint2(left, right);
-    range[i] = make_uint2(first, last);
-    parent[left] = i;
-    parent[right] = i;
-    flags[i] = 0u;
+    const uint32_t left = nb + (first == g ? N - 1 + g : g);
+    const uint32_t right = nb + (last == g + 1 ? N - 1 + g + 1 : g + 1);
+    const uint32_t self = nb + i;
+    child[self] = make_uint2(left, right);
+    range[self] = make_uint2(off + first, off + last);
+    parent[left] = self;
+    parent[right] = self;
+    flags[self] = 0u;
     // refit subtrees ("cuts"): maximal subtrees of at most RT leaves
     if (last - first + 1 > RT) {
         if (g - first + 1 <= RT)
@@ -438,8 +497,19 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
         if (last - g <= RT)
             append_cut(cuts, cut_cap, right);
     } else if (i == 0) {
-        append_cut(cuts, cut_cap, 0u);
+        append_cut(cuts, cut_cap, self);
     }
+}
+
+// meshes of one primitive: the leaf is the root and its own cut
+__global__ void k_mesh_roots(const MeshDesc* __restrict__ desc, uint32_t M, uint32_t* __restrict__ parent,
+                             uint32_t* __restrict__ cuts, uint32_t cut_cap)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M || desc[m].n != 1)
+        return;
+    parent[desc[m].nb] = INVALID;
+    append_cut(cuts, cut_cap, desc[m].nb);
 }
 
 __device__ __forceinline__ box3 ldbox(const float4* nb, uint32_t n)
@@ -463,21 +533,49 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
 // Bottom-up refit in two launches, with no device-wide fence.
 //
 // k_karras lists the maximal subtrees of at most RT leaves ("cuts"); they are
-// disjoint and cover every leaf.  k_refit gives each cut to one workgroup:
-// thread t initialises leaf first + t, then climbs; the arrival counter of an
-// internal node lives in LDS at (Karras index - first), and the box and cone
-// of every node of the cut are kept in LDS too, so the second arrival merges
-// its children without a trip to memory.  Workgroup-scope fences order it.
-// k_refit_top: one workgroup climbs from the cut roots through the nodes
-// above them (about 2N/RT of them) with arrival counters in global memory;
-// all of them are touched by this one workgroup, so workgroup scope orders
-// them too, and the launch boundary publishes the first launch's results.
+// disjoint and cover every leaf of every mesh.  k_refit gives each cut to one
+// workgroup: thread t initialises leaf first + t, then climbs; the arrival
+// counter of an internal node lives in LDS at (Karras position - first), and
+// the box and cone of every node of the cut are kept in LDS too, so the
+// second arrival merges its children without a trip to memory.
+// Workgroup-scope fences order it.  k_refit_top: one workgroup climbs from
+// the cut roots through the nodes above them (about 2N/RT of them) with
+// arrival counters in global memory; all of them are touched by this one
+// workgroup, so workgroup scope orders them too, and the launch boundary
+// publishes the first launch's results.
+
+// ids of one mesh: global node id <-> local id / sorted position
+struct MeshIds {
+    uint32_t off, n, nb;
+    __device__ __forceinline__ bool internal(uint32_t node) const { return node - nb < n - 1; }
+    // sorted position of a leaf node, or Karras position of an internal node
+    __device__ __forceinline__ uint32_t pos(uint32_t node) const
+    {
+        return internal(node) ? off + (node - nb) : off + (node - nb - (n - 1));
+    }
+    __device__ __forceinline__ uint32_t ref(uint32_t node) const
+    {
+        return internal(node) ? node - nb : LEAF_BIT | (node - nb - (n - 1));
+    }
+};
+
+// leaf run of an internal child covering at most leaf_size leaves
+__device__ __forceinline__ uint32_t run_of(const MeshIds& id, uint32_t node, const uint2* __restrict__ range,
+                                           uint32_t leaf_size)
+{
+    if (leaf_size <= 1 || !id.internal(node))
+        return INVALID;
+    const uint2 rg = range[node];
+    const uint32_t cnt = rg.y - rg.x + 1;
+    return cnt <= leaf_size ? ((cnt - 1) << RUN_SHIFT) | (rg.x - id.off) : INVALID;
+}
 
 // the parent box and cone of two children (bvh.cuh:520-604), and the records
 // of internal node p
 template <int DIM>
-__device__ __forceinline__ void refit_write(uint32_t p, uint32_t N, uint2 ch, const box3& lb, const box3& rb,
-                                            const cone3& lc, const cone3& rc, int conservative, box3& pb,
+__device__ __forceinline__ void refit_write(const MeshIds& id, uint32_t p, uint2 ch, const box3& lb, const box3& rb,
+                                            const cone3& lc, const cone3& rc, int conservative,
+                                            const uint2* __restrict__ range, uint32_t leaf_size, box3& pb,
                                             cone3& pc, float4* nbox, float4* ncone, float4* __restrict__ tnode,
                                             float4* __restrict__ tcone)
 {
@@ -488,19 +586,20 @@ __device__ __forceinline__ void refit_write(uint32_t p, uint32_t N, uint2 ch, co
     nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
     ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
     ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
-    const uint32_t lref = ch.x < N - 1 ? ch.x : (LEAF_BIT | (ch.x - (N - 1)));
-    const uint32_t rref = ch.y < N - 1 ? ch.y : (LEAF_BIT | (ch.y - (N - 1)));
-    tnode[4 * p + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
-    tnode[4 * p + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
-    tnode[4 * p + 2] = make_float4(rb.lo.z, rb.hi.x, rb.hi.y, rb.hi.z);
-    tnode[4 * p + 3] = make_float4(__uint_as_float(lref), __uint_as_float(rref), 0.0f, 0.0f);
-    tcone[3 * p + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
-    tcone[3 * p + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
-    tcone[3 * p + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+    const uint32_t r = p - id.off; // internal record: (off - m) + (p - nb)
+    tnode[4 * r + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
+    tnode[4 * r + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
+    tnode[4 * r + 2] = make_float4(rb.lo.z, rb.hi.x, rb.hi.y, rb.hi.z);
+    tnode[4 * r + 3] = make_float4(__uint_as_float(id.ref(ch.x)), __uint_as_float(id.ref(ch.y)),
+                                   __uint_as_float(run_of(id, ch.x, range, leaf_size)),
+                                   __uint_as_float(run_of(id, ch.y, range, leaf_size)));
+    tcone[3 * r + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
+    tcone[3 * r + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
+    tcone[3 * r + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
 }
 
 // box + cone of one node in LDS, structure of arrays over 2 * RT slots (slot
-// t: leaf first + t of the cut; slot RT + k: internal node of Karras index
+// t: leaf first + t of the cut; slot RT + k: internal node of Karras position
 // first + k)
 struct NodeLds {
     float v[11][2 * RT];
@@ -527,7 +626,8 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                                               const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
                                               const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
                                               const uint2* __restrict__ child, const uint2* __restrict__ range,
-                                              uint32_t N, int conservative, float4* nbox, float4* ncone,
+                                              const MeshDesc* __restrict__ desc, uint32_t M, uint32_t N_all,
+                                              int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
                                               float4* __restrict__ tnode, float4* __restrict__ tcone,
                                               float4* __restrict__ tleaf, const uint32_t* __restrict__ cuts,
                                               uint32_t cut_cap)
@@ -538,21 +638,24 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
     const uint32_t count = min(cuts[cut_cap], cut_cap);
     for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
         const uint32_t root = cuts[c];
+        const uint32_t m = mesh_search<true>(desc, M, root);
+        const MeshDesc D = desc[m];
+        const MeshIds id{D.off, D.n, D.nb};
         uint32_t first, last;
-        if (root >= N - 1) {
-            first = last = root - (N - 1);
-        } else {
+        if (id.internal(root)) {
             const uint2 rg = range[root];
             first = rg.x;
             last = rg.y;
+        } else {
+            first = last = id.pos(root);
         }
         lflag[t] = 0u;
         __syncthreads();
         const uint32_t j = first + t;
-        if (last - first < RT && j <= last && last < N) {
+        if (last - first < RT && j <= last && last < N_all) {
             const uint32_t orig = sidx[j];
-            if (orig < N) { // a permutation always; otherwise the node stays unwritten
-                uint32_t node = N - 1 + j;
+            if (orig - D.off < D.n) { // a permutation of the mesh's primitives; else the node stays unwritten
+                uint32_t node = D.nb + (D.n - 1) + (j - D.off);
                 const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
                 const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
                 nbox[2 * node] = b0;
@@ -570,36 +673,39 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                     nd.put(t, b, cc);
                 }
                 const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
+                const uint32_t lo = orig - D.off, les = es - D.eoff;
                 if (DIM == 3) {
                     const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], d = V[F[3 * orig + 2]];
-                    tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(orig));
-                    tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(es));
+                    tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(lo));
+                    tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(les));
                     tleaf[3 * j + 2] = make_float4(d.x, d.y, d.z, __uint_as_float(ec));
                 } else {
                     const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
-                    tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(orig));
-                    tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(es));
+                    tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(lo));
+                    tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(les));
                     tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
                 }
                 // climb inside the cut (at most its depth, < RT levels)
                 for (uint32_t guard = 0; node != root && guard < RT; ++guard) {
                     const uint32_t p = parent[node];
-                    if (p >= N - 1 || p < first || p - first >= RT)
+                    if (!id.internal(p) || id.pos(p) < first || id.pos(p) - first >= RT)
                         break; // malformed tree: never taken for a Karras tree
+                    const uint32_t k = id.pos(p) - first;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    const uint32_t old = atomicAdd(&lflag[p - first], 1u);
+                    const uint32_t old = atomicAdd(&lflag[k], 1u);
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     if (old == 0u)
                         break; // the sibling subtree is not finished; its last thread continues
                     const uint2 ch = child[p];
-                    const uint32_t sl = ch.x < N - 1 ? RT + (ch.x - first) : ch.x - (N - 1) - first;
-                    const uint32_t sr = ch.y < N - 1 ? RT + (ch.y - first) : ch.y - (N - 1) - first;
+                    const uint32_t sl = (id.internal(ch.x) ? RT : 0u) + (id.pos(ch.x) - first);
+                    const uint32_t sr = (id.internal(ch.y) ? RT : 0u) + (id.pos(ch.y) - first);
                     box3 lb, rb, pb;
                     cone3 lc, rc, pc;
                     nd.get(sl, lb, lc);
                     nd.get(sr, rb, rc);
-                    refit_write<DIM>(p, N, ch, lb, rb, lc, rc, conservative, pb, pc, nbox, ncone, tnode, tcone);
-                    nd.put(RT + (p - first), pb, pc);
+                    refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
+                                     tnode, tcone);
+                    nd.put(RT + k, pb, pc);
                     node = p;
                 }
             }
@@ -610,38 +716,37 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
 
 template <int DIM>
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ parent,
-                                                    const uint2* __restrict__ child, uint32_t* flags, uint32_t N,
-                                                    int conservative, float4* nbox, float4* ncone,
-                                                    float4* __restrict__ tnode, float4* __restrict__ tcone,
-                                                    const uint32_t* __restrict__ cuts, uint32_t cut_cap)
+                                                    const uint2* __restrict__ child, const uint2* __restrict__ range,
+                                                    uint32_t* flags, const MeshDesc* __restrict__ desc, uint32_t M,
+                                                    uint32_t n_nodes, int conservative, uint32_t leaf_size,
+                                                    float4* nbox, float4* ncone, float4* __restrict__ tnode,
+                                                    float4* __restrict__ tcone, const uint32_t* __restrict__ cuts,
+                                                    uint32_t cut_cap)
 {
     const uint32_t count = min(cuts[cut_cap], cut_cap);
     for (uint32_t k = threadIdx.x; k < count; k += 1024) {
         uint32_t node = cuts[k];
-        if (node >= 2 * N - 1)
+        if (node >= n_nodes)
             continue;
+        const uint32_t m = mesh_search<true>(desc, M, node);
+        const MeshDesc D = desc[m];
+        const MeshIds id{D.off, D.n, D.nb};
         uint32_t p = parent[node];
-        for (int guard = 0; p < N - 1 && guard < 96; ++guard) {
+        for (int guard = 0; p != INVALID && id.internal(p) && guard < 96; ++guard) {
             const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u)
                 break;
             const uint2 ch = child[p];
             box3 pb;
             cone3 pc;
-            refit_write<DIM>(p, N, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
-                             ldcone(ncone, ch.y), conservative, pb, pc, nbox, ncone, tnode, tcone);
+            refit_write<DIM>(id, p, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
+                             ldcone(ncone, ch.y), conservative, range, leaf_size, pb, pc, nbox, ncone, tnode, tcone);
             node = p;
             p = parent[p];
         }
     }
 }
 
-__global__ void k_single_root(uint32_t* parent, uint32_t* cuts, uint32_t cut_cap)
-{
-    parent[0] = INVALID;
-    cuts[0] = 0u; // the single leaf is node 0 and its own cut
-    cuts[cut_cap] = 1u;
-}
 __global__ void k_zero_word(uint32_t* w) { *w = 0u; }
 
 inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
@@ -659,10 +764,13 @@ inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
 
 } // namespace
 
-hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
+hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st)
 {
-    const uint32_t N = s->nF, NN = 2 * N - 1, E = s->dim == 3 ? s->nE : s->n_internal_sil;
+    const uint32_t N = s->nF, M = s->n_meshes, NN = 2 * N - M, NI = N - M;
+    const uint32_t E = s->dim == 3 ? s->nE : s->n_internal_sil;
     const int conservative = (flags & SNCH_BUILD_REFERENCE_RADIUS) ? 0 : 1;
+    const MeshDesc* desc = s->dMesh.as<MeshDesc>();
+    const uint32_t* pmesh = M > 1 ? s->dPmesh.as<uint32_t>() : nullptr;
     BuildScratch& w = s->scratch;
     hipError_t e;
 #define SNCH_TRY(x)              \
@@ -672,64 +780,73 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st)
     } while (0)
     SNCH_TRY(w.leafbox.ensure(sizeof(float4) * 2 * N));
     SNCH_TRY(w.leafcone.ensure(sizeof(float4) * 2 * N));
-    SNCH_TRY(w.scene_bits.ensure(sizeof(int) * 8));
+    SNCH_TRY(w.scene_bits.ensure(sizeof(int) * 8 * M));
     SNCH_TRY(w.codes.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.idx.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.codes_alt.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.idx_alt.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.parent.ensure(sizeof(uint32_t) * NN));
-    SNCH_TRY(w.child.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
-    SNCH_TRY(w.range.ensure(sizeof(uint2) * (N > 1 ? N - 1 : 1)));
-    SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * (N > 1 ? N - 1 : 1)));
-    const uint32_t cut_cap = N;
+    SNCH_TRY(w.child.ensure(sizeof(uint2) * NN));
+    SNCH_TRY(w.range.ensure(sizeof(uint2) * NN));
+    SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * NN));
+    const uint32_t cut_cap = N; // cuts are disjoint subtrees of at least one leaf
     SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 1)));
     uint32_t* cuts = w.top.as<uint32_t>();
-    SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (N > 1 ? N - 1 : 1)));
-    SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (N > 1 ? N - 1 : 1)));
+    SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
+    SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
     SNCH_TRY(s->tedge.ensure(sizeof(float4) * 4 * (E > 0 ? E : 1)));
     SNCH_TRY(s->nbox.ensure(sizeof(float4) * 2 * NN));
     SNCH_TRY(s->ncone.ensure(sizeof(float4) * 2 * NN));
 
-    hipLaunchKernelGGL(k_init_scene_box, dim3(1), dim3(64), 0, st, w.scene_bits.as<int>());
+    hipLaunchKernelGGL(k_init_scene_box, dim3((8 * M + 255) / 256), dim3(256), 0, st, w.scene_bits.as<int>(), M);
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc);
         hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
-                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh);
     } else {
         hipLaunchKernelGGL(k_leafgen2d, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>());
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh);
         hipLaunchKernelGGL(k_morton2d, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
-                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh);
     }
+    // stable sort by (mesh, code, index): code passes, then mesh-id passes
+    int mesh_bits = 0;
+    while (mesh_bits < 32 && ((M - 1) >> mesh_bits) != 0)
+        ++mesh_bits;
     bool in_alt = false;
     SNCH_TRY(radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
-                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt));
+                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt, pmesh, pmesh ? mesh_bits : 0));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, st, cuts + cut_cap);
-    if (N > 1)
-        hipLaunchKernelGGL(k_karras, dim3(blocks(N - 1)), dim3(BT), 0, st, codes, sidx, N, w.parent.as<uint32_t>(),
-                           w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), cuts, cut_cap);
-    else
-        hipLaunchKernelGGL(k_single_root, dim3(1), dim3(1), 0, st, w.parent.as<uint32_t>(), cuts, cut_cap);
-    // cuts hold at least N / RT leaves each but one per side of a big node,
-    // so about 2N / RT of them; each workgroup loops over its share
-    const uint32_t refit_blocks = std::max(1u, std::min(2u * (N + RT - 1) / RT + 1u,
-                                                        (uint32_t)std::max(1, s->ctx ? 2 * s->ctx->n_cu : 512)));
+    if (NI > 0)
+        hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, desc, M,
+                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(),
+                           cuts, cut_cap);
+    hipLaunchKernelGGL(k_mesh_roots, dim3((M + 255) / 256), dim3(256), 0, st, desc, M, w.parent.as<uint32_t>(), cuts,
+                       cut_cap);
+    // about 2N / RT cuts (each holds at least RT / 2 leaves but for one per
+    // side of a large node, plus one per small mesh); workgroups loop over them
+    const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
+    const uint32_t refit_blocks = std::max(1u, std::min(2u * ((N + RT - 1) / RT) + M, 2u * n_cu));
+    bool any_big = false;
+    for (uint32_t m = 0; m < M && !any_big; ++m)
+        any_big = s->mesh_off[m + 1] - s->mesh_off[m] > RT;
     SNCH_DISPATCH_REFIT(s->dim, {
         hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RT), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
-                           N, conservative, s->nbox.as<float4>(), s->ncone.as<float4>(), s->tnode.as<float4>(),
-                           s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
-        if (N > RT)
+                           desc, M, N, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
+                           s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
+        if (any_big)
             hipLaunchKernelGGL(k_refit_top<DIM>, dim3(1), dim3(1024), 0, st, w.parent.as<uint32_t>(),
-                               w.child.as<uint2>(), w.flags.as<uint32_t>(), N, conservative, s->nbox.as<float4>(),
-                               s->ncone.as<float4>(), s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
+                               w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), desc, M, NN,
+                               conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
+                               s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
     });
     s->sorted_in_alt = in_alt;
 #undef SNCH_TRY

@@ -71,13 +71,18 @@ uint32_t* traversal_error_word()
 // opp1); each edge owned by the first triangle listing it.  Computed by
 // sorting (lo, hi, position) instead of a std::map; the owned edges of a
 // triangle are then a run of consecutive ids starting at estart[i].
-static void edge_table_3d(snch_scene* s)
+struct EdgeTable {
+    std::vector<int32_t> slots;   // 4 per edge
+    std::vector<int32_t> owned;   // 3 per triangle
+    std::vector<uint32_t> estart; // nF + 1
+    uint32_t nE = 0;
+};
+
+static void edge_table_3d(const int32_t* F, uint32_t nF, EdgeTable* s)
 {
-    const uint32_t nF = s->nF;
     const size_t n = 3 * (size_t)nF;
     std::vector<uint64_t> key(n);
     std::vector<uint32_t> order(n);
-    const int32_t* F = s->F.data();
     for (size_t p = 0; p < n; ++p) {
         const size_t i = p / 3, j = p % 3;
         uint32_t I = (uint32_t)F[3 * i + j], J = (uint32_t)F[3 * i + (j + 1) % 3];
@@ -137,20 +142,65 @@ static void edge_table_3d(snch_scene* s)
     }
 }
 
+// device inputs of a scene: vertices as float4, primitives / silhouette
+// slots with scene-global vertex indices, first owned silhouette per
+// primitive (global), the mesh table and, for batches, the mesh of each
+// primitive
+static hipError_t upload_scene(snch_scene* s, int dim, const std::vector<float>& Vflat,
+                               const std::vector<int32_t>& Fg, const std::vector<int32_t>& slots_g,
+                               const std::vector<uint32_t>& estart_g)
+{
+    const size_t nV = Vflat.size() / dim;
+    std::vector<float> V4(4 * nV, 0.0f);
+    for (size_t k = 0; k < nV; ++k)
+        for (int c = 0; c < dim; ++c)
+            V4[4 * k + c] = Vflat[dim * k + c];
+    const uint32_t M = s->n_meshes;
+    std::vector<MeshDesc> desc(M);
+    for (uint32_t m = 0; m < M; ++m)
+        desc[m] = MeshDesc{s->mesh_off[m], s->mesh_off[m + 1] - s->mesh_off[m], s->mesh_eoff[m],
+                           2 * s->mesh_off[m] - m};
+    hipError_t e;
+    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dF.ensure(sizeof(int32_t) * Fg.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dF.p, Fg.data(), sizeof(int32_t) * Fg.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(slots_g.size(), 4))) != hipSuccess ||
+        (slots_g.size() && (e = hipMemcpy(s->dSlots.p, slots_g.data(), sizeof(int32_t) * slots_g.size(),
+                                          hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = s->dEstart.ensure(sizeof(uint32_t) * estart_g.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dEstart.p, estart_g.data(), sizeof(uint32_t) * estart_g.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = s->dMesh.ensure(sizeof(MeshDesc) * M)) != hipSuccess ||
+        (e = hipMemcpy(s->dMesh.p, desc.data(), sizeof(MeshDesc) * M, hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    if (M > 1) {
+        std::vector<uint32_t> pm(s->nF);
+        for (uint32_t m = 0; m < M; ++m)
+            std::fill(pm.begin() + s->mesh_off[m], pm.begin() + s->mesh_off[m + 1], m);
+        if ((e = s->dPmesh.ensure(sizeof(uint32_t) * pm.size())) != hipSuccess ||
+            (e = hipMemcpy(s->dPmesh.p, pm.data(), sizeof(uint32_t) * pm.size(), hipMemcpyHostToDevice)) !=
+                hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 } // namespace snch
 
 using namespace snch;
 
-snch::TreeView snch_scene::view() const
+snch::TreeView snch_scene::view(uint32_t m) const
 {
+    const uint32_t off = mesh_off[m], n = mesh_off[m + 1] - off, eoff = mesh_eoff[m];
     TreeView t{};
-    t.n_leaves = nF;
-    t.tnode = tnode.as<float4>();
-    t.tcone = tcone.as<float4>();
-    t.tleaf = tleaf.as<float4>();
-    t.tedge = tedge.as<float4>();
-    t.nbox = nbox.as<float4>();
-    t.ncone = ncone.as<float4>();
+    t.n_leaves = n;
+    t.tnode = tnode.as<float4>() + 4 * (size_t)(off - m);
+    t.tcone = tcone.as<float4>() + 3 * (size_t)(off - m);
+    t.tleaf = tleaf.as<float4>() + 3 * (size_t)off;
+    t.tedge = tedge.as<float4>() + 4 * (size_t)eoff;
+    t.nbox = nbox.as<float4>() + 2 * (size_t)(2 * off - m);
+    t.ncone = ncone.as<float4>() + 2 * (size_t)(2 * off - m);
     return t;
 }
 
@@ -202,45 +252,78 @@ int snch_scene_create_tri(snch_context* ctx, const float* V, uint32_t nV, const 
 {
     SNCH_CHECK_ARG(ctx && out, "snch_scene_create_tri: null context or output");
     *out = nullptr;
+    const uint32_t voff[2] = {0, nV}, foff[2] = {0, nF};
     if (nF == 0) {
         set_error("snch_scene_create_tri: empty mesh");
         return SNCH_ERR_EMPTY;
     }
-    SNCH_CHECK_ARG(V && F && nV > 0, "snch_scene_create_tri: null vertex or index array");
-    SNCH_CHECK_ARG(nF < 0x40000000u, "snch_scene_create_tri: too many triangles (limit 2^30)");
+    return snch_scene_create_tri_batch(ctx, 1, V, voff, F, foff, out);
+}
+
+int snch_scene_create_tri_batch(snch_context* ctx, uint32_t n_meshes, const float* V, const uint32_t* v_off,
+                                const int32_t* F, const uint32_t* f_off, snch_scene** out)
+{
+    SNCH_CHECK_ARG(ctx && out, "snch_scene_create_tri_batch: null context or output");
+    *out = nullptr;
+    SNCH_CHECK_ARG(n_meshes > 0 && v_off && f_off, "snch_scene_create_tri_batch: no meshes or null offsets");
+    SNCH_CHECK_ARG(v_off[0] == 0 && f_off[0] == 0, "snch_scene_create_tri_batch: offsets must start at 0");
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        SNCH_CHECK_ARG(v_off[m + 1] >= v_off[m] && f_off[m + 1] >= f_off[m],
+                       "snch_scene_create_tri_batch: offsets must not decrease");
+        if (f_off[m + 1] == f_off[m]) {
+            set_error("snch_scene_create_tri_batch: empty mesh");
+            return SNCH_ERR_EMPTY;
+        }
+        SNCH_CHECK_ARG(v_off[m + 1] > v_off[m], "snch_scene_create_tri_batch: mesh without vertices");
+        SNCH_CHECK_ARG(f_off[m + 1] - f_off[m] <= RUN_FIRST_MASK,
+                       "snch_scene_create_tri_batch: too many triangles in one mesh (limit 2^27 - 1)");
+    }
+    const uint32_t nV = v_off[n_meshes], nF = f_off[n_meshes];
+    SNCH_CHECK_ARG(V && F, "snch_scene_create_tri_batch: null vertex or index array");
+    SNCH_CHECK_ARG(nF < 0x40000000u, "snch_scene_create_tri_batch: too many triangles (limit 2^30)");
     for (size_t k = 0; k < 3 * (size_t)nV; ++k)
-        SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_tri: non-finite vertex coordinate");
-    for (size_t k = 0; k < 3 * (size_t)nF; ++k)
-        SNCH_CHECK_ARG(F[k] >= 0 && (uint32_t)F[k] < nV, "snch_scene_create_tri: vertex index out of range");
+        SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_tri_batch: non-finite vertex coordinate");
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        const uint32_t nv = v_off[m + 1] - v_off[m];
+        for (size_t k = 3 * (size_t)f_off[m]; k < 3 * (size_t)f_off[m + 1]; ++k)
+            SNCH_CHECK_ARG(F[k] >= 0 && (uint32_t)F[k] < nv, "snch_scene_create_tri_batch: vertex index out of range");
+    }
     SNCH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto* s = new snch_scene();
     s->ctx = ctx;
     s->dim = 3;
+    s->n_meshes = n_meshes;
     s->nV = nV;
     s->nF = nF;
-    s->V.assign(V, V + 3 * (size_t)nV);
-    s->F.assign(F, F + 3 * (size_t)nF);
-    edge_table_3d(s);
-    std::vector<float> V4(4 * (size_t)nV, 0.0f);
-    for (size_t k = 0; k < nV; ++k) {
-        V4[4 * k] = V[3 * k];
-        V4[4 * k + 1] = V[3 * k + 1];
-        V4[4 * k + 2] = V[3 * k + 2];
+    s->mesh_off.assign(f_off, f_off + n_meshes + 1);
+    s->mesh_voff.assign(v_off, v_off + n_meshes + 1);
+    s->mesh_toff.assign(1, 0);
+    s->mesh_eoff.assign(1, 0);
+    s->F.resize(3 * (size_t)nF);
+    s->estart.assign(1, 0);
+    std::vector<int32_t> slots_g;
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        const int32_t* Fm = F + 3 * (size_t)f_off[m];
+        const uint32_t nf = f_off[m + 1] - f_off[m], vo = v_off[m], eo = s->mesh_eoff[m];
+        EdgeTable t;
+        edge_table_3d(Fm, nf, &t);
+        for (size_t k = 0; k < 3 * (size_t)nf; ++k)
+            s->F[3 * (size_t)f_off[m] + k] = Fm[k] + (int32_t)vo;
+        s->slots.insert(s->slots.end(), t.slots.begin(), t.slots.end());
+        s->owned.insert(s->owned.end(), t.owned.begin(), t.owned.end());
+        for (int32_t x : t.slots)
+            slots_g.push_back(x < 0 ? x : x + (int32_t)vo);
+        for (uint32_t i = 1; i <= nf; ++i)
+            s->estart.push_back(eo + t.estart[i]);
+        s->mesh_toff.push_back(s->mesh_toff[m] + t.nE);
+        s->mesh_eoff.push_back(eo + t.nE); // in 3D every edge has exactly one owner
     }
-    hipError_t e;
-    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dF.ensure(sizeof(int32_t) * s->F.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dF.p, s->F.data(), sizeof(int32_t) * s->F.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(s->slots.size(), 4))) != hipSuccess ||
-        (s->slots.size() &&
-         (e = hipMemcpy(s->dSlots.p, s->slots.data(), sizeof(int32_t) * s->slots.size(), hipMemcpyHostToDevice)) !=
-             hipSuccess) ||
-        (e = s->dEstart.ensure(sizeof(uint32_t) * s->estart.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dEstart.p, s->estart.data(), sizeof(uint32_t) * s->estart.size(),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
+    s->nE = s->mesh_eoff[n_meshes];
+    std::vector<float> Vv(V, V + 3 * (size_t)nV);
+    const hipError_t e = upload_scene(s, 3, Vv, s->F, slots_g, s->estart);
+    if (e != hipSuccess) {
         delete s;
-        return hip_fail(e, "snch_scene_create_tri: upload");
+        return hip_fail(e, "snch_scene_create_tri_batch: upload");
     }
     *out = s;
     return SNCH_OK;
@@ -256,7 +339,7 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
         return SNCH_ERR_EMPTY;
     }
     SNCH_CHECK_ARG(V && S && nV > 0, "snch_scene_create_seg: null vertex or index array");
-    SNCH_CHECK_ARG(nS < 0x40000000u, "snch_scene_create_seg: too many segments (limit 2^30)");
+    SNCH_CHECK_ARG(nS <= RUN_FIRST_MASK, "snch_scene_create_seg: too many segments (limit 2^27 - 1)");
     for (size_t k = 0; k < 2 * (size_t)nV; ++k)
         SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_seg: non-finite vertex coordinate");
     for (size_t k = 0; k < 2 * (size_t)nS; ++k)
@@ -267,7 +350,6 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
     s->dim = 2;
     s->nV = nV;
     s->nF = nS;
-    s->V.assign(V, V + 2 * (size_t)nV);
     s->F.assign(S, S + 2 * (size_t)nS);
     // silhouette vertices (scene.cuh:631-652): (prev, v, next, -1) per vertex
     s->slots.assign(4 * (size_t)nV, -1);
@@ -299,23 +381,13 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
         s->estart[i + 1] = next;
     }
     s->nE = nV;
-    std::vector<float> V4(4 * (size_t)nV, 0.0f);
-    for (size_t k = 0; k < nV; ++k) {
-        V4[4 * k] = V[2 * k];
-        V4[4 * k + 1] = V[2 * k + 1];
-    }
-    hipError_t e;
-    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dF.ensure(sizeof(int32_t) * s->F.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dF.p, s->F.data(), sizeof(int32_t) * s->F.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(internal_slots.size(), 4))) != hipSuccess ||
-        (internal_slots.size() &&
-         (e = hipMemcpy(s->dSlots.p, internal_slots.data(), sizeof(int32_t) * internal_slots.size(),
-                        hipMemcpyHostToDevice)) != hipSuccess) ||
-        (e = s->dEstart.ensure(sizeof(uint32_t) * s->estart.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dEstart.p, s->estart.data(), sizeof(uint32_t) * s->estart.size(),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
+    s->mesh_off = {0, nS};
+    s->mesh_voff = {0, nV};
+    s->mesh_toff = {0, nV};
+    s->mesh_eoff = {0, next};
+    std::vector<float> Vv(V, V + 2 * (size_t)nV);
+    const hipError_t e = upload_scene(s, 2, Vv, s->F, internal_slots, s->estart);
+    if (e != hipSuccess) {
         delete s;
         return hip_fail(e, "snch_scene_create_seg: upload");
     }
@@ -350,14 +422,14 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
     SNCH_CHECK_ARG(s, "snch_scene_build: null scene");
     const uint32_t flags = opts ? opts->flags : 0u;
     const uint32_t leaf = opts && opts->leaf_size ? opts->leaf_size : 1u;
-    SNCH_CHECK_ARG(leaf == 1u, "snch_scene_build: leaf_size other than 1 is not available in this build");
+    SNCH_CHECK_ARG(leaf <= MAX_LEAF_SIZE, "snch_scene_build: leaf_size must be 1..8");
     SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipEvent_t e0, e1;
     SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
     (void)hipEventRecord(e0, st);
-    hipError_t e = build_tree(s, flags, st);
+    hipError_t e = build_tree(s, flags, leaf, st);
     (void)hipEventRecord(e1, st);
     hipError_t es = hipStreamSynchronize(st);
     float ms = 0.0f;
@@ -368,8 +440,17 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
         return hip_fail(e, "snch_scene_build");
     if (es != hipSuccess)
         return hip_fail(es, "snch_scene_build: synchronize");
+    {
+        std::vector<TreeView> views(s->n_meshes);
+        for (uint32_t m = 0; m < s->n_meshes; ++m)
+            views[m] = s->view(m);
+        SNCH_HIP(s->batch.views.ensure(sizeof(TreeView) * views.size()), "snch_scene_build: views");
+        SNCH_HIP(hipMemcpy(s->batch.views.p, views.data(), sizeof(TreeView) * views.size(), hipMemcpyHostToDevice),
+                 "snch_scene_build: views");
+    }
     s->built = true;
     s->build_flags = flags;
+    s->leaf_size = leaf;
     s->last_build_ms = ms;
     return SNCH_OK;
 }
@@ -390,6 +471,12 @@ int snch_scene_build_time(const snch_scene* s, float* ms)
         }                                                        \
     } while (0)
 
+#define SNCH_NEED_SINGLE(s, fn)                                              \
+    do {                                                                     \
+        SNCH_NEED_BUILT(s, fn);                                              \
+        SNCH_CHECK_ARG((s)->n_meshes == 1, fn ": the scene holds several meshes; use the batch entry points"); \
+    } while (0)
+
 // query visiting order: 0 = Morton order from SNCH_ORDER_MIN_QUERIES queries
 // up, 1 = always Morton order, 2 = input order
 static bool want_order(const snch_query_opts* opts, uint32_t n)
@@ -400,7 +487,7 @@ static bool want_order(const snch_query_opts* opts, uint32_t n)
 
 int snch_query_order(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_perm, void* stream)
 {
-    SNCH_NEED_BUILT(s, "snch_query_order");
+    SNCH_NEED_SINGLE(s, "snch_query_order");
     SNCH_CHECK_ARG(n == 0 || (q && out_perm), "snch_query_order: null pointer");
     if (n == 0)
         return SNCH_OK;
@@ -418,7 +505,7 @@ int snch_query_order(const snch_scene* s, const float* q, uint32_t n, uint32_t* 
 int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint32_t* out_idx, float* out_dist,
                           const snch_query_opts* opts, void* stream)
 {
-    SNCH_NEED_BUILT(s, "snch_closest_point");
+    SNCH_NEED_SINGLE(s, "snch_closest_point");
     SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_closest_point: null pointer");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = opts ? opts->order : nullptr;
@@ -439,7 +526,7 @@ int snch_closest_point(const snch_scene* s, const float* q, uint32_t n, uint32_t
 int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, int flip, const float* seed,
                                uint32_t* out_sil, float* out_dist, const snch_query_opts* opts, void* stream)
 {
-    SNCH_NEED_BUILT(s, "snch_closest_silhouette");
+    SNCH_NEED_SINGLE(s, "snch_closest_silhouette");
     SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
     uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
@@ -465,7 +552,7 @@ int snch_ray_ex(const snch_scene* s, const float* o, const float* d, const float
                 uint8_t* out_hit, float* out_t, float* out_uv, uint32_t* out_idx, const snch_query_opts* opts,
                 void* stream)
 {
-    SNCH_NEED_BUILT(s, "snch_ray");
+    SNCH_NEED_SINGLE(s, "snch_ray");
     SNCH_CHECK_ARG(n == 0 || (o && d && out_hit), "snch_ray: null pointer");
     SNCH_HIP(launch_ray(s->view(), (int)s->dim, o, d, tmax, n, any_hit, out_hit, out_t, out_uv, out_idx,
                         opts ? opts->stats : nullptr, static_cast<hipStream_t>(stream)),
@@ -482,7 +569,7 @@ int snch_ray(const snch_scene* s, const float* o, const float* d, const float* t
 int snch_sample_in_sphere(const snch_scene* s, const float* c, const float* r, const float* u, uint32_t n,
                           int32_t* out_idx, float* out_pdf, float* out_pt, void* stream)
 {
-    SNCH_NEED_BUILT(s, "snch_sample_in_sphere");
+    SNCH_NEED_SINGLE(s, "snch_sample_in_sphere");
     SNCH_CHECK_ARG(n == 0 || (c && r && u && out_idx && out_pdf), "snch_sample_in_sphere: null pointer");
     SNCH_HIP(launch_sample(s->view(), (int)s->dim, s->dV.as<float4>(), s->dF.as<int32_t>(), c, r, u, n, out_idx, out_pdf, out_pt,
                            static_cast<hipStream_t>(stream)),
@@ -501,31 +588,197 @@ int snch_sample_on_object(const snch_scene* s, const int32_t* idx, const float* 
     return SNCH_OK;
 }
 
+static int batch_config(const snch_query_opts* opts, uint32_t n, BatchConfig* cfg, const char* fn)
+{
+    cfg->block_size = opts && opts->block_size ? opts->block_size : 256u;
+    cfg->stack_depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
+    cfg->queries_per_wave = opts && opts->queries_per_wave ? opts->queries_per_wave : 64u;
+    const uint32_t mode = opts ? opts->coherent : 0u;
+    cfg->coherent = mode == 1u || (mode == 0u && n >= SNCH_ORDER_MIN_QUERIES);
+    const uint32_t b = cfg->block_size, d = cfg->stack_depth, w = cfg->queries_per_wave;
+    if (!(b == 64 || b == 128 || b == 256 || b == 512) || !(d == 16 || d == 24 || d == 32) || w % 64 != 0 ||
+        w > 4096 || (opts && opts->order)) {
+        set_error(std::string(fn) + ": block_size must be 64..512 (power of 2), stack_depth 16, 24 or 32, "
+                                    "queries_per_wave a multiple of 64 up to 4096, and order NULL");
+        return SNCH_ERR_INVALID;
+    }
+    return SNCH_OK;
+}
+
+static int check_qoff(const snch_scene* s, const uint32_t* q_off, const char* fn)
+{
+    if (!q_off || q_off[0] != 0) {
+        set_error(std::string(fn) + ": q_off must be a host array of n_meshes + 1 entries starting at 0");
+        return SNCH_ERR_INVALID;
+    }
+    for (uint32_t m = 0; m < s->n_meshes; ++m)
+        if (q_off[m + 1] < q_off[m]) {
+            set_error(std::string(fn) + ": q_off must not decrease");
+            return SNCH_ERR_INVALID;
+        }
+    return SNCH_OK;
+}
+
+int snch_batch_closest_point(const snch_scene* s, const float* q, const uint32_t* q_off, uint32_t* out_idx,
+                             float* out_dist, const snch_query_opts* opts, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_batch_closest_point");
+    SNCH_CHECK_ARG(s->dim == 3, "snch_batch_closest_point: batches hold triangle meshes");
+    int rc = check_qoff(s, q_off, "snch_batch_closest_point");
+    if (rc != SNCH_OK)
+        return rc;
+    const uint32_t n = q_off[s->n_meshes];
+    SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_batch_closest_point: null pointer");
+    BatchConfig cfg;
+    if ((rc = batch_config(opts, n, &cfg, "snch_batch_closest_point")) != SNCH_OK)
+        return rc;
+    SNCH_HIP(launch_batch(s, 0, q, q_off, 0, out_idx, out_dist, cfg, static_cast<hipStream_t>(stream)),
+             "snch_batch_closest_point");
+    return SNCH_OK;
+}
+
+int snch_batch_closest_silhouette(const snch_scene* s, const float* q, const uint32_t* q_off, int flip,
+                                  uint32_t* out_sil, float* out_dist, const snch_query_opts* opts, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_batch_closest_silhouette");
+    SNCH_CHECK_ARG(s->dim == 3, "snch_batch_closest_silhouette: batches hold triangle meshes");
+    int rc = check_qoff(s, q_off, "snch_batch_closest_silhouette");
+    if (rc != SNCH_OK)
+        return rc;
+    const uint32_t n = q_off[s->n_meshes];
+    SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_batch_closest_silhouette: null pointer");
+    BatchConfig cfg;
+    if ((rc = batch_config(opts, n, &cfg, "snch_batch_closest_silhouette")) != SNCH_OK)
+        return rc;
+    SNCH_HIP(launch_batch(s, 1, q, q_off, flip, out_sil, out_dist, cfg, static_cast<hipStream_t>(stream)),
+             "snch_batch_closest_silhouette");
+    return SNCH_OK;
+}
+
+int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint32_t* q_off,
+                        uint32_t sample_per_mesh, int flip, snch_tune_result* out, void* stream)
+{
+    SNCH_NEED_BUILT(s, "snch_batch_autotune");
+    SNCH_CHECK_ARG(out && kind <= 1u && sample_per_mesh > 0 && s->dim == 3, "snch_batch_autotune: bad argument");
+    int rc = check_qoff(s, q_off, "snch_batch_autotune");
+    if (rc != SNCH_OK)
+        return rc;
+    SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t M = s->n_meshes;
+    // the sample: the first queries of each mesh, gathered into one batch
+    std::vector<uint32_t> soff(M + 1, 0);
+    for (uint32_t m = 0; m < M; ++m)
+        soff[m + 1] = soff[m] + std::min(q_off[m + 1] - q_off[m], sample_per_mesh);
+    const uint32_t ns = soff[M];
+    SNCH_CHECK_ARG(ns > 0 && q, "snch_batch_autotune: no queries");
+    DeviceBuffer sq, sidx, sdist;
+    SNCH_HIP(sq.ensure(sizeof(float) * 3 * (size_t)ns), "snch_batch_autotune: alloc");
+    SNCH_HIP(sidx.ensure(sizeof(uint32_t) * ns), "snch_batch_autotune: alloc");
+    SNCH_HIP(sdist.ensure(sizeof(float) * ns), "snch_batch_autotune: alloc");
+    for (uint32_t m = 0; m < M; ++m)
+        if (soff[m + 1] > soff[m])
+            SNCH_HIP(hipMemcpyAsync(sq.as<float>() + 3 * (size_t)soff[m], q + 3 * (size_t)q_off[m],
+                                    sizeof(float) * 3 * (soff[m + 1] - soff[m]), hipMemcpyDeviceToDevice, st),
+                     "snch_batch_autotune: gather");
+    hipEvent_t e0, e1;
+    SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
+    SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    const uint32_t flags = s->build_flags;
+    const uint32_t leaves[4] = {1, 2, 4, 8}, blocks[4] = {64, 128, 256, 512}, depths[3] = {16, 24, 32},
+                   qpws[3] = {64, 128, 256};
+    snch_tune_result best{1, 256, 16, 64, 1e30f, 0.0f, 0};
+    hipError_t err = hipSuccess;
+    for (uint32_t L : leaves) {
+        if ((err = build_tree(s, flags, L, st)) != hipSuccess)
+            break;
+        for (uint32_t B : blocks)
+            for (uint32_t D : depths)
+                for (uint32_t W : qpws) {
+                    BatchConfig cfg;
+                    cfg.block_size = B;
+                    cfg.stack_depth = D;
+                    cfg.queries_per_wave = W;
+                    cfg.coherent = true;
+                    float ms = 0.0f;
+                    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {
+                        if (rep == 1)
+                            (void)hipEventRecord(e0, st);
+                        err = launch_batch(s, (int)kind, sq.as<float>(), soff.data(), flip, sidx.as<uint32_t>(),
+                                           sdist.as<float>(), cfg, st);
+                    }
+                    (void)hipEventRecord(e1, st);
+                    if (err == hipSuccess)
+                        err = hipEventSynchronize(e1);
+                    if (err != hipSuccess)
+                        break;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    ms *= 0.5f;
+                    ++best.n_tried;
+                    if (L == 1 && B == 256 && D == 16 && W == 64)
+                        best.ms_default = ms;
+                    if (ms < best.ms) {
+                        best.ms = ms;
+                        best.leaf_size = L;
+                        best.block_size = B;
+                        best.stack_depth = D;
+                        best.queries_per_wave = W;
+                    }
+                }
+        if (err != hipSuccess)
+            break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err == hipSuccess)
+        err = build_tree(s, flags, best.leaf_size, st);
+    if (err == hipSuccess)
+        err = hipStreamSynchronize(st);
+    if (err != hipSuccess)
+        return hip_fail(err, "snch_batch_autotune");
+    s->leaf_size = best.leaf_size;
+    *out = best;
+    return SNCH_OK;
+}
+
 int snch_scene_export(const snch_scene* s, uint32_t* nodes, float* boxes, float* cones, uint32_t* sorted_codes,
                       uint32_t* sorted_idx)
 {
-    SNCH_NEED_BUILT(s, "snch_scene_export");
+    SNCH_NEED_SINGLE(s, "snch_scene_export");
+    return snch_scene_export_mesh(s, 0, nodes, boxes, cones, sorted_codes, sorted_idx);
+}
+
+int snch_scene_export_mesh(const snch_scene* s, uint32_t mesh, uint32_t* nodes, float* boxes, float* cones,
+                           uint32_t* sorted_codes, uint32_t* sorted_idx)
+{
+    SNCH_NEED_BUILT(s, "snch_scene_export_mesh");
+    SNCH_CHECK_ARG(mesh < s->n_meshes, "snch_scene_export_mesh: mesh index out of range");
     SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
-    const uint32_t N = s->nF, NN = 2 * N - 1;
+    const uint32_t off = s->mesh_off[mesh], N = s->mesh_off[mesh + 1] - off, NN = 2 * N - 1;
+    const uint32_t nb = 2 * off - mesh;
     const BuildScratch& w = s->scratch;
     const uint32_t* dcodes = s->sorted_in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* didx = s->sorted_in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
-    std::vector<uint32_t> sidx(N), parent(NN), child(2 * (size_t)(N > 1 ? N - 1 : 1));
-    SNCH_HIP(hipMemcpy(sidx.data(), didx, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
+    std::vector<uint32_t> sidx(N), parent(NN), child(2 * (size_t)NN);
+    SNCH_HIP(hipMemcpy(sidx.data(), didx + off, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
+    for (auto& x : sidx)
+        x -= off;
     if (sorted_codes)
-        SNCH_HIP(hipMemcpy(sorted_codes, dcodes, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
+        SNCH_HIP(hipMemcpy(sorted_codes, dcodes + off, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "export");
     if (sorted_idx)
         std::memcpy(sorted_idx, sidx.data(), sizeof(uint32_t) * N);
     if (nodes) {
-        SNCH_HIP(hipMemcpy(parent.data(), w.parent.p, sizeof(uint32_t) * NN, hipMemcpyDeviceToHost), "export");
-        if (N > 1)
-            SNCH_HIP(hipMemcpy(child.data(), w.child.p, sizeof(uint32_t) * 2 * (N - 1), hipMemcpyDeviceToHost),
-                     "export");
+        SNCH_HIP(hipMemcpy(parent.data(), w.parent.as<uint32_t>() + nb, sizeof(uint32_t) * NN,
+                           hipMemcpyDeviceToHost),
+                 "export");
+        SNCH_HIP(hipMemcpy(child.data(), w.child.as<uint32_t>() + 2 * (size_t)nb, sizeof(uint32_t) * 2 * NN,
+                           hipMemcpyDeviceToHost),
+                 "export");
         for (uint32_t i = 0; i < NN; ++i) {
-            nodes[4 * i] = parent[i];
+            nodes[4 * i] = parent[i] == INVALID ? INVALID : parent[i] - nb;
             if (i + 1 < N) {
-                nodes[4 * i + 1] = child[2 * i];
-                nodes[4 * i + 2] = child[2 * i + 1];
+                nodes[4 * i + 1] = child[2 * i] - nb;
+                nodes[4 * i + 2] = child[2 * i + 1] - nb;
                 nodes[4 * i + 3] = INVALID;
             } else {
                 nodes[4 * i + 1] = INVALID;
@@ -535,17 +788,21 @@ int snch_scene_export(const snch_scene* s, uint32_t* nodes, float* boxes, float*
         }
     }
     if (boxes || cones) {
-        std::vector<float> b(8 * (size_t)NN), c(8 * (size_t)NN);
-        SNCH_HIP(hipMemcpy(b.data(), s->nbox.p, sizeof(float) * b.size(), hipMemcpyDeviceToHost), "export");
-        SNCH_HIP(hipMemcpy(c.data(), s->ncone.p, sizeof(float) * c.size(), hipMemcpyDeviceToHost), "export");
+        std::vector<float> bx(8 * (size_t)NN), c(8 * (size_t)NN);
+        SNCH_HIP(hipMemcpy(bx.data(), s->nbox.as<float>() + 8 * (size_t)nb, sizeof(float) * bx.size(),
+                           hipMemcpyDeviceToHost),
+                 "export");
+        SNCH_HIP(hipMemcpy(c.data(), s->ncone.as<float>() + 8 * (size_t)nb, sizeof(float) * c.size(),
+                           hipMemcpyDeviceToHost),
+                 "export");
         for (uint32_t i = 0; i < NN; ++i) {
             if (boxes) {
-                boxes[6 * i + 0] = b[8 * i + 0];
-                boxes[6 * i + 1] = b[8 * i + 1];
-                boxes[6 * i + 2] = b[8 * i + 2];
-                boxes[6 * i + 3] = b[8 * i + 4];
-                boxes[6 * i + 4] = b[8 * i + 5];
-                boxes[6 * i + 5] = b[8 * i + 6];
+                boxes[6 * i + 0] = bx[8 * i + 0];
+                boxes[6 * i + 1] = bx[8 * i + 1];
+                boxes[6 * i + 2] = bx[8 * i + 2];
+                boxes[6 * i + 3] = bx[8 * i + 4];
+                boxes[6 * i + 4] = bx[8 * i + 5];
+                boxes[6 * i + 5] = bx[8 * i + 6];
             }
             if (cones) {
                 cones[5 * i + 0] = c[8 * i + 0];
@@ -556,6 +813,13 @@ int snch_scene_export(const snch_scene* s, uint32_t* nodes, float* boxes, float*
             }
         }
     }
+    return SNCH_OK;
+}
+
+int snch_scene_mesh_count(const snch_scene* s, uint32_t* n_meshes)
+{
+    SNCH_CHECK_ARG(s && n_meshes, "snch_scene_mesh_count: null argument");
+    *n_meshes = s->n_meshes;
     return SNCH_OK;
 }
 

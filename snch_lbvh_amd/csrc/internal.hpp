@@ -20,9 +20,12 @@ namespace snch {
 //               [0] L.lo.x L.lo.y L.lo.z L.hi.x
 //               [1] L.hi.y L.hi.z R.lo.x R.lo.y
 //               [2] R.lo.z R.hi.x R.hi.y R.hi.z
-//               [3] ref(L) ref(R) 0      0
+//               [3] ref(L) ref(R) run(L) run(R)
 //             ref(c) = c for an internal child, LEAF_BIT | j for the leaf in
-//             sorted position j.
+//             sorted position j.  run(c) = INVALID, or, when the tree is
+//             built with leaf size k > 1 and internal child c covers at most
+//             k leaves, (count - 1) << 27 | first: the traversal queries then
+//             treat c as one leaf holding sorted leaves [first, first + count).
 //   tcone[i]  3 x float4 = 48 B per internal node: the children's normal
 //             cones, read only by the silhouette query:
 //               [0] L.axis L.half_angle  [1] R.axis R.half_angle
@@ -36,8 +39,21 @@ namespace snch {
 //               bit1 has face 1)  [2] n0 xyz  [3] n1 xyz (unit face normals)
 // The reference layout (node{parent,left,right,object}, aabb, cone per node;
 // bvh.cuh:27-33) is kept alongside for export and for the refit's own use.
+//
+// A scene may hold M meshes (a batch).  Mesh m owns primitives (and sorted
+// leaf positions) [off, off + n), silhouette records [eoff, ...), nodes
+// [nb, nb + 2n - 1) with nb = 2 off - m (its n - 1 internal nodes first, root
+// at nb), and internal records [off - m, off - m + n - 1).  A single mesh is
+// the batch of one, with every offset zero.  Records hold mesh-local ids.
 // ---------------------------------------------------------------------------
 constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t RUN_SHIFT = 27;
+constexpr uint32_t RUN_FIRST_MASK = (1u << RUN_SHIFT) - 1u;
+constexpr uint32_t MAX_LEAF_SIZE = 8;
+
+struct MeshDesc {
+    uint32_t off, n, eoff, nb;
+};
 
 struct TreeView {
     uint32_t n_leaves;
@@ -66,6 +82,19 @@ struct QueryScratch {
     DeviceBuffer codes, idx, codes_alt, idx_alt, hist;
 };
 
+// per-scene state of batched queries: device copies of the per-mesh views
+// (uploaded at build time) and of the current query table
+struct BatchState {
+    DeviceBuffer views, tab, counter, qmesh;
+    std::vector<uint32_t> host_tab;
+    QueryScratch order;
+};
+
+struct BatchConfig {
+    uint32_t block_size = 256, stack_depth = 16, queries_per_wave = 64;
+    bool coherent = true;
+};
+
 struct BuildScratch {
     DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
         top, status;
@@ -81,25 +110,33 @@ struct snch_context {
 struct snch_scene {
     snch_context* ctx = nullptr;
     uint32_t dim = 3;
+    uint32_t n_meshes = 1;
+    // per-mesh offsets (n_meshes + 1 each): primitives, vertices, silhouette
+    // table entries, packed (owned) silhouette records
+    std::vector<uint32_t> mesh_off, mesh_voff, mesh_toff, mesh_eoff;
+    snch::DeviceBuffer dMesh, dPmesh; // MeshDesc per mesh; mesh id per primitive (batches only)
+    uint32_t leaf_size = 1;
     uint32_t nV = 0, nF = 0, nE = 0; // nE: silhouette table size (edges in 3D, vertices in 2D)
     uint32_t n_internal_sil = 0;     // packed silhouette records (owned silhouettes)
-    // host copies of the inputs and of the silhouette table
-    std::vector<float> V;
+    // host copies: primitives (scene-global vertex indices), the silhouette
+    // table and ownership of each mesh (mesh-local ids, concatenated), and
+    // the first owned silhouette record of each primitive (scene-global)
     std::vector<int32_t> F;
-    std::vector<int32_t> slots;  // 4 per silhouette
-    std::vector<int32_t> owned;  // 3 per primitive
-    std::vector<uint32_t> estart; // first owned silhouette id per primitive (nF + 1)
+    std::vector<int32_t> slots;   // 4 per silhouette
+    std::vector<int32_t> owned;   // 3 (2 in 2D) per primitive
+    std::vector<uint32_t> estart; // nF + 1
     // device inputs
     snch::DeviceBuffer dV, dF, dSlots, dEstart;
     // device tree
     snch::DeviceBuffer tnode, tcone, tleaf, tedge, nbox, ncone;
     snch::BuildScratch scratch;
     mutable snch::QueryScratch qscratch; // used by const query entry points
+    mutable snch::BatchState batch;
     bool built = false;
     bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
     uint32_t build_flags = 0;
     float last_build_ms = 0.0f;
-    snch::TreeView view() const;
+    snch::TreeView view(uint32_t mesh = 0) const;
 };
 
 namespace snch {
@@ -107,10 +144,12 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
 // build.hip
-hipError_t build_tree(snch_scene* s, uint32_t flags, hipStream_t st);
+hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st);
 // sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
+// of the key, then (seg != null) on the low `seg_bits` bits of seg[value]
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
-                            int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt);
+                            int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt,
+                            const uint32_t* seg = nullptr, int seg_bits = 0);
 // query.hip (dim = 2 or 3).  perm (nullable): slot -> query index, the
 // order in which queries are visited (query_order computes a Morton order)
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
@@ -128,5 +167,8 @@ hipError_t launch_sample(const TreeView& t, int dim, const float4* V, const int3
                          hipStream_t st);
 hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
                                    const float* uv, uint32_t n, float* pt, hipStream_t st);
+// batched closest point (kind 0) / closest silhouette (kind 1) over all meshes
+hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
+                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st);
 uint32_t* traversal_error_word();
 } // namespace snch

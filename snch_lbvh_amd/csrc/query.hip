@@ -25,13 +25,16 @@
 //    range of queries (blocks are dispatched round-robin over XCDs).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "internal.hpp"
 
 namespace snch {
 
 namespace {
 
-constexpr int SS = 16;  // LDS stack entries per lane
+constexpr int SS = 16;  // default LDS stack entries per lane
 constexpr int OVF = 64; // private overflow entries per lane
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
@@ -40,7 +43,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
     return x * q + (x < r ? x : r) + k;
 }
 
-template <int BS>
+// per-lane traversal stack: the top SSZ entries in LDS (a column per lane, so
+// the lanes of a wave hit distinct banks), older entries spilled to a private
+// array; the tree depth is at most 64, so the stack never loses an entry
+template <int BS, int SSZ>
 struct LaneStack {
     uint2* lds; // this lane's column: entry s at lds[s * BS]
     int sp;
@@ -56,9 +62,9 @@ struct LaneStack {
     }
     __device__ __forceinline__ void push(uint32_t n, float d)
     {
-        const int s = sp & (SS - 1);
-        if (sp >= SS) {
-            const int o = sp - SS;
+        const int s = sp % SSZ;
+        if (sp >= SSZ) {
+            const int o = sp - SSZ;
             if (o < OVF) {
                 const uint2 old = lds[s * BS];
                 ovn[o] = old.x;
@@ -74,12 +80,12 @@ struct LaneStack {
     __device__ __forceinline__ void pop(uint32_t& n, float& d)
     {
         --sp;
-        const int s = sp & (SS - 1);
+        const int s = sp % SSZ;
         const uint2 e = lds[s * BS];
         n = e.x;
         d = __uint_as_float(e.y);
-        if (sp >= SS) {
-            const int o = sp - SS;
+        if (sp >= SSZ) {
+            const int o = sp - SSZ;
             lds[s * BS] = make_uint2(ovn[o], __float_as_uint(ovd[o]));
         }
     }
@@ -107,7 +113,7 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
 
 struct NodeRec {
     box3 L, R;
-    uint32_t lref, rref;
+    uint32_t lref, rref, lrun, rrun;
 };
 
 __device__ __forceinline__ NodeRec load_node(const float4* __restrict__ tnode, uint32_t i)
@@ -120,7 +126,26 @@ __device__ __forceinline__ NodeRec load_node(const float4* __restrict__ tnode, u
     n.R.hi = mk(c.y, c.z, c.w);
     n.lref = __float_as_uint(r.x);
     n.rref = __float_as_uint(r.y);
+    n.lrun = __float_as_uint(r.z);
+    n.rrun = __float_as_uint(r.w);
     return n;
+}
+
+// a child the traversal treats as a leaf: a true leaf, or a run of sorted
+// leaves [first, first + count) under an internal node (leaf size > 1)
+__device__ __forceinline__ bool as_leaf(uint32_t ref, uint32_t run, uint32_t& first, uint32_t& count)
+{
+    if (ref & LEAF_BIT) {
+        first = ref & ~LEAF_BIT;
+        count = 1;
+        return true;
+    }
+    if (run != INVALID) {
+        first = run & RUN_FIRST_MASK;
+        count = (run >> RUN_SHIFT) + 1;
+        return true;
+    }
+    return false;
 }
 
 __device__ __forceinline__ box3 ldbox2(const float4* nb, uint32_t n)
@@ -132,13 +157,13 @@ __device__ __forceinline__ box3 ldbox2(const float4* nb, uint32_t n)
     return b;
 }
 
-// ------------------------------------------------------------ closest point
 template <int DIM>
 __device__ __forceinline__ v3 load_point(const float* __restrict__ q, uint32_t i)
 {
     return DIM == 3 ? mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]) : mk(q[2 * i], q[2 * i + 1], 0.0f);
 }
 
+// ------------------------------------------------------------ closest point
 template <int DIM>
 __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, float& best, uint32_t& best_idx)
 {
@@ -160,6 +185,76 @@ __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, flo
     }
 }
 
+// query_device(nearest) (query.cuh:238-318) for one query point; best is the
+// squared distance
+template <int BS, int SSZ, bool STATS, int DIM>
+__device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint2* stk_col, float& best, uint32_t& best_idx,
+                                         Counts& cnt, bool& overflow)
+{
+    best = F_INF;
+    best_idx = INVALID;
+    if (t.n_leaves == 1) {
+        cp_leaf<DIM>(t, 0, p, best, best_idx);
+        return;
+    }
+    LaneStack<BS, SSZ> st;
+    st.init(stk_col);
+    uint32_t node = 0;
+    // each internal node is visited at most once; the cap and the range
+    // checks only guard against a malformed tree
+    const uint32_t cap_iters = 2 * t.n_leaves + 64;
+    for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+        const NodeRec nr = load_node(t.tnode, node);
+        const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
+        const float Lmm = box_minmaxdist(nr.L, p), Rmm = box_minmaxdist(nr.R, p);
+        const bool cL = Lmd <= Rmm, cR = Rmd <= Lmm; // query.cuh:279, :297
+        uint32_t lf, lc, rf, rc;
+        const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
+        if (STATS) {
+            cnt.node += 1;
+            cnt.box += 2;
+        }
+        if (cL && lLeaf && Lmd <= best) {
+            if (STATS)
+                cnt.leaf += lc;
+            for (uint32_t k = 0; k < lc; ++k)
+                cp_leaf<DIM>(t, lf + k, p, best, best_idx);
+        }
+        if (cR && rLeaf && Rmd <= best) {
+            if (STATS)
+                cnt.leaf += rc;
+            for (uint32_t k = 0; k < rc; ++k)
+                cp_leaf<DIM>(t, rf + k, p, best, best_idx);
+        }
+        const bool gL = cL && !lLeaf && Lmd <= best;
+        const bool gR = cR && !rLeaf && Rmd <= best;
+        if (gL && gR) {
+            const bool lfirst = Lmd <= Rmd;
+            st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
+            node = lfirst ? nr.lref : nr.rref;
+            continue;
+        }
+        if (gL || gR) {
+            node = gL ? nr.lref : nr.rref;
+            continue;
+        }
+        bool found = false;
+        while (st.sp > 0) {
+            uint32_t nn;
+            float dd;
+            st.pop(nn, dd);
+            if (dd <= best) { // query.cuh:259
+                node = nn;
+                found = true;
+                break;
+            }
+        }
+        if (!found)
+            break;
+    }
+    overflow = overflow || st.overflow;
+}
+
 template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* __restrict__ q, uint32_t n,
                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
@@ -177,63 +272,12 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
         return;
     }
     const uint32_t qi = perm ? min(perm[slot], n - 1) : slot;
-    const v3 p = load_point<DIM>(q, qi);
-    float best = F_INF;
-    uint32_t best_idx = INVALID;
-    if (t.n_leaves == 1) {
-        cp_leaf<DIM>(t, 0, p, best, best_idx);
-    } else {
-        LaneStack<BS> st;
-        st.init(stk + threadIdx.x);
-        uint32_t node = 0;
-        // each internal node is visited at most once; the cap and the range
-        // checks only guard against a malformed tree
-        const uint32_t cap_iters = 2 * t.n_leaves + 64;
-        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
-            const NodeRec nr = load_node(t.tnode, node);
-            const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
-            const float Lmm = box_minmaxdist(nr.L, p), Rmm = box_minmaxdist(nr.R, p);
-            const bool cL = Lmd <= Rmm, cR = Rmd <= Lmm; // query.cuh:279, :297
-            const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
-            if (STATS) {
-                cnt.node += 1;
-                cnt.box += 2;
-                cnt.leaf += (cL && lLeaf && Lmd <= best) + (cR && rLeaf && Rmd <= best);
-            }
-            if (cL && lLeaf && Lmd <= best)
-                cp_leaf<DIM>(t, nr.lref & ~LEAF_BIT, p, best, best_idx);
-            if (cR && rLeaf && Rmd <= best)
-                cp_leaf<DIM>(t, nr.rref & ~LEAF_BIT, p, best, best_idx);
-            const bool gL = cL && !lLeaf && Lmd <= best;
-            const bool gR = cR && !rLeaf && Rmd <= best;
-            if (gL && gR) {
-                const bool lfirst = Lmd <= Rmd;
-                st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
-                node = lfirst ? nr.lref : nr.rref;
-                continue;
-            }
-            if (gL || gR) {
-                node = gL ? nr.lref : nr.rref;
-                continue;
-            }
-            bool found = false;
-            while (st.sp > 0) {
-                uint32_t nn;
-                float dd;
-                st.pop(nn, dd);
-                if (dd <= best) { // query.cuh:259
-                    node = nn;
-                    found = true;
-                    break;
-                }
-            }
-            if (!found)
-                break;
-        }
-        if (st.overflow)
-            if (err)
-                atomicOr(err, 1u);
-    }
+    float best;
+    uint32_t best_idx;
+    bool ovf = false;
+    cp_query<BS, SS, STATS, DIM>(t, load_point<DIM>(q, qi), stk + threadIdx.x, best, best_idx, cnt, ovf);
+    if (ovf && err)
+        atomicOr(err, 1u);
     out_idx[qi] = best_idx;
     out_dist[qi] = dm::sqrt_(best);
     if (STATS)
@@ -299,6 +343,94 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
                     : cone_overlap2d(mk(C.x, C.y, 0.0f), C.w, radius, p, b, md);
 }
 
+// query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
+// A seeded radius `cap` (SEEDED) caps the search: nodes farther than cap are
+// skipped.  A silhouette found at distance <= cap is the global answer,
+// because every node that could hold one at least as close was visited;
+// otherwise the search reruns without the cap, keeping what it found as the
+// initial bound.  With index tie-breaking the result is bit-identical to the
+// unseeded search.
+template <int BS, int SSZ, bool STATS, int DIM, bool SEEDED>
+__device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint2* stk_col,
+                                          float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
+{
+    best = F_INF;
+    best_e = INVALID;
+    if (t.n_leaves == 1) {
+        const box3 b = ldbox2(t.nbox, 0);
+        const float4 c0 = t.ncone[0], c1 = t.ncone[1];
+        if (cone_valid(c0.w) && overlap_dim<DIM>(c0, c1.x, p, b, box_mindist(b, p)))
+            sil_leaf<DIM>(t, 0, p, flip, best, best_e, cnt);
+        return;
+    }
+    if (!SEEDED || !(cap >= 0.0f))
+        cap = F_INF;
+    LaneStack<BS, SSZ> st;
+    for (int pass = 0; pass < (SEEDED ? 2 : 1); ++pass) {
+        st.init(stk_col);
+        const float cap2 = cap * cap;
+        uint32_t node = 0;
+        const uint32_t cap_iters = 2 * t.n_leaves + 64;
+        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+            const NodeRec nr = load_node(t.tnode, node);
+            const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
+            const float b2 = best * best;
+            bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
+            if (STATS) {
+                cnt.node += 1;
+                cnt.box += 2;
+            }
+            if (hL || hR) {
+                if (STATS)
+                    cnt.cone += 1;
+                const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
+                // query.cuh:366-367
+                hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
+                hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
+            }
+            uint32_t lf, lc, rf, rc;
+            const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
+            if (hL && lLeaf)
+                for (uint32_t k = 0; k < lc; ++k)
+                    sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
+            if (hR && rLeaf)
+                for (uint32_t k = 0; k < rc; ++k)
+                    sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
+            const float nb2 = best * best;
+            const bool gL = hL && !lLeaf && Lmd <= nb2;
+            const bool gR = hR && !rLeaf && Rmd <= nb2;
+            if (gL && gR) {
+                const bool lfirst = Lmd <= Rmd;
+                st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
+                node = lfirst ? nr.lref : nr.rref;
+                continue;
+            }
+            if (gL || gR) {
+                node = gL ? nr.lref : nr.rref;
+                continue;
+            }
+            bool found = false;
+            while (st.sp > 0) {
+                uint32_t nn;
+                float dd;
+                st.pop(nn, dd);
+                if (dd <= best * best) { // query.cuh:347
+                    node = nn;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found)
+                break;
+        }
+        overflow = overflow || st.overflow;
+        // every candidate within `best` was examined when best <= cap
+        if (cap == F_INF || (best_e != INVALID && best <= cap))
+            break;
+        cap = F_INF; // nothing within the seeded radius: exact fallback
+    }
+}
+
 template <int BS, bool STATS, int DIM, bool SEEDED>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
@@ -317,88 +449,13 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
         return;
     }
     const uint32_t qi = perm ? min(perm[slot], n - 1) : slot;
-    const v3 p = load_point<DIM>(q, qi);
-    float best = F_INF;
-    uint32_t best_e = INVALID;
-    if (t.n_leaves == 1) {
-        const box3 b = ldbox2(t.nbox, 0);
-        const float4 c0 = t.ncone[0], c1 = t.ncone[1];
-        if (cone_valid(c0.w) && overlap_dim<DIM>(c0, c1.x, p, b, box_mindist(b, p)))
-            sil_leaf<DIM>(t, 0, p, flip != 0, best, best_e, cnt);
-    } else {
-        // A seeded radius r0 caps the search: nodes farther than r0 are
-        // skipped.  A silhouette found at distance <= r0 is the global answer,
-        // because every node that could hold one at least as close was
-        // visited; otherwise the search reruns without the cap, keeping what
-        // it found as the initial bound.  With index tie-breaking the result
-        // is bit-identical to the unseeded search.
-        float cap = SEEDED ? seed[qi] : F_INF;
-        if (!(cap >= 0.0f))
-            cap = F_INF;
-        LaneStack<BS> st;
-        for (int pass = 0; pass < (SEEDED ? 2 : 1); ++pass) {
-            st.init(stk + threadIdx.x);
-            const float cap2 = cap * cap;
-            uint32_t node = 0;
-            const uint32_t cap_iters = 2 * t.n_leaves + 64;
-            for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
-                const NodeRec nr = load_node(t.tnode, node);
-                const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
-                const float b2 = best * best;
-                bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
-                if (STATS) {
-                    cnt.node += 1;
-                    cnt.box += 2;
-                }
-                if (hL || hR) {
-                    if (STATS)
-                        cnt.cone += 1;
-                    const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
-                    // query.cuh:366-367
-                    hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
-                    hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
-                }
-                const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
-                if (hL && lLeaf)
-                    sil_leaf<DIM>(t, nr.lref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
-                if (hR && rLeaf)
-                    sil_leaf<DIM>(t, nr.rref & ~LEAF_BIT, p, flip != 0, best, best_e, cnt);
-                const float nb2 = best * best;
-                const bool gL = hL && !lLeaf && Lmd <= nb2;
-                const bool gR = hR && !rLeaf && Rmd <= nb2;
-                if (gL && gR) {
-                    const bool lfirst = Lmd <= Rmd;
-                    st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
-                    node = lfirst ? nr.lref : nr.rref;
-                    continue;
-                }
-                if (gL || gR) {
-                    node = gL ? nr.lref : nr.rref;
-                    continue;
-                }
-                bool found = false;
-                while (st.sp > 0) {
-                    uint32_t nn;
-                    float dd;
-                    st.pop(nn, dd);
-                    if (dd <= best * best) { // query.cuh:347
-                        node = nn;
-                        found = true;
-                        break;
-                    }
-                }
-                if (!found)
-                    break;
-            }
-            if (st.overflow)
-                if (err)
-                atomicOr(err, 1u);
-            // every candidate within `best` was examined when best <= cap
-            if (cap == F_INF || (best_e != INVALID && best <= cap))
-                break;
-            cap = F_INF; // nothing within the seeded radius: exact fallback
-        }
-    }
+    float best;
+    uint32_t best_e;
+    bool ovf = false;
+    sil_query<BS, SS, STATS, DIM, SEEDED>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
+                                          stk + threadIdx.x, best, best_e, cnt, ovf);
+    if (ovf && err)
+        atomicOr(err, 1u);
     if (out_e)
         out_e[qi] = best_e;
     out_d[qi] = best;
@@ -457,7 +514,7 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
     if (t.n_leaves == 1) {
         leaf(0);
     } else {
-        LaneStack<BS> st;
+        LaneStack<BS, SS> st;
         st.init(stk + threadIdx.x);
         uint32_t node = 0;
         const uint32_t cap_iters = 2 * t.n_leaves + 64;
@@ -470,15 +527,24 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
             }
             bool Lh = box_ray(nr.L, ro, inv, max_dist, Ld) && Ld <= tbest;
             bool Rh = box_ray(nr.R, ro, inv, max_dist, Rd) && Rd <= tbest;
-            const bool lLeaf = (nr.lref & LEAF_BIT) != 0, rLeaf = (nr.rref & LEAF_BIT) != 0;
+            uint32_t lf, lc, rf, rc;
+            const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
             // nearer child first, as query.cuh:136-149
             const bool rfirst = Lh && Rh && Rd < Ld;
             if (rfirst) {
-                if (Rh && rLeaf) leaf(nr.rref & ~LEAF_BIT);
-                if (Lh && lLeaf && Ld <= tbest) leaf(nr.lref & ~LEAF_BIT);
+                if (Rh && rLeaf)
+                    for (uint32_t k = 0; k < rc; ++k)
+                        leaf(rf + k);
+                if (Lh && lLeaf && Ld <= tbest)
+                    for (uint32_t k = 0; k < lc; ++k)
+                        leaf(lf + k);
             } else {
-                if (Lh && lLeaf) leaf(nr.lref & ~LEAF_BIT);
-                if (Rh && rLeaf && Rd <= tbest) leaf(nr.rref & ~LEAF_BIT);
+                if (Lh && lLeaf)
+                    for (uint32_t k = 0; k < lc; ++k)
+                        leaf(lf + k);
+                if (Rh && rLeaf && Rd <= tbest)
+                    for (uint32_t k = 0; k < rc; ++k)
+                        leaf(rf + k);
             }
             if (any_hit && found)
                 break;
@@ -530,6 +596,111 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
     }
     if (out_idx)
         out_idx[qi] = bidx;
+}
+
+// ------------------------------------------------------------ batches
+// One persistent launch over the queries of every mesh of a batch, given as
+// one concatenated query array; the queries of each mesh are cut into chunks of `qpw` (a multiple of 64), and chunks are handed out in
+// schedule order (meshes with deeper trees first) through one atomic counter.
+// Queries of mesh m are [qoff[m], qoff[m + 1]).
+// Each wave takes a chunk, answers it 64 queries at a time against that
+// mesh's tree, and takes the next, until the counter passes the last chunk:
+// every wave reaches that exit.
+struct BatchTable {
+    const TreeView* views;  // per mesh
+    const uint32_t* qoff;   // per mesh, n_meshes + 1 entries
+    const uint32_t* sched;  // schedule entry -> mesh
+    const uint32_t* cstart; // schedule entry -> first chunk (n_sched + 1 entries)
+    const uint32_t* perm;   // optional visiting order of all queries (null: input order)
+    uint32_t* counter;
+    uint32_t n_sched, total_chunks, qpw;
+};
+
+template <int BS, int SSZ, int KIND>
+__global__ __launch_bounds__(BS) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
+                                              uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
+                                              uint32_t* __restrict__ err)
+{
+    __shared__ uint2 stk[SSZ * BS];
+    const uint32_t lane = threadIdx.x & 63;
+    bool ovf = false;
+    Counts cnt;
+    for (uint32_t guard = 0; guard < bt.total_chunks + 1; ++guard) {
+        uint32_t c = 0;
+        if (lane == 0)
+            c = atomicAdd(bt.counter, 1u);
+        c = __shfl(c, 0, 64);
+        if (c >= bt.total_chunks)
+            break;
+        uint32_t lo = 0, hi = bt.n_sched; // last entry with cstart <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bt.cstart[mid] <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t m = bt.sched[lo];
+        const TreeView tv = bt.views[m];
+        const uint32_t q0 = bt.qoff[m] + (c - bt.cstart[lo]) * bt.qpw;
+        const uint32_t q1 = min(q0 + bt.qpw, bt.qoff[m + 1]);
+        for (uint32_t base = q0; base < q1; base += 64) {
+            const uint32_t slot = base + lane;
+            if (slot >= q1)
+                continue;
+            const uint32_t qi = bt.perm ? bt.perm[slot] : slot;
+            const v3 p = load_point<3>(q, qi);
+            float best;
+            uint32_t bi;
+            if (KIND == 0) {
+                cp_query<BS, SSZ, false, 3>(tv, p, stk + threadIdx.x, best, bi, cnt, ovf);
+                best = dm::sqrt_(best);
+            } else {
+                sil_query<BS, SSZ, false, 3, false>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
+            }
+            if (out_idx)
+                out_idx[qi] = bi;
+            out_dist[qi] = best;
+        }
+    }
+    if (ovf && err)
+        atomicOr(err, 1u);
+}
+
+// Morton code of each query in its own mesh's root box (batches)
+__global__ __launch_bounds__(256) void k_batch_codes(const TreeView* __restrict__ views,
+                                                     const uint32_t* __restrict__ qmesh, const float* __restrict__ q,
+                                                     uint32_t n, uint32_t* __restrict__ codes,
+                                                     uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const TreeView& t = views[qmesh[i]];
+    const float4 lo = t.nbox[0], hi = t.nbox[1];
+    const v3 p = load_point<3>(q, i);
+    codes[i] = morton3(mk((p.x - lo.x) / (hi.x - lo.x), (p.y - lo.y) / (hi.y - lo.y), (p.z - lo.z) / (hi.z - lo.z)));
+    idx[i] = i;
+}
+
+__global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
+
+// mesh of each query of a batch (last m with qoff[m] <= i)
+__global__ __launch_bounds__(256) void k_query_mesh(const uint32_t* __restrict__ qoff, uint32_t M, uint32_t n,
+                                                    uint32_t* __restrict__ qmesh)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t lo = 0, hi = M;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    qmesh[i] = lo;
 }
 
 // ------------------------------------------------------------ sampling
@@ -796,6 +967,127 @@ hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, u
     SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_sample_on_object<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, V,
                                               F, nF, idx, uv, n, pt));
     return hipGetLastError();
+}
+
+template <int BS, int SSZ, int KIND>
+static hipError_t launch_batch_cfg(const BatchTable& bt, const float* q, int flip, uint32_t* out_idx,
+                                   float* out_dist, uint32_t n_cu, hipStream_t st)
+{
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<BS, SSZ, KIND>, BS, 0);
+    if (e != hipSuccess)
+        return e;
+    const uint32_t waves = n_cu * (uint32_t)std::max(per_cu, 1) * (BS / 64);
+    // no more waves than chunks
+    const uint32_t grid = std::max(1u, std::min(n_cu * (uint32_t)std::max(per_cu, 1),
+                                                (std::min(waves, bt.total_chunks) + BS / 64 - 1) / (BS / 64)));
+    hipLaunchKernelGGL((k_batch<BS, SSZ, KIND>), dim3(grid), dim3(BS), 0, st, bt, q, flip, out_idx, out_dist,
+                       traversal_error_word());
+    return hipGetLastError();
+}
+
+template <int BS, int KIND>
+static hipError_t launch_batch_bs(const BatchTable& bt, uint32_t ssz, const float* q, int flip, uint32_t* out_idx,
+                                  float* out_dist, uint32_t n_cu, hipStream_t st)
+{
+    switch (ssz) {
+    case 24: return launch_batch_cfg<BS, 24, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
+    case 32: return launch_batch_cfg<BS, 32, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
+    default: return launch_batch_cfg<BS, 16, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
+    }
+}
+
+template <int KIND>
+static hipError_t launch_batch_kind(const BatchTable& bt, uint32_t bs, uint32_t ssz, const float* q, int flip,
+                                    uint32_t* out_idx, float* out_dist, uint32_t n_cu, hipStream_t st)
+{
+    switch (bs) {
+    case 64: return launch_batch_bs<64, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
+    case 128: return launch_batch_bs<128, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
+    case 512: return launch_batch_bs<512, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
+    default: return launch_batch_bs<256, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
+    }
+}
+
+hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
+                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st)
+{
+    const uint32_t M = s->n_meshes;
+    BatchState& b = s->batch;
+    const uint32_t n = q_off_host[M];
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t qpw = cfg.queries_per_wave;
+    // schedule: meshes with more leaves (deeper trees, costlier queries) first
+    std::vector<uint32_t> order(M);
+    for (uint32_t m = 0; m < M; ++m)
+        order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+        return s->mesh_off[a + 1] - s->mesh_off[a] > s->mesh_off[c + 1] - s->mesh_off[c];
+    });
+    // host table: qoff (M + 1) | sched (M) | cstart (M + 1)
+    b.host_tab.assign(3 * (size_t)M + 2, 0);
+    uint32_t* h_qoff = b.host_tab.data();
+    uint32_t* h_sched = h_qoff + M + 1;
+    uint32_t* h_cstart = h_sched + M;
+    for (uint32_t m = 0; m <= M; ++m)
+        h_qoff[m] = q_off_host[m];
+    uint32_t ns = 0, total = 0;
+    for (uint32_t k = 0; k < M; ++k) {
+        const uint32_t m = order[k], cnt = q_off_host[m + 1] - q_off_host[m];
+        if (cnt == 0)
+            continue;
+        h_sched[ns] = m;
+        h_cstart[ns] = total;
+        total += (cnt + qpw - 1) / qpw;
+        ++ns;
+    }
+    h_cstart[ns] = total;
+    hipError_t e;
+    if ((e = b.tab.ensure(sizeof(uint32_t) * b.host_tab.size())) != hipSuccess ||
+        (e = b.counter.ensure(64)) != hipSuccess)
+        return e;
+    uint32_t* d_tab = b.tab.as<uint32_t>();
+    if ((e = hipMemcpyAsync(d_tab, b.host_tab.data(), sizeof(uint32_t) * b.host_tab.size(), hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+        return e;
+    BatchTable bt;
+    bt.views = b.views.as<TreeView>();
+    bt.qoff = d_tab;
+    bt.sched = d_tab + M + 1;
+    bt.cstart = d_tab + 2 * M + 1;
+    bt.perm = nullptr;
+    bt.counter = b.counter.as<uint32_t>();
+    bt.n_sched = ns;
+    bt.total_chunks = total;
+    bt.qpw = qpw;
+    if (cfg.coherent) {
+        QueryScratch& w = b.order;
+        if ((e = w.codes.ensure(sizeof(uint32_t) * n)) != hipSuccess ||
+            (e = w.idx.ensure(sizeof(uint32_t) * n)) != hipSuccess ||
+            (e = w.codes_alt.ensure(sizeof(uint32_t) * n)) != hipSuccess ||
+            (e = w.idx_alt.ensure(sizeof(uint32_t) * n)) != hipSuccess ||
+            (e = b.qmesh.ensure(sizeof(uint32_t) * n)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_query_mesh, dim3(nblocks(n, 256)), dim3(256), 0, st, d_tab, M, n,
+                           b.qmesh.as<uint32_t>());
+        hipLaunchKernelGGL(k_batch_codes, dim3(nblocks(n, 256)), dim3(256), 0, st, bt.views, b.qmesh.as<uint32_t>(),
+                           q, n, w.codes.as<uint32_t>(), w.idx.as<uint32_t>());
+        int mesh_bits = 0;
+        while (mesh_bits < 32 && ((M - 1) >> mesh_bits) != 0)
+            ++mesh_bits;
+        bool in_alt = false;
+        if ((e = radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
+                                  w.idx_alt.as<uint32_t>(), n, 30, w.hist, st, &in_alt,
+                                  M > 1 ? b.qmesh.as<uint32_t>() : nullptr, M > 1 ? mesh_bits : 0)) != hipSuccess)
+            return e;
+        bt.perm = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
+    }
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, bt.counter);
+    const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
+    if (kind == 0)
+        return launch_batch_kind<0>(bt, cfg.block_size, cfg.stack_depth, q, flip, out_idx, out_dist, n_cu, st);
+    return launch_batch_kind<1>(bt, cfg.block_size, cfg.stack_depth, q, flip, out_idx, out_dist, n_cu, st);
 }
 
 } // namespace snch

@@ -27,7 +27,11 @@ constexpr int TILE = ST * SI;    // 4096 keys
 constexpr int RADIX = 256;
 constexpr int WAVES = ST / 64;
 
-__global__ __launch_bounds__(ST) void k_upsweep(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+// SEG: the digit comes from seg[value] (a segment id per value) instead of
+// the key, which sorts stably by segment after the key passes
+template <bool SEG>
+__global__ __launch_bounds__(ST) void k_upsweep(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                 const uint32_t* __restrict__ seg, uint32_t n, int shift,
                                                  uint32_t ntiles, uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[RADIX];
@@ -39,7 +43,7 @@ __global__ __launch_bounds__(ST) void k_upsweep(const uint32_t* __restrict__ key
     for (int i = 0; i < SI; ++i) {
         uint32_t pos = base + i * ST + tid;
         if (pos < n)
-            atomicAdd(&h[(keys[pos] >> shift) & 0xFFu], 1u);
+            atomicAdd(&h[((SEG ? seg[vals[pos]] : keys[pos]) >> shift) & 0xFFu], 1u);
     }
     __syncthreads();
     hist[tid * ntiles + blockIdx.x] = h[tid];
@@ -152,10 +156,11 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* __restrict__ a, u
         *reinterpret_cast<uint4*>(a + idx) = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
 }
 
+template <bool SEG>
 __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                   uint32_t n, int shift, uint32_t ntiles,
-                                                   const uint32_t* __restrict__ offs)
+                                                   const uint32_t* __restrict__ seg, uint32_t* __restrict__ kout,
+                                                   uint32_t* __restrict__ vout, uint32_t n, int shift,
+                                                   uint32_t ntiles, const uint32_t* __restrict__ offs)
 {
     __shared__ uint32_t wcnt[WAVES][RADIX];
     __shared__ uint32_t toff[RADIX];
@@ -176,10 +181,14 @@ __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ k
         k[i] = valid ? kin[pos] : 0u;
         v[i] = valid ? vin[pos] : 0u;
     }
+    uint32_t dsrc[SI];
+#pragma unroll
+    for (int i = 0; i < SI; ++i)
+        dsrc[i] = SEG ? (base + i * 64 + lane < n ? seg[v[i]] : 0u) : k[i];
 #pragma unroll
     for (int i = 0; i < SI; ++i) {
         bool valid = base + i * 64 + lane < n;
-        uint32_t d = (k[i] >> shift) & 0xFFu;
+        uint32_t d = (dsrc[i] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -214,7 +223,7 @@ __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ k
     for (int i = 0; i < SI; ++i) {
         bool valid = base + i * 64 + lane < n;
         if (valid) {
-            uint32_t d = (k[i] >> shift) & 0xFFu;
+            uint32_t d = (dsrc[i] >> shift) & 0xFFu;
             uint32_t dst = wcnt[w][d] + rank[i];
             if (dst < n) { // always true when histogram and ranks agree
                 kout[dst] = k[i];
@@ -227,7 +236,8 @@ __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ k
 } // namespace
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
-                            int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt)
+                            int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt, const uint32_t* seg,
+                            int seg_bits)
 {
     *result_in_alt = false;
     if (n <= 1)
@@ -246,8 +256,14 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     uint32_t* h = hist.as<uint32_t>();
     uint32_t* sums = h + hlen;
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
-    for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_upsweep, dim3(ntiles), dim3(ST), 0, st, ki, n, shift, ntiles, h);
+    const int key_passes = (bits + 7) / 8, seg_passes = seg ? (seg_bits + 7) / 8 : 0;
+    for (int pass = 0; pass < key_passes + seg_passes; ++pass) {
+        const bool sp = pass >= key_passes;
+        const int shift = 8 * (sp ? pass - key_passes : pass);
+        if (sp)
+            hipLaunchKernelGGL(k_upsweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, n, shift, ntiles, h);
+        else
+            hipLaunchKernelGGL(k_upsweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, n, shift, ntiles, h);
         if (chunked) {
             hipLaunchKernelGGL(k_chunk_sums, dim3(nchunks), dim3(1024), 0, st, h, hlen, sums);
             hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, sums, (nchunks + 3) & ~3u);
@@ -255,7 +271,12 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
         } else {
             hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, h, hlen);
         }
-        hipLaunchKernelGGL(k_downsweep, dim3(ntiles), dim3(ST), 0, st, ki, vi, ko, vo, n, shift, ntiles, h);
+        if (sp)
+            hipLaunchKernelGGL(k_downsweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift, ntiles,
+                               h);
+        else
+            hipLaunchKernelGGL(k_downsweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
+                               ntiles, h);
         uint32_t* t = ki;
         ki = ko;
         ko = t;
