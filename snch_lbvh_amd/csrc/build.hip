@@ -593,9 +593,13 @@ __device__ __forceinline__ void refit_write(const MeshIds& id, uint32_t p, uint2
     tnode[4 * r + 3] = make_float4(__uint_as_float(id.ref(ch.x)), __uint_as_float(id.ref(ch.y)),
                                    __uint_as_float(run_of(id, ch.x, range, leaf_size)),
                                    __uint_as_float(run_of(id, ch.y, range, leaf_size)));
-    tcone[3 * r + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
-    tcone[3 * r + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
-    tcone[3 * r + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+    float ls, lcs, rs, rcs;
+    dm::sincosf_(lc.half_angle, &ls, &lcs);
+    dm::sincosf_(rc.half_angle, &rs, &rcs);
+    tcone[4 * r + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
+    tcone[4 * r + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
+    tcone[4 * r + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+    tcone[4 * r + 3] = make_float4(ls, lcs, rs, rcs);
 }
 
 // box + cone of one node in LDS, structure of arrays over 2 * RT slots (slot
@@ -793,7 +797,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 1)));
     uint32_t* cuts = w.top.as<uint32_t>();
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
-    SNCH_TRY(s->tcone.ensure(sizeof(float4) * 3 * (NI > 0 ? NI : 1)));
+    SNCH_TRY(s->tcone.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
     SNCH_TRY(s->tedge.ensure(sizeof(float4) * 4 * (E > 0 ? E : 1)));
     SNCH_TRY(s->nbox.ensure(sizeof(float4) * 2 * NN));

@@ -196,7 +196,9 @@ snch::TreeView snch_scene::view(uint32_t m) const
     TreeView t{};
     t.n_leaves = n;
     t.tnode = tnode.as<float4>() + 4 * (size_t)(off - m);
-    t.tcone = tcone.as<float4>() + 3 * (size_t)(off - m);
+    t.tcone = tcone.as<float4>() + 4 * (size_t)(off - m);
+    // the trig-free cone test is conservative only with the enclosing radius
+    t.flags = (dim == 3 && !(build_flags & SNCH_BUILD_REFERENCE_RADIUS)) ? TREE_FAST_CONE : 0u;
     t.tleaf = tleaf.as<float4>() + 3 * (size_t)off;
     t.tedge = tedge.as<float4>() + 4 * (size_t)eoff;
     t.nbox = nbox.as<float4>() + 2 * (size_t)(2 * off - m);
@@ -440,6 +442,7 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
         return hip_fail(e, "snch_scene_build");
     if (es != hipSuccess)
         return hip_fail(es, "snch_scene_build: synchronize");
+    s->build_flags = flags;
     {
         std::vector<TreeView> views(s->n_meshes);
         for (uint32_t m = 0; m < s->n_meshes; ++m)

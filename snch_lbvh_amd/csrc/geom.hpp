@@ -191,6 +191,47 @@ SNCH_HD bool cone_overlap(v3 axis, float half_angle, float radius, v3 o, const b
     return sum >= F_PI_2 ? true : inrange(F_PI_2, mn, mx);
 }
 
+// The same test without inverse trigonometry, for the traversal.  With
+// C = cos(angle(axis, view direction)) and vha the view cone's half angle
+// (asin(r / l), or atan2(d, s) when the view point is inside the bounding
+// sphere), the reference's condition |acos(C) - pi/2| <= ha + vha reads
+// |C| <= sin(ha + vha), and ha + vha >= pi/2 reads cos(ha + vha) <= 0; both
+// expand through the precomputed sin(ha), cos(ha) and the sine and cosine of
+// vha, which are ratios of lengths.  A slack of 1e-5 (l + r) on the
+// permissive side keeps every node the reference test keeps despite either
+// test's rounding; a node kept in excess only costs a visit, never changes a
+// result.  Conservative only with the enclosing cone radius.
+SNCH_HD bool cone_overlap_fast(v3 axis, float half_angle, float sin_ha, float cos_ha, float radius, v3 o,
+                               const box3& b, float dist_to_box)
+{
+    if (half_angle >= F_PI_2 || dist_to_box < F_EPS)
+        return true;
+    const v3 c = box_centroid(b);
+    const v3 v = mk(c.x - o.x, c.y - o.y, c.z - o.z);
+    const float l2 = dot3(v, v), l = dm::sqrt_(l2);
+    const float av = dm::fabs_(dot3(axis, v)); // l |C|
+    const float tol = 1e-5f * (l + radius);
+    if (l > radius) {
+        const float w = dm::sqrt_(fmx(l2 - radius * radius, 0.0f)); // l cos(vha); l sin(vha) = radius
+        if (cos_ha * w <= sin_ha * radius + tol)
+            return true; // ha + vha >= pi/2
+        return av <= sin_ha * w + cos_ha * radius + tol;
+    }
+    if (!(l > 0.0f))
+        return true;
+    const float il = 1.0f / l;
+    const v3 vn = mk(v.x * il, v.y * il, v.z * il);
+    const v3 e = mk(b.hi.x - c.x, b.hi.y - c.y, b.hi.z - c.z);
+    const float s = l - dot3(e, cabs3(vn));
+    if (s <= 0.0f)
+        return true;
+    const float d = project_to_plane(vn, e); // tan(vha) = d / s
+    const float h = dm::sqrt_(d * d + s * s);
+    if (cos_ha * s <= sin_ha * d + tol)
+        return true;
+    return av * il * h <= sin_ha * s + cos_ha * d + tol;
+}
+
 // Rodrigues rotation (cone.cuh:288-302).  When the axes are (anti)parallel the
 // reference divides by a zero-length cross product; here the rotation axis
 // falls back to a vector perpendicular to u.

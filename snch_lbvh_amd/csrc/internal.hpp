@@ -26,10 +26,11 @@ namespace snch {
 //             built with leaf size k > 1 and internal child c covers at most
 //             k leaves, (count - 1) << 27 | first: the traversal queries then
 //             treat c as one leaf holding sorted leaves [first, first + count).
-//   tcone[i]  3 x float4 = 48 B per internal node: the children's normal
+//   tcone[i]  4 x float4 = 64 B per internal node: the children's normal
 //             cones, read only by the silhouette query:
 //               [0] L.axis L.half_angle  [1] R.axis R.half_angle
 //               [2] L.radius R.radius 0 0
+//               [3] sin, cos of L.half_angle, sin, cos of R.half_angle
 //   tleaf[j]  3 x float4 = 48 B per leaf, in Morton (sorted) order:
 //               [k] vertex k xyz, w = (original triangle, first owned edge,
 //               owned edge count) for k = 0, 1, 2
@@ -55,9 +56,11 @@ struct MeshDesc {
     uint32_t off, n, eoff, nb;
 };
 
+constexpr uint32_t TREE_FAST_CONE = 1u; // TreeView::flags: cone_overlap_fast may replace cone_overlap
+
 struct TreeView {
     uint32_t n_leaves;
-    uint32_t pad_;
+    uint32_t flags;
     const float4* __restrict__ tnode;
     const float4* __restrict__ tcone;
     const float4* __restrict__ tleaf;

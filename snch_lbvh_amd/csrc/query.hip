@@ -383,10 +383,18 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
             if (hL || hR) {
                 if (STATS)
                     cnt.cone += 1;
-                const float4 CL = t.tcone[3 * node], CR = t.tcone[3 * node + 1], Cr = t.tcone[3 * node + 2];
+                const float4 CL = t.tcone[4 * node], CR = t.tcone[4 * node + 1], Cr = t.tcone[4 * node + 2];
                 // query.cuh:366-367
-                hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
-                hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
+                if (DIM == 3 && (t.flags & TREE_FAST_CONE)) {
+                    const float4 SC = t.tcone[4 * node + 3];
+                    hL = hL && cone_valid(CL.w) &&
+                         cone_overlap_fast(mk(CL.x, CL.y, CL.z), CL.w, SC.x, SC.y, Cr.x, p, nr.L, Lmd);
+                    hR = hR && cone_valid(CR.w) &&
+                         cone_overlap_fast(mk(CR.x, CR.y, CR.z), CR.w, SC.z, SC.w, Cr.y, p, nr.R, Rmd);
+                } else {
+                    hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
+                    hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
+                }
             }
             uint32_t lf, lc, rf, rc;
             const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
