@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--build-reps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--input-order", action="store_true", help="also time the kernel over unordered queries")
+    ap.add_argument("--workload", default="silhouette", choices=["silhouette", "small", "batched", "wost", "mlp"],
+                    help="silhouette: the headline (1M-triangle torus, 2^22 queries); small: 20,480-triangle "
+                         "icosphere, 65,536 closest-point queries; batched: 64 uneven meshes in one launch; "
+                         "wost: walk on stars, 2^20 walks; mlp: learned-bound MLP training and inference")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="short run for hardware-counter passes: 2 silhouette launches + a 256 MiB copy")
     ap.add_argument("--train-points", type=int, default=1 << 18)
@@ -113,12 +117,246 @@ def pmc_mode(sc, q, n, S, torch):
     print(json.dumps({"pmc_mode": True, "silhouette_launches": 2, "copy_bytes": src.numel() * 4}))
 
 
+def run_timed(torch, fn, steps, warmup, stream=None):
+    """Wall time and HIP-event time per call of `steps` calls after `warmup`
+    calls; events on the stream the kernels are launched on."""
+    stream = stream or torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps, ev0.elapsed_time(ev1) / steps
+
+
+def base_line(metric, value, unit, args, ms_per_step, config, data, dtype="f32", higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": data, "config": config}
+
+
+def omp_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+
+
+def workload_small(args):
+    """Plain method, small: 20,480-triangle noised icosphere, LBVH build, then
+    65,536 closest-point queries uniform in [-1.5, 1.5]^3."""
+    import torch
+
+    import oracle
+    import snch_lbvh_amd as S
+    from snch_lbvh_amd import workloads as W
+
+    V, F, q_np = W.config_small(1 << 16)
+    sc = S.Scene(V, F)
+    ms = []
+    for _ in range(args.build_reps + 1):
+        sc.build_bvh()
+        ms.append(sc.build_ms())
+    build_ms = float(np.median(ms[1:]))
+    q = torch.from_numpy(q_np).cuda()
+    n = q.shape[0]
+    idx = torch.empty(n, dtype=torch.int32, device=q.device)
+    dist = torch.empty(n, dtype=torch.float32, device=q.device)
+    opts = S.QueryOpts(0, 0, None, None)
+
+    def step():
+        S.check(S.lib().snch_closest_point_ex(sc._h, S._ptr(q), n, S._ptr(idx), S._ptr(dist), ctypes.byref(opts),
+                                              S._stream()))
+
+    wall, ev = run_timed(torch, step, args.steps, args.warmup)
+    # end to end in the reference's protocol: allocate, copy in, query, copy out
+    def e2e():
+        qq = torch.from_numpy(q_np).cuda()
+        i2, d2 = sc.closest_point(qq)
+        return i2.cpu(), d2.cpu()
+
+    e2e_wall, _ = run_timed(torch, e2e, 8, 1)
+    tree = oracle.build_tree(V, F)
+    th = omp_threads()
+    t0 = time.perf_counter()
+    oi, od = oracle.closest_point(tree, q_np, nthreads=th)
+    cpu_s = time.perf_counter() - t0
+    agree = float((dist.cpu().numpy() == od).mean())
+    line = base_line("closest-point queries/sec (small: 20,480-triangle icosphere)", n / wall, "queries/s", args,
+                     wall * 1e3, {"workload": "icosphere level 5 (20,480 triangles, N(0,0.02) radial noise), 65,536 "
+                                              "closest-point queries uniform in [-1.5,1.5]^3", "triangles": len(F),
+                                  "queries": n},
+                     "synthetic (seeded icosphere, uniform queries)")
+    line.update({"kernel_ms": ev, "end_to_end_qps": n / e2e_wall, "build_ms": build_ms, "builds_per_s": 1e3 / build_ms,
+                 "distance_agreement_with_oracle": agree,
+                 "cpu_baseline": {"value": n / cpu_s, "unit": "queries/s", "cores": th, "kind": "port",
+                                  "sample": f"all {n} queries, oracle/snch_oracle.c with OpenMP"}})
+    print(json.dumps(line))
+
+
+def workload_batched(args):
+    """64 noised spheres / cut tori, triangles log-uniform in [1e3, 5e5],
+    queries log-uniform in [2^12, 2^20] per mesh: one build and one
+    persistent closest-silhouette launch for all, autotuned; against one
+    scene and one launch per mesh."""
+    import torch
+
+    import snch_lbvh_amd as S
+    from snch_lbvh_amd import workloads as W
+
+    meshes, q_np, q_off = W.batched_meshes()
+    t0 = time.perf_counter()
+    b = S.BatchScene(meshes)
+    t_create = time.perf_counter() - t0
+    ms = []
+    for _ in range(args.build_reps + 1):
+        b.build_bvh()
+        ms.append(b.build_ms())
+    build_ms = float(np.median(ms[1:]))
+    q = torch.from_numpy(q_np).cuda()
+    n = q.shape[0]
+    sil = torch.empty(n, dtype=torch.int32, device=q.device)
+    dist = torch.empty(n, dtype=torch.float32, device=q.device)
+    qo = np.ascontiguousarray(q_off, np.uint32)
+
+    def launch(tune):
+        opts = b._opts(tune, 0)
+        S.check(S.lib().snch_batch_closest_silhouette(b._h, S._ptr(q), qo.ctypes.data, 0, S._ptr(sil), S._ptr(dist),
+                                                      ctypes.byref(opts), S._stream()))
+
+    wall0, ev0 = run_timed(torch, lambda: launch(None), max(2, args.steps // 3), 1)
+    t0 = time.perf_counter()
+    tune = b.autotune(q, q_off, kind="silhouette", sample_per_mesh=4096)
+    t_tune = time.perf_counter() - t0
+    wall, ev = run_timed(torch, lambda: launch(tune), args.steps, args.warmup)
+    ref = (sil.clone(), dist.clone())
+    # one scene and one launch per mesh, the way the reference is used
+    singles = [S.Scene(V, F).build_bvh() for V, F in meshes]
+    outs = []
+
+    def per_mesh():
+        outs.clear()
+        for m, sc in enumerate(singles):
+            a, z = int(q_off[m]), int(q_off[m + 1])
+            outs.append(sc.closest_silhouette(q[a:z]))
+
+    wall_pm, _ = run_timed(torch, per_mesh, max(2, args.steps // 3), 1)
+    same = all(torch.equal(o[1], ref[1][int(q_off[m]):int(q_off[m + 1])]) for m, o in enumerate(outs))
+    line = base_line("batched closest-silhouette queries/sec (64 uneven meshes, one persistent launch)", n / wall,
+                     "queries/s", args, wall * 1e3,
+                     {"workload": f"64 meshes (spheres / cut tori), {sum(len(f) for _, f in meshes)} triangles, "
+                                  f"{n} queries; autotuned", "triangles": int(sum(len(f) for _, f in meshes)),
+                      "queries": int(n)}, "synthetic (seeded meshes and queries)")
+    line.update({"kernel_ms": ev, "default_config_qps": n / wall0, "tuned": tune, "autotune_s": t_tune,
+                 "per_mesh_launches_qps": n / wall_pm, "per_mesh_results_identical": bool(same),
+                 "batch_build_ms": build_ms, "batch_builds_per_s": 1e3 / build_ms, "scene_create_s": t_create})
+    print(json.dumps(line))
+
+
+def workload_wost(args):
+    """Walk on stars on the 1M-triangle torus: Dirichlet (g = x y) where the
+    triangle centroid has z >= 0, zero Neumann flux elsewhere; 2^20 walks from
+    uniform interior points, eps 1e-3, at most 64 steps, step loop replayed
+    as a captured hipGraph."""
+    import torch
+
+    import snch_lbvh_amd as S
+    from snch_lbvh_amd import workloads as W
+
+    V, F = W.cut_torus(args.nu, args.nv, seed=3)
+    FD, FN = W.split_dirichlet_neumann(V, F)
+    sd = S.Scene(V, FD).build_bvh()
+    sn = S.Scene(V, FN).build_bvh()
+    n = 1 << 20
+    x0 = torch.from_numpy(W.torus_interior_points(n, seed=8)).cuda()
+    ws = S.WoSt(sd, sn, max_walks=n)
+    out = torch.empty(n, dtype=torch.float32, device=x0.device)
+    res = {}
+
+    def solve(graph):
+        res["r"] = ws.solve(x0, max_steps=64, eps=1e-3, seed=11, use_graph=graph, out=out)
+
+    wall_e, _ = run_timed(torch, lambda: solve(False), 1, 1)
+    eager = out.clone()
+    wall, ev = run_timed(torch, lambda: solve(True), max(2, args.steps // 3), 1)
+    same = bool(torch.equal(out, eager))
+    line = base_line("walk-on-stars walks/sec (1M-triangle torus, 2^20 walks, <= 64 steps)", n / wall, "walks/s",
+                     args, wall * 1e3,
+                     {"workload": "WoSt Laplace, cut torus 1,048,576 triangles split Dirichlet (z >= 0) / Neumann, "
+                                  "2^20 walks, eps 1e-3, max 64 steps, hipGraph", "walks": n,
+                      "dirichlet_triangles": len(FD), "neumann_triangles": len(FN)},
+                     "synthetic (seeded torus and starts)")
+    line.update({"kernel_ms": ev, "eager_walks_per_s": n / wall_e, "graph_equals_eager": same,
+                 "live_after": int(res["r"][1]), "mean_estimate": float(out.mean())})
+    print(json.dumps(line))
+
+
+def workload_mlp(args):
+    """Learned distance bound: 2^24 training points (half uniform in the
+    padded AABB, half near-surface with N(0, 0.05) offsets) labelled by the
+    exact GPU closest point on the 1M-triangle torus; training epochs and
+    2^22-point inference on BF16 MFMA; error against the FP64 brute force."""
+    import torch
+
+    import oracle
+    import snch_lbvh_amd as S
+    from snch_lbvh_amd import workloads as W
+
+    V, F = W.cut_torus(args.nu, args.nv, seed=3)
+    lo, hi = W.padded_box(V)
+    sc = S.Scene(V, F).build_bvh()
+    pts = torch.from_numpy(W.mlp_training_points(V, F, 1 << 24, seed=9)).cuda()
+    _, lab = sc.closest_point(pts)
+    mlp = S.LearnedBound(lo.astype(np.float32), hi.astype(np.float32), sigma=4.0, seed=1)
+    epochs = 3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = mlp.train(pts, lab, batch=16384, epochs=epochs, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    qi = torch.from_numpy(W.uniform_points(1 << 22, lo, hi, seed=4)).cuda()
+    pred = torch.empty(qi.shape[0], dtype=torch.float32, device=qi.device)
+    wall, ev = run_timed(torch, lambda: mlp.predict(qi, out=pred), args.steps, args.warmup)
+    flop_fwd = 2.0 * (64 * 128 + 3 * 128 * 128 + 128)  # per point
+    n_inf = qi.shape[0]
+    infer_tflops = n_inf * flop_fwd / (ev * 1e-3) / 1e12
+    train_tflops = epochs * (1 << 24) * 3 * flop_fwd / t_train / 1e12
+    # error against the FP64 brute force on a sample
+    ns = 512
+    qs = W.uniform_points(ns, lo, hi, seed=77)
+    _, d64 = oracle.bf_closest_point(V, F, qs, nthreads=omp_threads())
+    p = mlp.predict(qs).cpu().numpy()
+    rel = np.abs(p - d64) / np.maximum(d64, 1e-6)
+    line = base_line("learned-bound MLP inference points/sec (BF16 MFMA, 2^22 points)", n_inf / wall, "points/s",
+                     args, wall * 1e3,
+                     {"workload": "3->64 Fourier (sigma 4) -> 4x128 SiLU -> 1 MLP; train 2^24 points labelled by "
+                                  "exact closest point on the 1M-triangle torus; infer 2^22", "train_points": 1 << 24,
+                      "infer_points": n_inf}, "synthetic (seeded torus, sampled points)", dtype="bf16")
+    line.update({"kernel_ms": ev, "train_s": t_train, "train_epochs": epochs,
+                 "train_points_per_s": epochs * (1 << 24) / t_train,
+                 "loss_first_last": [float(losses[0]), float(losses[-1])],
+                 "roofline": {"bound": "mfma", "achieved": infer_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": infer_tflops / 2500.0, "traffic": None, "kernel": "k_mlp_infer",
+                              "train_tflops": train_tflops},
+                 "error_vs_fp64": {"sample": ns, "median_rel": float(np.median(rel)),
+                                   "p90_rel": float(np.quantile(rel, 0.9)), "max_rel": float(rel.max())}})
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(queries shard without communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "silhouette":
+        {"small": workload_small, "batched": workload_batched, "wost": workload_wost,
+         "mlp": workload_mlp}[args.workload](args)
+        return
     import torch
 
     import snch_lbvh_amd as S

@@ -198,6 +198,24 @@ def closest_silhouette(tree, q, flip=False, nthreads=0, counts=False):
     return (edge, dist, cnt) if counts else (edge, dist)
 
 
+def wost(dirichlet, neumann, x0, max_steps=64, eps=1e-3, seed=0, nthreads=0):
+    """Walk on stars, one walk at a time (oracle/snch_oracle.c orc_wost): the
+    library's per-walk arithmetic and random streams over the oracle's
+    queries.  dirichlet / neumann: Tree (neumann may be None)."""
+    L = lib()
+    if not getattr(L, "_wost_declared", False):
+        L.orc_wost.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint32,
+                               ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L._wost_declared = True
+    x0 = _c(x0, F32).reshape(-1, 3)
+    n = x0.shape[0]
+    out = np.empty(n, F32)
+    steps = np.empty(n, U32)
+    L.orc_wost(dirichlet.ref, neumann.ref if neumann is not None else None, _p(x0), n, max_steps, float(eps),
+               int(seed), _p(out), _p(steps), nthreads)
+    return out, steps
+
+
 def ray(tree, o, d, tmax=None, any_hit=False, nthreads=0, counts=False):
     o = _c(o, F32).reshape(-1, 3)
     d = _c(d, F32).reshape(-1, 3)

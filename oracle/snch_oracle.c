@@ -1275,6 +1275,128 @@ static f3 sample_triangle(f3 pa, f3 pb, f3 pc, float u, float v)
     return mk(w * pa.x + u * pb.x + v * pc.x, w * pa.y + u * pb.y + v * pc.y, w * pa.z + u * pb.z + v * pc.z);
 }
 
+/* ------------------------------------------------------------------ */
+/* Walk on stars (Sawhney, Miller, Gkioulekas, Crane, SIGGRAPH 2023) for */
+/* the Laplace equation: g = x y on the Dirichlet scene, zero Neumann    */
+/* flux on the Neumann scene.  The library's wavefront loop              */
+/* (snch_lbvh_amd/csrc/wost.hip) restated one walk at a time with the    */
+/* same per-walk FP32 arithmetic and counter-based random streams; the   */
+/* queries are the reference-order ones above.  No reference            */
+/* counterpart (the reference's WoSt caller lives outside it).           */
+/* ------------------------------------------------------------------ */
+static void set_threads(int nthreads);
+
+static float wost_rnd(uint64_t seed, uint32_t walk, uint32_t step, uint32_t k) /* splitmix64 finaliser */
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (((uint64_t)walk << 24) ^ ((uint64_t)step << 4) ^ k);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((uint32_t)(z >> 40) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+/* Ericson, Real-Time Collision Detection 5.1.5: the closest point itself */
+static f3 closest_point_on_tri(f3 a, f3 b, f3 c, f3 p)
+{
+    f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a), bp, cp;
+    float d1 = dot3(ab, ap), d2 = dot3(ac, ap), d3, d4, d5, d6, vc, vb, va, v, w, den;
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    bp = sub(p, b);
+    d3 = dot3(ab, bp);
+    d4 = dot3(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        v = d1 / (d1 - d3);
+        return mk(a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z);
+    }
+    cp = sub(p, c);
+    d5 = dot3(ab, cp);
+    d6 = dot3(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        w = d2 / (d2 - d6);
+        return mk(a.x + w * ac.x, a.y + w * ac.y, a.z + w * ac.z);
+    }
+    va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return mk(b.x + w * (c.x - b.x), b.y + w * (c.y - b.y), b.z + w * (c.z - b.z));
+    }
+    den = 1.0f / (va + vb + vc);
+    v = vb * den;
+    w = vc * den;
+    return mk(a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w, a.z + ab.z * v + ac.z * w);
+}
+
+static f3 tri_vert(const orc_tree *t, uint32_t tri, int k) { return vget(t->V, t->F[3 * (size_t)tri + k]); }
+
+static float wost_g(const orc_tree *D, uint32_t tri, f3 p)
+{
+    f3 c = closest_point_on_tri(tri_vert(D, tri, 0), tri_vert(D, tri, 1), tri_vert(D, tri, 2), p);
+    return c.x * c.y;
+}
+
+void orc_wost(const orc_tree *D, const orc_tree *N, const float *x0, int64_t n, uint32_t max_steps, float eps,
+              uint64_t seed, float *out, uint32_t *steps_out, int nthreads)
+{
+    int64_t wi;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (wi = 0; wi < n; ++wi) {
+        const uint32_t wk = (uint32_t)wi;
+        f3 p = mk(x0[3 * wi], x0[3 * wi + 1], x0[3 * wi + 2]), nrm = mk(0.0f, 0.0f, 0.0f);
+        int on_bnd = 0;
+        uint32_t steps = 0;
+        float value = 0.0f;
+        for (;;) {
+            uint32_t cidx, sidx, ridx;
+            float dD, dS = INFINITY, R, u1, u2, z, s, sp, cp, rt, uv[2];
+            uint8_t hit = 0;
+            f3 d, o;
+            cp_one(D, p, &cidx, &dD, NULL);
+            if (N) sil_one(N, p, 0, &sidx, &dS, NULL);
+            if (dD < eps) {
+                value = wost_g(D, cidx, p);
+                break;
+            }
+            R = dD < dS ? dD : dS;
+            u1 = wost_rnd(seed, wk, steps, 0);
+            u2 = wost_rnd(seed, wk, steps, 1);
+            z = 1.0f - 2.0f * u1;
+            s = sqrtf(smax(0.0f, 1.0f - z * z));
+            odm_sincosf(6.283185307179586f * u2, &sp, &cp);
+            d = mk(s * cp, s * sp, z);
+            o = p;
+            if (on_bnd) {
+                float dn = d.x * nrm.x + d.y * nrm.y + d.z * nrm.z;
+                if (dn < 0.0f) d = mk(d.x - 2.0f * dn * nrm.x, d.y - 2.0f * dn * nrm.y, d.z - 2.0f * dn * nrm.z);
+                o = mk(p.x + 1e-5f * nrm.x, p.y + 1e-5f * nrm.y, p.z + 1e-5f * nrm.z);
+            }
+            if (N) ray_one(N, o, d, R, 0, &ridx, &rt, uv, &hit, NULL);
+            if (N && hit && rt < R) {
+                f3 a = tri_vert(N, ridx, 0), b = tri_vert(N, ridx, 1), c = tri_vert(N, ridx, 2);
+                f3 nn = normalize3(cross3(sub(b, a), sub(c, a)));
+                if (dot3(nn, d) > 0.0f) nn = mk(-nn.x, -nn.y, -nn.z);
+                p = mk(o.x + rt * d.x, o.y + rt * d.y, o.z + rt * d.z);
+                nrm = nn;
+                on_bnd = 1;
+            } else {
+                p = mk(p.x + R * d.x, p.y + R * d.y, p.z + R * d.z);
+                on_bnd = 0;
+            }
+            ++steps;
+            if (steps >= max_steps) {
+                value = wost_g(D, cidx, p);
+                break;
+            }
+        }
+        out[wi] = value;
+        if (steps_out) steps_out[wi] = steps;
+    }
+}
+
 static void set_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -162,9 +162,9 @@ __global__ void k_star(Walks w, int cur, uint32_t cap, const float4* __restrict_
     // direction uniform on S^2 (z = 1 - 2u, phi = 2 pi u')
     const float u1 = rnd(seed, wk, step, 0), u2 = rnd(seed, wk, step, 1);
     const float z = 1.0f - 2.0f * u1;
-    const float s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float s = dm::sqrt_(fmx(0.0f, 1.0f - z * z));
     float sp, cp;
-    sincosf(6.283185307179586f * u2, &sp, &cp);
+    dm::sincosf_(6.283185307179586f * u2, &sp, &cp); // deterministic: the CPU restatement matches bit for bit
     v3 d = mk(s * cp, s * sp, z);
     v3 o = p;
     if (p4.w > 0.5f) { // on the Neumann boundary: reflect into the domain, nudge off the surface

@@ -17,7 +17,8 @@ import numpy as np
 
 __all__ = [
     "icosphere", "cut_torus", "uv_sphere", "padded_box", "uniform_points", "unit_directions",
-    "log_uniform", "config_small", "config_large",
+    "log_uniform", "config_small", "config_large", "circle_in_square", "polyline_soup", "torus_interior_points",
+    "split_dirichlet_neumann", "surface_points", "mlp_training_points", "batched_meshes",
 ]
 
 
@@ -121,18 +122,18 @@ def uv_sphere(n_tri, sigma=0.02, seed=1):
     ring = np.stack(np.broadcast_arrays(st * np.cos(phi), st * np.sin(phi), ct + 0 * phi), -1).reshape(-1, 3)
     V = np.concatenate([[[0, 0, 1.0]], ring, [[0, 0, -1.0]]])
     rows = nv - 1
+    k = np.arange(nu)
     idx = lambda r_, c_: 1 + r_ * nu + (c_ % nu)  # noqa: E731
-    tris = [[0, idx(0, k), idx(0, k + 1)] for k in range(nu)]
-    for r_ in range(rows - 1):
-        for k in range(nu):
-            a, b, c, d = idx(r_, k), idx(r_ + 1, k), idx(r_ + 1, k + 1), idx(r_, k + 1)
-            tris.append([a, b, c])
-            tris.append([a, c, d])
+    top = np.stack([np.zeros(nu, np.int64), idx(0, k), idx(0, k + 1)], 1)
+    r_ = np.arange(rows - 1)[:, None]
+    a, b, c, d = idx(r_, k), idx(r_ + 1, k), idx(r_ + 1, k + 1), idx(r_, k + 1)
+    mid = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], 2).reshape(-1, 3)
     last = V.shape[0] - 1
-    tris += [[last, idx(rows - 1, k + 1), idx(rows - 1, k)] for k in range(nu)]
+    bot = np.stack([np.full(nu, last), idx(rows - 1, k + 1), idx(rows - 1, k)], 1)
+    tris = np.concatenate([top, mid, bot])
     rng = np.random.default_rng(seed)
     V = V * (1.0 + rng.normal(0.0, sigma, size=(V.shape[0], 1)))
-    return V.astype(np.float32), np.asarray(tris, np.int32)
+    return V.astype(np.float32), tris.astype(np.int32)
 
 
 def padded_box(V, pad=0.1):
@@ -214,3 +215,50 @@ def split_dirichlet_neumann(V, F):
     """Dirichlet part: triangles with centroid z >= 0; Neumann: the rest."""
     cz = V[F].mean(1)[:, 2]
     return F[cz >= 0], F[cz < 0]
+
+
+def surface_points(V, F, n, seed):
+    """n points uniform by area on the mesh surface."""
+    rng = np.random.default_rng(seed)
+    a, b, c = (V[F[:, k]].astype(np.float64) for k in range(3))
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    tri = rng.choice(len(F), size=n, p=area / area.sum())
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    return (a[tri] + u[:, None] * (b[tri] - a[tri]) + v[:, None] * (c[tri] - a[tri])).astype(np.float32)
+
+
+def mlp_training_points(V, F, n, sigma=0.05, seed=9):
+    """Half uniform in the 10%-padded AABB, half near the surface (a surface
+    point plus an N(0, sigma) offset per axis)."""
+    lo, hi = padded_box(V)
+    h = n // 2
+    far = uniform_points(n - h, lo, hi, seed)
+    near = surface_points(V, F, h, seed + 1)
+    near = near + np.random.default_rng(seed + 2).normal(0.0, sigma, near.shape).astype(np.float32)
+    return np.concatenate([far, near]).astype(np.float32)
+
+
+def batched_meshes(n_meshes=64, f_lo=1e3, f_hi=5e5, q_lo=2 ** 12, q_hi=2 ** 20, seed=7):
+    """Alternating noised spheres and cut tori with triangle counts
+    log-uniform in [f_lo, f_hi] and query counts log-uniform in [q_lo, q_hi];
+    queries uniform in each mesh's 10%-padded AABB.  Returns
+    (meshes, queries concatenated by mesh, query offsets)."""
+    rng = np.random.default_rng(seed)
+    f = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), n_meshes))
+    nq = np.exp(rng.uniform(np.log(q_lo), np.log(q_hi), n_meshes)).astype(np.int64)
+    meshes, qs = [], []
+    for k in range(n_meshes):
+        if k % 2 == 0:
+            V, F = uv_sphere(int(f[k]), sigma=0.02, seed=100 + k)
+        else:
+            nu = max(8, int(round(f[k] ** 0.5)))
+            V, F = cut_torus(nu, max(4, nu // 2), seed=100 + k)
+        V = V + np.float32([3.0 * (k % 8), 3.0 * (k // 8), 0.0])
+        meshes.append((V, F))
+        lo, hi = padded_box(V)
+        qs.append(uniform_points(int(nq[k]), lo, hi, 200 + k))
+    q_off = np.zeros(n_meshes + 1, np.uint32)
+    q_off[1:] = np.cumsum([len(x) for x in qs])
+    return meshes, np.concatenate(qs).astype(np.float32), q_off

@@ -53,3 +53,23 @@ def test_mixed_graph_matches_eager(S):
     assert np.all(np.isfinite(a))
     # estimates are values of g on the Dirichlet part (z >= 0 there, |x y| <= ~0.9)
     assert np.all(np.abs(a) < 1.0)
+
+
+def test_mixed_matches_oracle(S, oracle):
+    """The CPU restatement walks the same random streams with the oracle's
+    queries: walks agree bit for bit unless a distance tie between two
+    triangles was resolved differently (smallest index here, visit order in
+    the reference)."""
+    V, F = W.cut_torus(64, 32)
+    FD, FN = W.split_dirichlet_neumann(V, F)
+    sd = S.Scene(V, FD).build_bvh()
+    sn = S.Scene(V, FN).build_bvh()
+    x0 = W.torus_interior_points(4096, seed=8)
+    ws = S.WoSt(sd, sn, max_walks=len(x0))
+    g, live = ws.solve(x0, max_steps=64, eps=1e-3, seed=3)
+    assert live == 0
+    o, steps = oracle.wost(oracle.build_tree(V, FD), oracle.build_tree(V, FN), x0, 64, 1e-3, 3)
+    g = g.cpu().numpy()
+    same = g == o
+    assert same.mean() >= 0.99, same.mean()
+    assert abs(g.mean() - o.mean()) < 0.01
