@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--build-reps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--input-order", action="store_true", help="also time the kernel over unordered queries")
+    ap.add_argument("--sweep", action="store_true", help="also time block size x stack depth launch shapes")
     ap.add_argument("--workload", default="silhouette", choices=["silhouette", "small", "batched", "wost", "mlp"],
                     help="silhouette: the headline (1M-triangle torus, 2^22 queries); small: 20,480-triangle "
                          "icosphere, 65,536 closest-point queries; batched: 64 uneven meshes in one launch; "
@@ -420,6 +421,15 @@ def main():
                                                    ctypes.byref(kopts), S._stream()))
 
     _, kernel_ms = run_timed(kstep, args.steps, 1)
+    # launch shapes of the same kernel (results identical): block size x LDS stack depth
+    sweep = {}
+    if args.sweep:
+        for bs in (64, 128, 256, 512):
+            for depth in (16, 24, 32):
+                o2 = S.QueryOpts(bs, 0, None, order.data_ptr(), depth, 0)
+                _, ms2 = run_timed(lambda: S.check(S.lib().snch_closest_silhouette_ex(
+                    sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d), ctypes.byref(o2), S._stream())), 2, 1)
+                sweep[f"{bs}x{depth}"] = ms2
     perm_buf = torch.empty(n, dtype=torch.int32, device=dev)
     _, order_ms = run_timed(lambda: S.check(S.lib().snch_query_order(sc._h, S._ptr(q), n, S._ptr(perm_buf),
                                                                      S._stream())), args.steps, 1)
@@ -439,7 +449,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(), "bytes_per_query": per_q,
                 "kernel": "k_closest_silhouette<256,false,3,false>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
-                "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms,
+                "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms, "shape_sweep_ms": sweep,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
 
     # learned-bound form: seed radius from the Fourier MLP (BF16 MFMA), then

@@ -64,8 +64,8 @@ typedef struct snch_query_opts {
                             visited, cone records read (silhouette query only) */
     const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
                               snch_query_order): the visiting order; overrides coherent */
-    uint32_t stack_depth;      /* batched queries: LDS stack entries per lane, 16, 24 or 32
-                                  (0 = 16) */
+    uint32_t stack_depth;      /* closest silhouette (3D) and batched queries: LDS stack
+                                  entries per lane, 16, 24 or 32 (0 = 16) */
     uint32_t queries_per_wave; /* batched queries: queries a wavefront takes from the work
                                   queue at a time, a multiple of 64 up to 4096 (0 = 64) */
 } snch_query_opts;

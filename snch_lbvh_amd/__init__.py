@@ -240,7 +240,8 @@ class Scene:
         check(lib().snch_closest_point_ex(self._h, _ptr(q), n, _ptr(idx), _ptr(dist), ctypes.byref(opts), _stream()))
         return idx, dist
 
-    def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None, coherent=0, order=None):
+    def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None, coherent=0, order=None,
+                           stack_depth=0):
         import torch
 
         q = _dev_f32(q, self.dim, self.device)
@@ -248,7 +249,7 @@ class Scene:
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), 0, 0)
+        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0)
         check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
                                                _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist

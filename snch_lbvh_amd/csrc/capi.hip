@@ -532,15 +532,17 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     SNCH_NEED_SINGLE(s, "snch_closest_silhouette");
     SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
     uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
+    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
+    SNCH_CHECK_ARG(depth == 16 || depth == 24 || depth == 32, "snch_closest_silhouette: stack_depth must be 16, 24 or 32");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = opts ? opts->order : nullptr;
     if (!perm && want_order(opts, n))
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block, perm,
-                                       opts ? opts->stats : nullptr, st),
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block, depth,
+                                       perm, opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
     return SNCH_OK;
 }

@@ -160,7 +160,7 @@ hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, Q
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
                                 const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block,
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint32_t depth,
                                      const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,

@@ -439,14 +439,14 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     }
 }
 
-template <int BS, bool STATS, int DIM, bool SEEDED>
+template <int BS, bool STATS, int DIM, bool SEEDED, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
                                                            const uint32_t* __restrict__ perm,
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
-    __shared__ uint2 stk[SS * BS];
+    __shared__ uint2 stk[SSZ * BS];
     const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     float best;
     uint32_t best_e;
     bool ovf = false;
-    sil_query<BS, SS, STATS, DIM, SEEDED>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
+    sil_query<BS, SSZ, STATS, DIM, SEEDED>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
                                           stk + threadIdx.x, best, best_e, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
@@ -899,29 +899,44 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
     return hipGetLastError();
 }
 
-template <int BS, int DIM>
-static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
-                          float* dist, const uint32_t* perm, unsigned long long* stats, hipStream_t st)
+template <int BS, int DIM, int SSZ>
+static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
+                           float* dist, const uint32_t* perm, unsigned long long* stats, hipStream_t st)
 {
+    const dim3 g(nblocks(n, BS)), b(BS);
+    uint32_t* err = traversal_error_word();
     if (stats) {
         if (seed)
-            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t,
-                               q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true, SSZ>), g, b, 0, st, t, q, n, flip, seed, sil,
+                               dist, perm, err, stats);
         else
-            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
-                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, false, SSZ>), g, b, 0, st, t, q, n, flip, seed,
+                               sil, dist, perm, err, stats);
     } else {
         if (seed)
-            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
-                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true, SSZ>), g, b, 0, st, t, q, n, flip, seed,
+                               sil, dist, perm, err, stats);
         else
-            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, false>), dim3(nblocks(n, BS)), dim3(BS), 0, st,
-                               t, q, n, flip, seed, sil, dist, perm, traversal_error_word(), stats);
+            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, false, SSZ>), g, b, 0, st, t, q, n, flip, seed,
+                               sil, dist, perm, err, stats);
     }
 }
 
+template <int BS, int DIM>
+static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
+                          float* dist, uint32_t depth, const uint32_t* perm, unsigned long long* stats,
+                          hipStream_t st)
+{
+    if (DIM == 2 || depth == 16) // scene<2> uses the default depth only
+        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+    else if (depth == 24)
+        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+    else
+        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+}
+
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block,
+                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint32_t depth,
                                      const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
@@ -929,10 +944,10 @@ hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q,
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
-        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, perm, sp, st); break;
+        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
+        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
         }
     });
     return hipGetLastError();

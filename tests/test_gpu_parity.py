@@ -178,9 +178,10 @@ def test_seeded_silhouette_identical(S):
         np.testing.assert_array_equal(e1, e0)
         np.testing.assert_array_equal(d1, d0)
     for bs in (64, 128, 512):
-        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, block_size=bs))
-        np.testing.assert_array_equal(e1, e0)
-        np.testing.assert_array_equal(d1, d0)
+        for depth in (16, 24, 32):
+            e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, block_size=bs, stack_depth=depth))
+            np.testing.assert_array_equal(e1, e0)
+            np.testing.assert_array_equal(d1, d0)
 
 
 def test_visiting_order_invariance(S, oracle):
