@@ -54,7 +54,7 @@ typedef struct snch_build_opts {
 
 /* Traversal tuning (chosen by the autotuner; defaults are safe) */
 typedef struct snch_query_opts {
-    uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (256) */
+    uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (64) */
     uint32_t coherent;   /* visiting order: 0 = Morton order of the query points when
                             n >= SNCH_ORDER_MIN_QUERIES, 1 = always Morton order,
                             2 = input order.  Results are written in input order and
@@ -64,10 +64,13 @@ typedef struct snch_query_opts {
                             visited, cone records read (silhouette query only) */
     const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
                               snch_query_order): the visiting order; overrides coherent */
-    uint32_t stack_depth;      /* closest silhouette (3D) and batched queries: LDS stack
-                                  entries per lane, 16, 24 or 32 (0 = 16) */
+    uint32_t stack_depth;      /* LDS stack entries per lane (0 = 16): closest point 8 or 16,
+                                  closest silhouette (3D) and batched queries 8, 16, 24 or 32 */
     uint32_t queries_per_wave; /* batched queries: queries a wavefront takes from the work
                                   queue at a time, a multiple of 64 up to 4096 (0 = 64) */
+    uint32_t seed_is_cap;      /* closest silhouette: 1 = seed_radius is a hard search radius
+                                  (the answer is exact when it lies within the radius; otherwise
+                                  the returned distance exceeds the radius, possibly +inf) */
 } snch_query_opts;
 
 const char *snch_last_error(void);
@@ -169,14 +172,14 @@ int snch_batch_closest_silhouette(const snch_scene *scene, const float *q, const
 
 /* Autotuner for the batched queries (kind 0 = closest point, 1 = closest
  * silhouette): times every leaf size {1, 2, 4, 8} x block size {64, 128, 256,
- * 512} x stack depth {16, 24, 32} x queries per wave {64, 128, 256} on the
+ * 512} x stack depth {8, 16, 24, 32} x queries per wave {64, 128, 256} on the
  * first min(count, sample_per_mesh) queries of each mesh (HIP events), then
  * rebuilds the scene with the best leaf size.  Pass the result's block_size,
  * stack_depth and queries_per_wave in snch_query_opts.  Synchronises. */
 typedef struct snch_tune_result {
     uint32_t leaf_size, block_size, stack_depth, queries_per_wave;
     float ms;         /* best time of one batched launch over the sample */
-    float ms_default; /* the same for leaf 1, block 256, stack 16, 64 queries per wave */
+    float ms_default; /* the same for leaf 1, block 64, stack 16, 64 queries per wave */
     uint32_t n_tried;
 } snch_tune_result;
 int snch_batch_autotune(snch_scene *scene, uint32_t kind, const float *q, const uint32_t *q_off,

@@ -53,7 +53,8 @@ class BuildOpts(ctypes.Structure):
 
 class QueryOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p),
-                ("order", ctypes.c_void_p), ("stack_depth", ctypes.c_uint32), ("queries_per_wave", ctypes.c_uint32)]
+                ("order", ctypes.c_void_p), ("stack_depth", ctypes.c_uint32), ("queries_per_wave", ctypes.c_uint32),
+                ("seed_is_cap", ctypes.c_uint32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -226,7 +227,7 @@ class Scene:
         check(lib().snch_query_order(self._h, _ptr(q), q.shape[0], _ptr(perm), _stream()))
         return perm
 
-    def closest_point(self, q, stats=None, coherent=0, order=None):
+    def closest_point(self, q, stats=None, coherent=0, order=None, block_size=0, stack_depth=0):
         """stats: optional int64 CUDA tensor of 5 counters (child box tests,
         leaf evaluations, primitive tests, internal nodes visited, cone
         records read), accumulated."""
@@ -236,12 +237,12 @@ class Scene:
         n = q.shape[0]
         idx = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(0, coherent, _stats_ptr(stats), _order_ptr(order, n), 0, 0)
+        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0)
         check(lib().snch_closest_point_ex(self._h, _ptr(q), n, _ptr(idx), _ptr(dist), ctypes.byref(opts), _stream()))
         return idx, dist
 
     def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None, coherent=0, order=None,
-                           stack_depth=0):
+                           stack_depth=0, seed_is_cap=False):
         import torch
 
         q = _dev_f32(q, self.dim, self.device)
@@ -249,7 +250,8 @@ class Scene:
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
-        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0)
+        opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0,
+                         int(bool(seed_is_cap)))
         check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
                                                _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist

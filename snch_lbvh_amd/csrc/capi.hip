@@ -514,7 +514,12 @@ int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint3
     const uint32_t* perm = opts ? opts->order : nullptr;
     if (!perm && want_order(opts, n))
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_closest_point: ordering");
-    SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, perm,
+    const uint32_t block = opts && opts->block_size ? opts->block_size : 64u;
+    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
+    SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
+                   "snch_closest_point: block_size must be 64, 128, 256 or 512");
+    SNCH_CHECK_ARG(depth == 8 || depth == 16, "snch_closest_point: stack_depth must be 8 or 16");
+    SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, block, depth, perm,
                                   opts ? opts->stats : nullptr, st),
              "snch_closest_point");
     return SNCH_OK;
@@ -531,18 +536,19 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
 {
     SNCH_NEED_SINGLE(s, "snch_closest_silhouette");
     SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
-    uint32_t block = opts && opts->block_size ? opts->block_size : 256u;
+    uint32_t block = opts && opts->block_size ? opts->block_size : 64u;
     const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
-    SNCH_CHECK_ARG(depth == 16 || depth == 24 || depth == 32, "snch_closest_silhouette: stack_depth must be 16, 24 or 32");
+    SNCH_CHECK_ARG(depth == 8 || depth == 16 || depth == 24 || depth == 32,
+                   "snch_closest_silhouette: stack_depth must be 8, 16, 24 or 32");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = opts ? opts->order : nullptr;
     if (!perm && want_order(opts, n))
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, block, depth,
-                                       perm, opts ? opts->stats : nullptr, st),
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, opts && opts->seed_is_cap,
+                                       out_sil, out_dist, block, depth, perm, opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
     return SNCH_OK;
 }
@@ -595,15 +601,15 @@ int snch_sample_on_object(const snch_scene* s, const int32_t* idx, const float* 
 
 static int batch_config(const snch_query_opts* opts, uint32_t n, BatchConfig* cfg, const char* fn)
 {
-    cfg->block_size = opts && opts->block_size ? opts->block_size : 256u;
+    cfg->block_size = opts && opts->block_size ? opts->block_size : 64u;
     cfg->stack_depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
     cfg->queries_per_wave = opts && opts->queries_per_wave ? opts->queries_per_wave : 64u;
     const uint32_t mode = opts ? opts->coherent : 0u;
     cfg->coherent = mode == 1u || (mode == 0u && n >= SNCH_ORDER_MIN_QUERIES);
     const uint32_t b = cfg->block_size, d = cfg->stack_depth, w = cfg->queries_per_wave;
-    if (!(b == 64 || b == 128 || b == 256 || b == 512) || !(d == 16 || d == 24 || d == 32) || w % 64 != 0 ||
+    if (!(b == 64 || b == 128 || b == 256 || b == 512) || !(d == 8 || d == 16 || d == 24 || d == 32) || w % 64 != 0 ||
         w > 4096 || (opts && opts->order)) {
-        set_error(std::string(fn) + ": block_size must be 64..512 (power of 2), stack_depth 16, 24 or 32, "
+        set_error(std::string(fn) + ": block_size must be 64..512 (power of 2), stack_depth 8, 16, 24 or 32, "
                                     "queries_per_wave a multiple of 64 up to 4096, and order NULL");
         return SNCH_ERR_INVALID;
     }
@@ -690,9 +696,9 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
     SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
     const uint32_t flags = s->build_flags;
-    const uint32_t leaves[4] = {1, 2, 4, 8}, blocks[4] = {64, 128, 256, 512}, depths[3] = {16, 24, 32},
+    const uint32_t leaves[4] = {1, 2, 4, 8}, blocks[4] = {64, 128, 256, 512}, depths[4] = {8, 16, 24, 32},
                    qpws[3] = {64, 128, 256};
-    snch_tune_result best{1, 256, 16, 64, 1e30f, 0.0f, 0};
+    snch_tune_result best{1, 64, 16, 64, 1e30f, 0.0f, 0};
     hipError_t err = hipSuccess;
     for (uint32_t L : leaves) {
         if ((err = build_tree(s, flags, L, st)) != hipSuccess)
@@ -720,7 +726,7 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
                     (void)hipEventElapsedTime(&ms, e0, e1);
                     ms *= 0.5f;
                     ++best.n_tried;
-                    if (L == 1 && B == 256 && D == 16 && W == 64)
+                    if (L == 1 && B == 64 && D == 16 && W == 64)
                         best.ms_default = ms;
                     if (ms < best.ms) {
                         best.ms = ms;

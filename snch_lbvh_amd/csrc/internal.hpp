@@ -94,7 +94,7 @@ struct BatchState {
 };
 
 struct BatchConfig {
-    uint32_t block_size = 256, stack_depth = 16, queries_per_wave = 64;
+    uint32_t block_size = 64, stack_depth = 16, queries_per_wave = 64;
     bool coherent = true;
 };
 
@@ -158,10 +158,13 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
                        const uint32_t** perm);
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
-                                const uint32_t* perm, uint64_t* stats, hipStream_t st);
+                                uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
+                                hipStream_t st);
+// capped: seed is a hard radius (answers beyond it are not searched; the
+// returned distance is then larger than the radius, possibly +inf)
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint32_t depth,
-                                     const uint32_t* perm, uint64_t* stats, hipStream_t st);
+                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

@@ -255,13 +255,13 @@ __device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint2* stk_col
     overflow = overflow || st.overflow;
 }
 
-template <int BS, bool STATS, int DIM>
+template <int BS, bool STATS, int DIM, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* __restrict__ q, uint32_t n,
                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
                                                       const uint32_t* __restrict__ perm, uint32_t* __restrict__ err,
                                                       unsigned long long* stats)
 {
-    __shared__ uint2 stk[SS * BS];
+    __shared__ uint2 stk[SSZ * BS];
     const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
     float best;
     uint32_t best_idx;
     bool ovf = false;
-    cp_query<BS, SS, STATS, DIM>(t, load_point<DIM>(q, qi), stk + threadIdx.x, best, best_idx, cnt, ovf);
+    cp_query<BS, SSZ, STATS, DIM>(t, load_point<DIM>(q, qi), stk + threadIdx.x, best, best_idx, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
     out_idx[qi] = best_idx;
@@ -350,7 +350,7 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 // otherwise the search reruns without the cap, keeping what it found as the
 // initial bound.  With index tie-breaking the result is bit-identical to the
 // unseeded search.
-template <int BS, int SSZ, bool STATS, int DIM, bool SEEDED>
+template <int BS, int SSZ, bool STATS, int DIM, bool SEEDED, bool FALLBACK = true>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint2* stk_col,
                                           float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
 {
@@ -366,7 +366,7 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     if (!SEEDED || !(cap >= 0.0f))
         cap = F_INF;
     LaneStack<BS, SSZ> st;
-    for (int pass = 0; pass < (SEEDED ? 2 : 1); ++pass) {
+    for (int pass = 0; pass < (SEEDED && FALLBACK ? 2 : 1); ++pass) {
         st.init(stk_col);
         const float cap2 = cap * cap;
         uint32_t node = 0;
@@ -439,7 +439,7 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     }
 }
 
-template <int BS, bool STATS, int DIM, bool SEEDED, int SSZ>
+template <int BS, bool STATS, int DIM, bool SEEDED, int SSZ, bool FALLBACK = true>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     float best;
     uint32_t best_e;
     bool ovf = false;
-    sil_query<BS, SSZ, STATS, DIM, SEEDED>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
+    sil_query<BS, SSZ, STATS, DIM, SEEDED, FALLBACK>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
                                           stk + threadIdx.x, best, best_e, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
@@ -882,29 +882,61 @@ hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, Q
     return hipGetLastError();
 }
 
+template <int BS, int DIM, int SSZ>
+static void launch_cp_cfg(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
+                          const uint32_t* perm, unsigned long long* stats, hipStream_t st)
+{
+    if (stats)
+        hipLaunchKernelGGL((k_closest_point<BS, true, DIM, SSZ>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n, idx,
+                           dist, perm, traversal_error_word(), stats);
+    else
+        hipLaunchKernelGGL((k_closest_point<BS, false, DIM, SSZ>), dim3(nblocks(n, BS)), dim3(BS), 0, st, t, q, n,
+                           idx, dist, perm, traversal_error_word(), stats);
+}
+
+template <int BS, int DIM>
+static void launch_cp_bs(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist, uint32_t depth,
+                         const uint32_t* perm, unsigned long long* stats, hipStream_t st)
+{
+    if (DIM == 3 && depth == 8)
+        launch_cp_cfg<BS, DIM, 8>(t, q, n, idx, dist, perm, stats, st);
+    else
+        launch_cp_cfg<BS, DIM, 16>(t, q, n, idx, dist, perm, stats, st);
+}
+
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
-                                const uint32_t* perm, uint64_t* stats, hipStream_t st)
+                                uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
-        if (stats)
-            hipLaunchKernelGGL((k_closest_point<256, true, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
-                               idx, dist, perm, traversal_error_word(), sp);
-        else
-            hipLaunchKernelGGL((k_closest_point<256, false, DIM>), dim3(nblocks(n, 256)), dim3(256), 0, st, t, q, n,
-                               idx, dist, perm, traversal_error_word(), sp);
+        switch (block) {
+        case 128: launch_cp_bs<128, DIM>(t, q, n, idx, dist, depth, perm, sp, st); break;
+        case 256: launch_cp_bs<256, DIM>(t, q, n, idx, dist, depth, perm, sp, st); break;
+        case 512: launch_cp_bs<512, DIM>(t, q, n, idx, dist, depth, perm, sp, st); break;
+        default: launch_cp_bs<64, DIM>(t, q, n, idx, dist, depth, perm, sp, st); break;
+        }
     });
     return hipGetLastError();
 }
 
 template <int BS, int DIM, int SSZ>
-static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
-                           float* dist, const uint32_t* perm, unsigned long long* stats, hipStream_t st)
+static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
+                           uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
+                           hipStream_t st)
 {
     const dim3 g(nblocks(n, BS)), b(BS);
     uint32_t* err = traversal_error_word();
+    if (capped && seed) {
+        if (stats)
+            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true, SSZ, false>), g, b, 0, st, t, q, n, flip,
+                               seed, sil, dist, perm, err, stats);
+        else
+            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true, SSZ, false>), g, b, 0, st, t, q, n, flip,
+                               seed, sil, dist, perm, err, stats);
+        return;
+    }
     if (stats) {
         if (seed)
             hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true, SSZ>), g, b, 0, st, t, q, n, flip, seed, sil,
@@ -923,31 +955,33 @@ static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int fl
 }
 
 template <int BS, int DIM>
-static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, uint32_t* sil,
-                          float* dist, uint32_t depth, const uint32_t* perm, unsigned long long* stats,
-                          hipStream_t st)
+static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
+                          uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
+                          unsigned long long* stats, hipStream_t st)
 {
     if (DIM == 2 || depth == 16) // scene<2> uses the default depth only
-        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+    else if (depth == 8)
+        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
     else if (depth == 24)
-        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
     else
-        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, uint32_t* sil, float* dist, uint32_t block, uint32_t depth,
-                                     const uint32_t* perm, uint64_t* stats, hipStream_t st)
+                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 64: launch_sil_bs<64, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
-        default: launch_sil_bs<256, DIM>(t, q, n, flip, seed, sil, dist, depth, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
         }
     });
     return hipGetLastError();
@@ -1014,6 +1048,7 @@ static hipError_t launch_batch_bs(const BatchTable& bt, uint32_t ssz, const floa
                                   float* out_dist, uint32_t n_cu, hipStream_t st)
 {
     switch (ssz) {
+    case 8: return launch_batch_cfg<BS, 8, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
     case 24: return launch_batch_cfg<BS, 24, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
     case 32: return launch_batch_cfg<BS, 32, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
     default: return launch_batch_cfg<BS, 16, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);

@@ -8,7 +8,9 @@
 // captured once as a hipGraph and replayed):
 //   k_gather      live walk positions -> compact query arrays
 //   closest point on the Dirichlet scene     -> d_D
-//   closest silhouette on the Neumann scene  -> d_S
+//   closest silhouette on the Neumann scene, searched only within d_D (the
+//                 star radius is min(d_D, d_S), so a farther silhouette
+//                 cannot matter)                      -> d_S
 //   k_star        terminate inside the eps-shell (value = g at the closest
 //                 Dirichlet point); else star radius R = min(d_D, d_S), a
 //                 direction uniform on the sphere (hemisphere into the
@@ -272,10 +274,10 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
         vd.n_dev = s->w.count + cur;
         vn.n_dev = s->w.count + cur;
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, s->w, cur, cap);
-        if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, nullptr, nullptr, st)) != hipSuccess)
+        if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, 64, 16, nullptr, nullptr, st)) != hipSuccess)
             return e;
         if (has_n &&
-            (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, nullptr, s->w.sil_idx, s->w.sil_dist, 256, 16,
+            (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, s->w.cp_dist, true, s->w.sil_idx, s->w.sil_dist, 64, 16,
                                            nullptr, nullptr, st)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_star, dim3(nb), dim3(256), 0, st, s->w, cur, cap, s->dir->dV.as<float4>(),

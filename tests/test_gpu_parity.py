@@ -177,11 +177,25 @@ def test_seeded_silhouette_identical(S):
         e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=seed.astype(np.float32)))
         np.testing.assert_array_equal(e1, e0)
         np.testing.assert_array_equal(d1, d0)
-    for bs in (64, 128, 512):
-        for depth in (16, 24, 32):
+    for bs in (64, 128, 256, 512):
+        for depth in (8, 16, 24, 32):
             e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, block_size=bs, stack_depth=depth))
             np.testing.assert_array_equal(e1, e0)
             np.testing.assert_array_equal(d1, d0)
+    # a hard cap: exact within the radius, beyond it only "farther than the cap"
+    i0, c0 = (t.cpu().numpy() for t in sc.closest_point(q))
+    for bs in (64, 256):
+        for depth in (8, 16):
+            i1, c1 = (t.cpu().numpy() for t in sc.closest_point(q, block_size=bs, stack_depth=depth))
+            np.testing.assert_array_equal(i1, i0)
+            np.testing.assert_array_equal(c1, c0)
+    for cap in (d0, d0 * 0.5, rng.random(len(q)).astype(np.float32) * 0.3, c0):
+        cap = np.where(np.isfinite(cap), cap, np.float32(1.0)).astype(np.float32)
+        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=cap, seed_is_cap=True))
+        within = d0 <= cap
+        np.testing.assert_array_equal(d1[within], d0[within])
+        np.testing.assert_array_equal(e1[within], e0[within])
+        assert np.all(d1[~within] > cap[~within])
 
 
 def test_visiting_order_invariance(S, oracle):
