@@ -424,12 +424,20 @@ def main():
     # launch shapes of the same kernel (results identical): block size x LDS stack depth
     sweep = {}
     if args.sweep:
-        for bs in (64, 128, 256, 512):
-            for depth in (16, 24, 32):
+        for bs in (64, 128, 256):
+            for depth in (8, 16, 24):
                 o2 = S.QueryOpts(bs, 0, None, order.data_ptr(), depth, 0)
                 _, ms2 = run_timed(lambda: S.check(S.lib().snch_closest_silhouette_ex(
                     sc._h, S._ptr(q), n, 0, None, S._ptr(out_e), S._ptr(out_d), ctypes.byref(o2), S._stream())), 2, 1)
                 sweep[f"{bs}x{depth}"] = ms2
+        cp_i = torch.empty_like(out_e)
+        cp_d = torch.empty_like(out_d)
+        for bs in (64, 256):
+            for depth in (8, 16):
+                o3 = S.QueryOpts(bs, 0, None, order.data_ptr(), depth, 0)
+                _, ms3 = run_timed(lambda: S.check(S.lib().snch_closest_point_ex(
+                    sc._h, S._ptr(q), n, S._ptr(cp_i), S._ptr(cp_d), ctypes.byref(o3), S._stream())), 2, 1)
+                sweep[f"closest_point_{bs}x{depth}"] = ms3
     perm_buf = torch.empty(n, dtype=torch.int32, device=dev)
     _, order_ms = run_timed(lambda: S.check(S.lib().snch_query_order(sc._h, S._ptr(q), n, S._ptr(perm_buf),
                                                                      S._stream())), args.steps, 1)
@@ -448,7 +456,7 @@ def main():
     achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(), "bytes_per_query": per_q,
-                "kernel": "k_closest_silhouette<256,false,3,false>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
+                "kernel": "k_closest_silhouette<64,false,3,false,16>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
                 "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms, "shape_sweep_ms": sweep,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
 
@@ -470,22 +478,44 @@ def main():
         out_e2 = torch.empty_like(out_e)
         out_d2 = torch.empty_like(out_d)
 
-        def lstep():
-            mlp.seed_radius(q, rel=args.seed_rel, abs_margin=args.seed_abs, out=seed_r)
+        def seeded(seed_tensor):
             opts = S.QueryOpts(0, 0, None, None)
-            S.check(S.lib().snch_closest_silhouette_ex(sc._h, S._ptr(q), n, 0, S._ptr(seed_r), S._ptr(out_e2),
+            S.check(S.lib().snch_closest_silhouette_ex(sc._h, S._ptr(q), n, 0, S._ptr(seed_tensor), S._ptr(out_e2),
                                                        S._ptr(out_d2), ctypes.byref(opts), S._stream()))
+
+        # margin sweep: seed = prediction * (1 + rel) + abs
+        margins = {}
+        for rel in (0.1, 0.25, 0.5, 1.0):
+            def lstep(rel=rel):
+                mlp.seed_radius(q, rel=rel, abs_margin=args.seed_abs, out=seed_r)
+                seeded(seed_r)
+
+            lwall, _ = run_timed(lstep, max(2, args.steps // 3), 1)
+            assert torch.equal(out_e2, out_e) and torch.equal(out_d2, out_d), "seeded result differs"
+            fb = float(((out_d > seed_r) | ~torch.isfinite(out_d)).float().mean())
+            margins[rel] = {"ms_per_step": lwall * 1000.0 / max(2, args.steps // 3), "fallback_fraction": fb}
+        best_rel = min(margins, key=lambda r: margins[r]["ms_per_step"])
+
+        def lstep():
+            mlp.seed_radius(q, rel=best_rel, abs_margin=args.seed_abs, out=seed_r)
+            seeded(seed_r)
 
         lwall, lms = run_timed(lstep, args.steps, args.warmup)
         assert torch.equal(out_e2, out_e) and torch.equal(out_d2, out_d), "seeded result differs"
         st2 = torch.zeros(5, dtype=torch.int64, device=dev)
         sc.closest_silhouette(q, seed_radius=seed_r, stats=st2)
         fallback = float(((out_d > seed_r) | ~torch.isfinite(out_d)).float().mean())
+        # the bound a perfect predictor would give: the exact answer itself
+        ideal = torch.where(torch.isfinite(out_d), out_d, torch.full_like(out_d, -1.0))
+        iwall, _ = run_timed(lambda: seeded(ideal), max(2, args.steps // 3), 1)
+        st3 = torch.zeros(5, dtype=torch.int64, device=dev)
+        sc.closest_silhouette(q, seed_radius=ideal, stats=st3)
         learned = {"value": n * args.steps / lwall, "ms_per_step": lwall * 1000.0 / args.steps,
                    "train_s": t_train, "train_points": args.train_points, "train_epochs": args.train_epochs,
                    "loss_first_last": [float(losses[0]), float(losses[-1])],
-                   "nodes_per_query": st2[3].item() / n, "seed_rel": args.seed_rel, "seed_abs": args.seed_abs,
-                   "fallback_fraction": fallback}
+                   "nodes_per_query": st2[3].item() / n, "seed_rel": best_rel, "seed_abs": args.seed_abs,
+                   "fallback_fraction": fallback, "margin_sweep": margins,
+                   "exact_seed": {"value": n * max(2, args.steps // 3) / iwall, "nodes_per_query": st3[3].item() / n}}
 
     # other queries (reported beside the headline)
     extra = {}
