@@ -52,8 +52,9 @@ def parse():
                          "wost: walk on stars, 2^20 walks; mlp: learned-bound MLP training and inference")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="short run for hardware-counter passes: 2 silhouette launches + a 256 MiB copy")
-    ap.add_argument("--train-points", type=int, default=1 << 18)
-    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--train-points", type=int, default=1 << 20)
+    ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--tau", type=float, default=0.9, help="expectile of the learned bound (0.5: squared error)")
     ap.add_argument("--seed-rel", type=float, default=0.25)
     ap.add_argument("--seed-abs", type=float, default=0.005)
     return ap.parse_args()
@@ -231,7 +232,7 @@ def workload_batched(args):
 
     wall0, ev0 = run_timed(torch, lambda: launch(None), max(2, args.steps // 3), 1)
     t0 = time.perf_counter()
-    tune = b.autotune(q, q_off, kind="silhouette", sample_per_mesh=4096)
+    tune = b.autotune(q, q_off, kind="silhouette", sample_per_mesh=0)
     t_tune = time.perf_counter() - t0
     wall, ev = run_timed(torch, lambda: launch(tune), args.steps, args.warmup)
     ref = (sil.clone(), dist.clone())
@@ -313,10 +314,10 @@ def workload_mlp(args):
     pts = torch.from_numpy(W.mlp_training_points(V, F, 1 << 24, seed=9)).cuda()
     _, lab = sc.closest_point(pts)
     mlp = S.LearnedBound(lo.astype(np.float32), hi.astype(np.float32), sigma=4.0, seed=1)
-    epochs = 3
+    epochs = 4
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = mlp.train(pts, lab, batch=16384, epochs=epochs, lr=2e-3)
+    losses = mlp.train(pts, lab, batch=4096, epochs=epochs, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     qi = torch.from_numpy(W.uniform_points(1 << 22, lo, hi, seed=4)).cuda()
@@ -468,7 +469,7 @@ def main():
         lo32, hi32 = lo.astype(np.float32), hi.astype(np.float32)
         tp = torch.from_numpy(W.uniform_points(args.train_points, lo, hi, seed=21)).to(dev)
         _, td = sc.closest_silhouette(tp)
-        mlp = S.LearnedBound(lo32, hi32, sigma=4.0, seed=1)
+        mlp = S.LearnedBound(lo32, hi32, sigma=4.0, seed=1).set_loss(args.tau)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         losses = mlp.train(tp, td, batch=4096, epochs=args.train_epochs, lr=3e-3)
@@ -485,7 +486,7 @@ def main():
 
         # margin sweep: seed = prediction * (1 + rel) + abs
         margins = {}
-        for rel in (0.1, 0.25, 0.5, 1.0):
+        for rel in (0.0, 0.1, 0.25, 0.5, 1.0):
             def lstep(rel=rel):
                 mlp.seed_radius(q, rel=rel, abs_margin=args.seed_abs, out=seed_r)
                 seeded(seed_r)
@@ -514,7 +515,7 @@ def main():
                    "train_s": t_train, "train_points": args.train_points, "train_epochs": args.train_epochs,
                    "loss_first_last": [float(losses[0]), float(losses[-1])],
                    "nodes_per_query": st2[3].item() / n, "seed_rel": best_rel, "seed_abs": args.seed_abs,
-                   "fallback_fraction": fallback, "margin_sweep": margins,
+                   "fallback_fraction": fallback, "margin_sweep": margins, "tau": args.tau,
                    "exact_seed": {"value": n * max(2, args.steps // 3) / iwall, "nodes_per_query": st3[3].item() / n}}
 
     # other queries (reported beside the headline)

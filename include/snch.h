@@ -64,8 +64,9 @@ typedef struct snch_query_opts {
                             visited, cone records read (silhouette query only) */
     const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
                               snch_query_order): the visiting order; overrides coherent */
-    uint32_t stack_depth;      /* LDS stack entries per lane (0 = 16): closest point 8 or 16,
-                                  closest silhouette (3D) and batched queries 8, 16, 24 or 32 */
+    uint32_t stack_depth;      /* LDS stack entries per lane: closest point 8 or 16 (0 = 16),
+                                  closest silhouette (3D) 8, 16, 24 or 32 (0 = 8), batched
+                                  queries 8, 16, 24 or 32 (0 = 16) */
     uint32_t queries_per_wave; /* batched queries: queries a wavefront takes from the work
                                   queue at a time, a multiple of 64 up to 4096 (0 = 64) */
     uint32_t seed_is_cap;      /* closest silhouette: 1 = seed_radius is a hard search radius
@@ -171,15 +172,20 @@ int snch_batch_closest_silhouette(const snch_scene *scene, const float *q, const
                                   uint32_t *out_sil, float *out_dist, const snch_query_opts *opts, void *stream);
 
 /* Autotuner for the batched queries (kind 0 = closest point, 1 = closest
- * silhouette): times every leaf size {1, 2, 4, 8} x block size {64, 128, 256,
- * 512} x stack depth {8, 16, 24, 32} x queries per wave {64, 128, 256} on the
- * first min(count, sample_per_mesh) queries of each mesh (HIP events), then
- * rebuilds the scene with the best leaf size.  Pass the result's block_size,
- * stack_depth and queries_per_wave in snch_query_opts.  Synchronises. */
+ * silhouette): coordinate descent from leaf 1 / block 64 / stack 16 / 64
+ * queries per wave over leaf size {1, 2, 4, 8} (rebuilding), block size {64,
+ * 128, 256, 512}, stack depth {8, 16, 24, 32} and queries per wave {64, 128,
+ * 256}, each axis swept with the others at their best so far; one warm-up
+ * and one HIP-event-timed launch per configuration over all queries
+ * (sample_per_mesh = 0) or the first sample_per_mesh queries of each mesh.
+ * Leaves the scene built with the best leaf size.  Pass the result's
+ * block_size, stack_depth and queries_per_wave in snch_query_opts.
+ * Synchronises. */
 typedef struct snch_tune_result {
     uint32_t leaf_size, block_size, stack_depth, queries_per_wave;
     float ms;         /* best time of one batched launch over the sample */
     float ms_default; /* the same for leaf 1, block 64, stack 16, 64 queries per wave */
+    /* n_tried: configurations timed */
     uint32_t n_tried;
 } snch_tune_result;
 int snch_batch_autotune(snch_scene *scene, uint32_t kind, const float *q, const uint32_t *q_off,
@@ -221,6 +227,10 @@ int snch_mlp_params(snch_mlp *mlp, float *params_out, const float *params_in, fl
  * loss per epoch.  Synchronises the stream. */
 int snch_mlp_train(snch_mlp *mlp, const float *pts, const float *dist, uint32_t n, uint32_t batch, uint32_t epochs,
                    float lr, float *epoch_loss, void *stream);
+/* Loss: w (o - t)^2 with t = log(min(d / D, 1) + 1e-3), w = 2 tau when the
+ * prediction o is above t and 2 (1 - tau) below it (an expectile; tau = 0.5,
+ * the default, is the squared error).  tau > 0.5 trains an upper bound. */
+int snch_mlp_set_loss(snch_mlp *mlp, float tau);
 /* Gradient of the mean loss over n points (host output, FP32), for checking. */
 int snch_mlp_gradient(snch_mlp *mlp, const float *pts, const float *dist, uint32_t n, float *grad_out,
                       float *loss_out, void *stream);

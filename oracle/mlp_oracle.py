@@ -66,14 +66,15 @@ def forward(theta, x, fourier, frame, bf16_weights=False):
     return a @ w5 + b5, cache, (Ws, bs, w5, b5)
 
 
-def loss_and_grad(theta, x, d, fourier, frame, bf16_weights=False):
+def loss_and_grad(theta, x, d, fourier, frame, bf16_weights=False, tau=0.5):
     o, cache, (Ws, bs, w5, b5) = forward(theta, x, fourier, frame, bf16_weights)
     t = target(d, frame)
     n = len(o)
     err = o - t
-    loss = np.mean(err ** 2)
+    w = 2.0 * np.where(err > 0, tau, 1.0 - tau)  # expectile weights (tau = 0.5: squared error)
+    loss = np.mean(w * err ** 2)
     g = np.zeros(NPARAM)
-    go = 2.0 * err / n
+    go = 2.0 * w * err / n
     a4 = cache[4][1]
     g[OFF_W5:OFF_W5 + H] = go @ a4
     g[OFF_B5] = go.sum()

@@ -73,6 +73,7 @@ EXPORTED = [
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
     "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
+    "snch_mlp_set_loss",
     "snch_mlp_gradient", "snch_mlp_predict", "snch_wost_create", "snch_wost_destroy", "snch_wost_solve",
 ]
 
@@ -110,6 +111,7 @@ def _declare(L):
     L.snch_mlp_params.argtypes = [vp, vp, vp, vp, vp]
     L.snch_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, ctypes.c_float, vp, vp]
     L.snch_mlp_gradient.argtypes = [vp, vp, vp, u32, vp, vp, vp]
+    L.snch_mlp_set_loss.argtypes = [vp, ctypes.c_float]
     L.snch_mlp_predict.argtypes = [vp, vp, u32, c_int, ctypes.c_float, ctypes.c_float, vp, vp]
     L.snch_wost_create.argtypes = [vp, vp, vp, u32, ctypes.POINTER(vp)]
     L.snch_wost_destroy.argtypes = [vp]
@@ -406,7 +408,7 @@ class BatchScene:
                                                   _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist
 
-    def autotune(self, q, q_off, kind="silhouette", sample_per_mesh=4096, flip=False):
+    def autotune(self, q, q_off, kind="silhouette", sample_per_mesh=0, flip=False):
         """Sweeps leaf size, block size, stack depth and queries per wave;
         rebuilds with the best leaf size and returns the choice as a dict."""
         q = _dev_f32(q, 3, self.device)
@@ -461,6 +463,12 @@ class LearnedBound:
         th = np.ascontiguousarray(theta, np.float32)
         assert th.size == self.n_params
         check(lib().snch_mlp_params(self._h, None, th.ctypes.data, None, None))
+
+    def set_loss(self, tau=0.5):
+        """Expectile loss weight: tau > 0.5 penalises under-prediction more,
+        training an upper bound on the distance."""
+        check(lib().snch_mlp_set_loss(self._h, float(tau)))
+        return self
 
     def train(self, pts, dist, batch=16384, epochs=1, lr=1e-3):
         p = _dev_f32(pts, 3, self.device)

@@ -537,7 +537,7 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     SNCH_NEED_SINGLE(s, "snch_closest_silhouette");
     SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
     uint32_t block = opts && opts->block_size ? opts->block_size : 64u;
-    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
+    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 8u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
     SNCH_CHECK_ARG(depth == 8 || depth == 16 || depth == 24 || depth == 32,
@@ -670,85 +670,117 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
                         uint32_t sample_per_mesh, int flip, snch_tune_result* out, void* stream)
 {
     SNCH_NEED_BUILT(s, "snch_batch_autotune");
-    SNCH_CHECK_ARG(out && kind <= 1u && sample_per_mesh > 0 && s->dim == 3, "snch_batch_autotune: bad argument");
+    SNCH_CHECK_ARG(out && kind <= 1u && s->dim == 3, "snch_batch_autotune: bad argument");
     int rc = check_qoff(s, q_off, "snch_batch_autotune");
     if (rc != SNCH_OK)
         return rc;
     SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t M = s->n_meshes;
-    // the sample: the first queries of each mesh, gathered into one batch
+    // the tuning set: every query (sample_per_mesh = 0) or the first
+    // sample_per_mesh queries of each mesh, gathered into one batch
     std::vector<uint32_t> soff(M + 1, 0);
     for (uint32_t m = 0; m < M; ++m)
-        soff[m + 1] = soff[m] + std::min(q_off[m + 1] - q_off[m], sample_per_mesh);
+        soff[m + 1] = soff[m] + (sample_per_mesh ? std::min(q_off[m + 1] - q_off[m], sample_per_mesh)
+                                                 : q_off[m + 1] - q_off[m]);
     const uint32_t ns = soff[M];
     SNCH_CHECK_ARG(ns > 0 && q, "snch_batch_autotune: no queries");
     DeviceBuffer sq, sidx, sdist;
-    SNCH_HIP(sq.ensure(sizeof(float) * 3 * (size_t)ns), "snch_batch_autotune: alloc");
+    const float* tq = q;
+    if (sample_per_mesh) {
+        SNCH_HIP(sq.ensure(sizeof(float) * 3 * (size_t)ns), "snch_batch_autotune: alloc");
+        for (uint32_t m = 0; m < M; ++m)
+            if (soff[m + 1] > soff[m])
+                SNCH_HIP(hipMemcpyAsync(sq.as<float>() + 3 * (size_t)soff[m], q + 3 * (size_t)q_off[m],
+                                        sizeof(float) * 3 * (soff[m + 1] - soff[m]), hipMemcpyDeviceToDevice, st),
+                         "snch_batch_autotune: gather");
+        tq = sq.as<float>();
+    }
     SNCH_HIP(sidx.ensure(sizeof(uint32_t) * ns), "snch_batch_autotune: alloc");
     SNCH_HIP(sdist.ensure(sizeof(float) * ns), "snch_batch_autotune: alloc");
-    for (uint32_t m = 0; m < M; ++m)
-        if (soff[m + 1] > soff[m])
-            SNCH_HIP(hipMemcpyAsync(sq.as<float>() + 3 * (size_t)soff[m], q + 3 * (size_t)q_off[m],
-                                    sizeof(float) * 3 * (soff[m + 1] - soff[m]), hipMemcpyDeviceToDevice, st),
-                     "snch_batch_autotune: gather");
     hipEvent_t e0, e1;
     SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    hipError_t err = hipSuccess;
+    snch_tune_result cur{1, 64, 16, 64, 0.0f, 0.0f, 0};
+    // one warm-up and one timed launch of a configuration
+    auto measure = [&](const snch_tune_result& c, float* ms) -> hipError_t {
+        BatchConfig cfg;
+        cfg.block_size = c.block_size;
+        cfg.stack_depth = c.stack_depth;
+        cfg.queries_per_wave = c.queries_per_wave;
+        cfg.coherent = true;
+        hipError_t x = launch_batch(s, (int)kind, tq, soff.data(), flip, sidx.as<uint32_t>(), sdist.as<float>(),
+                                    cfg, st);
+        if (x == hipSuccess)
+            x = hipEventRecord(e0, st);
+        if (x == hipSuccess)
+            x = launch_batch(s, (int)kind, tq, soff.data(), flip, sidx.as<uint32_t>(), sdist.as<float>(), cfg, st);
+        if (x == hipSuccess)
+            x = hipEventRecord(e1, st);
+        if (x == hipSuccess)
+            x = hipEventSynchronize(e1);
+        if (x == hipSuccess)
+            x = hipEventElapsedTime(ms, e0, e1);
+        return x;
+    };
+    // coordinate descent from the default: leaf size (rebuilding), then block
+    // size, stack depth and queries per wave, each over its whole range with
+    // the others held at their best so far
     const uint32_t flags = s->build_flags;
+    float best_ms = 0.0f;
+    if ((err = build_tree(s, flags, 1, st)) == hipSuccess)
+        err = measure(cur, &best_ms);
+    cur.ms_default = best_ms;
+    cur.n_tried = 1;
     const uint32_t leaves[4] = {1, 2, 4, 8}, blocks[4] = {64, 128, 256, 512}, depths[4] = {8, 16, 24, 32},
                    qpws[3] = {64, 128, 256};
-    snch_tune_result best{1, 64, 16, 64, 1e30f, 0.0f, 0};
-    hipError_t err = hipSuccess;
     for (uint32_t L : leaves) {
-        if ((err = build_tree(s, flags, L, st)) != hipSuccess)
-            break;
-        for (uint32_t B : blocks)
-            for (uint32_t D : depths)
-                for (uint32_t W : qpws) {
-                    BatchConfig cfg;
-                    cfg.block_size = B;
-                    cfg.stack_depth = D;
-                    cfg.queries_per_wave = W;
-                    cfg.coherent = true;
-                    float ms = 0.0f;
-                    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {
-                        if (rep == 1)
-                            (void)hipEventRecord(e0, st);
-                        err = launch_batch(s, (int)kind, sq.as<float>(), soff.data(), flip, sidx.as<uint32_t>(),
-                                           sdist.as<float>(), cfg, st);
-                    }
-                    (void)hipEventRecord(e1, st);
-                    if (err == hipSuccess)
-                        err = hipEventSynchronize(e1);
-                    if (err != hipSuccess)
-                        break;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    ms *= 0.5f;
-                    ++best.n_tried;
-                    if (L == 1 && B == 64 && D == 16 && W == 64)
-                        best.ms_default = ms;
-                    if (ms < best.ms) {
-                        best.ms = ms;
-                        best.leaf_size = L;
-                        best.block_size = B;
-                        best.stack_depth = D;
-                        best.queries_per_wave = W;
-                    }
+        if (err != hipSuccess || L == 1)
+            continue;
+        float ms = 0.0f;
+        snch_tune_result c = cur;
+        c.leaf_size = L;
+        if ((err = build_tree(s, flags, L, st)) == hipSuccess && (err = measure(c, &ms)) == hipSuccess) {
+            ++cur.n_tried;
+            if (ms < best_ms) {
+                best_ms = ms;
+                cur.leaf_size = L;
+            }
+        }
+    }
+    if (err == hipSuccess)
+        err = build_tree(s, flags, cur.leaf_size, st);
+    for (int axis = 0; axis < 3 && err == hipSuccess; ++axis) {
+        const uint32_t* vals = axis == 0 ? blocks : axis == 1 ? depths : qpws;
+        const int nv = axis == 2 ? 3 : 4;
+        for (int k = 0; k < nv && err == hipSuccess; ++k) {
+            snch_tune_result c = cur;
+            uint32_t& field = axis == 0 ? c.block_size : axis == 1 ? c.stack_depth : c.queries_per_wave;
+            if (field == vals[k])
+                continue;
+            field = vals[k];
+            float ms = 0.0f;
+            if ((err = measure(c, &ms)) == hipSuccess) {
+                ++cur.n_tried;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    cur.block_size = c.block_size;
+                    cur.stack_depth = c.stack_depth;
+                    cur.queries_per_wave = c.queries_per_wave;
                 }
-        if (err != hipSuccess)
-            break;
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (err == hipSuccess)
-        err = build_tree(s, flags, best.leaf_size, st);
-    if (err == hipSuccess)
         err = hipStreamSynchronize(st);
     if (err != hipSuccess)
         return hip_fail(err, "snch_batch_autotune");
-    s->leaf_size = best.leaf_size;
-    *out = best;
+    s->leaf_size = cur.leaf_size;
+    cur.ms = best_ms;
+    *out = cur;
     return SNCH_OK;
 }
 

@@ -76,6 +76,7 @@ __host__ __device__ __forceinline__ int bwd_index(int l, int n, int k)
 
 struct Frame {
     float cx, cy, cz, ihx, ihy, ihz, invD;
+    float tau; // loss asymmetry: weight 2 tau on over-prediction, 2 (1 - tau) on under-prediction
 };
 
 constexpr int ACT_STRIDE = 136; // padded bf16 row (128 + 8) against LDS bank conflicts
@@ -250,9 +251,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(const unsigned short* __restr
             dn = dn < 1.0f ? dn : 1.0f; // +inf (no silhouette) saturates
             err = o - __logf(dn + LOG_EPS);
         }
+        // expectile loss w err^2: w = 2 tau when the prediction is above the
+        // target, 2 (1 - tau) below it (tau = 0.5: the plain squared error)
+        const float wgt = 2.0f * (err > 0.0f ? fr.tau : 1.0f - fr.tau);
         if (q == 0)
-            sG[row] = 2.0f * err * inv_batch;
-        float e2 = q == 0 ? err * err : 0.0f;
+            sG[row] = 2.0f * wgt * err * inv_batch;
+        float e2 = q == 0 ? wgt * err * err : 0.0f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
             e2 += __shfl_xor(e2, o, 64);
@@ -447,7 +451,7 @@ int snch_mlp_create(snch_context* ctx, const float* lo, const float* hi, float s
         d2 += (double)(hi[k] - lo[k]) * (hi[k] - lo[k]);
     }
     m->D = (float)std::sqrt(d2);
-    m->fr = Frame{c[0], c[1], c[2], ih[0], ih[1], ih[2], 1.0f / m->D};
+    m->fr = Frame{c[0], c[1], c[2], ih[0], ih[1], ih[2], 1.0f / m->D, 0.5f};
     // Fourier matrix 2 pi B, B ~ N(0, sigma^2) (Box-Muller), and He-uniform weights
     uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1;
     m->h_fourier.resize(3 * NFREQ);
@@ -512,6 +516,16 @@ int snch_mlp_params(snch_mlp* m, float* params_out, const float* params_in, floa
         const float f[8] = {m->fr.cx, m->fr.cy, m->fr.cz, m->fr.ihx, m->fr.ihy, m->fr.ihz, m->fr.invD, m->D};
         std::memcpy(frame_out, f, sizeof(f));
     }
+    return SNCH_OK;
+}
+
+int snch_mlp_set_loss(snch_mlp* m, float tau)
+{
+    if (!m || !(tau > 0.0f && tau < 1.0f)) {
+        set_error("snch_mlp_set_loss: tau must lie in (0, 1)");
+        return SNCH_ERR_INVALID;
+    }
+    m->fr.tau = tau;
     return SNCH_OK;
 }
 

@@ -28,14 +28,15 @@ def _data(n, seed=0):
     return lo.astype(np.float32), hi.astype(np.float32), pts, d
 
 
-def test_gradient_matches_fp64(S):
+@pytest.mark.parametrize("tau", [0.5, 0.85])
+def test_gradient_matches_fp64(S, tau):
     from oracle import mlp_oracle as M
 
     lo, hi, pts, d = _data(4096)
-    mlp = S.LearnedBound(lo, hi, sigma=4.0, seed=3)
+    mlp = S.LearnedBound(lo, hi, sigma=4.0, seed=3).set_loss(tau)
     th, four, frame = mlp.params()
     g, loss = mlp.gradient(pts, d)
-    lref, gref = M.loss_and_grad(th, pts, d, four, frame, bf16_weights=True)
+    lref, gref = M.loss_and_grad(th, pts, d, four, frame, bf16_weights=True, tau=tau)
     assert abs(loss - lref) <= 2e-2 * lref
     blocks = [(M.OFF_W[l], M.OFF_W[l] + M.H * M.KIN[l]) for l in range(4)] + \
              [(M.OFF_B[l], M.OFF_B[l] + M.H) for l in range(4)] + [(M.OFF_W5, M.OFF_B5 + 1)]
