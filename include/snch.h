@@ -219,7 +219,7 @@ uint32_t snch_mlp_num_params(void);
 int snch_mlp_create(snch_context *ctx, const float *lo, const float *hi, float sigma, uint64_t seed, snch_mlp **out);
 int snch_mlp_destroy(snch_mlp *mlp);
 /* host copies: params_out/params_in (snch_mlp_num_params() floats, FP32 master
- * weights), fourier_out (32 x 3, already scaled by 2 pi), frame_out (8 floats:
+ * weights), fourier_out (B, 32 x 3: features sin(B xhat), cos(B xhat)), frame_out (8 floats:
  * centre xyz, 1/half-extent xyz, 1/D, D).  Any may be NULL. */
 int snch_mlp_params(snch_mlp *mlp, float *params_out, const float *params_in, float *fourier_out, float *frame_out);
 /* Adam training on device points (n x 3) and exact distances (n); batch must be
@@ -228,7 +228,7 @@ int snch_mlp_params(snch_mlp *mlp, float *params_out, const float *params_in, fl
 int snch_mlp_train(snch_mlp *mlp, const float *pts, const float *dist, uint32_t n, uint32_t batch, uint32_t epochs,
                    float lr, float *epoch_loss, void *stream);
 /* Loss: w (o - t)^2 with t = log(min(d / D, 1) + 1e-3), w = 2 tau when the
- * prediction o is above t and 2 (1 - tau) below it (an expectile; tau = 0.5,
+ * prediction o is below t and 2 (1 - tau) above it (an expectile; tau = 0.5,
  * the default, is the squared error).  tau > 0.5 trains an upper bound. */
 int snch_mlp_set_loss(snch_mlp *mlp, float tau);
 /* Gradient of the mean loss over n points (host output, FP32), for checking. */

@@ -4,9 +4,11 @@ distance-bound MLP, for checking the GPU kernels in snch_lbvh_amd/csrc/mlp.hip.
 Parameter layout (flat FP32 vector, snch_mlp_num_params() entries):
   W1 128x64 @0, b1 @8192, W2 128x128 @8320, b2 @24704, W3 @24832, b3 @41216,
   W4 @41344, b4 @57728, w5 128 @57856, b5 @57984.
-Network: f = [sin(F xhat), cos(F xhat)] with F = 2 pi B (32 x 3) and
-xhat = (x - c) * ih; z_l = W_l a_{l-1} + b_l; a_l = silu(z_l); o = w5 . a4 + b5;
-loss = mean((o - log(min(d / D, 1) + 1e-3))^2).
+Network: f = [sin(B xhat), cos(B xhat)] with B the fixed 32 x 3 Fourier
+matrix the library reports and xhat = (x - c) * ih; z_l = W_l a_{l-1} + b_l;
+a_l = silu(z_l); o = w5 . a4 + b5; with t = log(min(d / D, 1) + 1e-3) and
+err = o - t, loss = mean(w err^2), w = 2 tau if err < 0 else 2 (1 - tau)
+(tau = 0.5: the squared error).
 The backward pass is pinned by central finite differences in tests."""
 import numpy as np
 
@@ -71,7 +73,7 @@ def loss_and_grad(theta, x, d, fourier, frame, bf16_weights=False, tau=0.5):
     t = target(d, frame)
     n = len(o)
     err = o - t
-    w = 2.0 * np.where(err > 0, tau, 1.0 - tau)  # expectile weights (tau = 0.5: squared error)
+    w = 2.0 * np.where(err < 0, tau, 1.0 - tau)  # expectile weights (tau = 0.5: squared error)
     loss = np.mean(w * err ** 2)
     g = np.zeros(NPARAM)
     go = 2.0 * w * err / n

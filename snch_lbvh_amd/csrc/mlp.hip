@@ -1,7 +1,8 @@
 // Learned distance bound: a Fourier-feature MLP on BF16 MFMA (gfx950).
 //
 //   x -> xhat = (x - c) / h              (scene box -> [-1, 1]^3)
-//   f = [sin(B xhat), cos(B xhat)]       (B: 32 x 3, N(0, sigma^2) * 2 pi, fixed)
+//   f = [sin(B xhat), cos(B xhat)]       (B: 32 x 3, N(0, sigma^2) radians per
+//                                         normalised unit, fixed)
 //   z1 = W1 f + b1, a1 = silu(z1)        (64 -> 128)
 //   z2..z4 likewise                      (128 -> 128, three layers)
 //   o = w5 . a4 + b5                     (128 -> 1)
@@ -76,7 +77,7 @@ __host__ __device__ __forceinline__ int bwd_index(int l, int n, int k)
 
 struct Frame {
     float cx, cy, cz, ihx, ihy, ihz, invD;
-    float tau; // loss asymmetry: weight 2 tau on over-prediction, 2 (1 - tau) on under-prediction
+    float tau; // loss asymmetry: weight 2 tau on under-prediction, 2 (1 - tau) on over-prediction
 };
 
 constexpr int ACT_STRIDE = 136; // padded bf16 row (128 + 8) against LDS bank conflicts
@@ -251,9 +252,10 @@ __global__ __launch_bounds__(256) void k_mlp_train(const unsigned short* __restr
             dn = dn < 1.0f ? dn : 1.0f; // +inf (no silhouette) saturates
             err = o - __logf(dn + LOG_EPS);
         }
-        // expectile loss w err^2: w = 2 tau when the prediction is above the
-        // target, 2 (1 - tau) below it (tau = 0.5: the plain squared error)
-        const float wgt = 2.0f * (err > 0.0f ? fr.tau : 1.0f - fr.tau);
+        // expectile loss w err^2: w = 2 tau when the prediction is below the
+        // target, 2 (1 - tau) above it (tau = 0.5: the plain squared error;
+        // tau > 0.5 pushes the prediction up, towards an upper bound)
+        const float wgt = 2.0f * (err < 0.0f ? fr.tau : 1.0f - fr.tau);
         if (q == 0)
             sG[row] = 2.0f * wgt * err * inv_batch;
         float e2 = q == 0 ? wgt * err * err : 0.0f;
@@ -452,12 +454,12 @@ int snch_mlp_create(snch_context* ctx, const float* lo, const float* hi, float s
     }
     m->D = (float)std::sqrt(d2);
     m->fr = Frame{c[0], c[1], c[2], ih[0], ih[1], ih[2], 1.0f / m->D, 0.5f};
-    // Fourier matrix 2 pi B, B ~ N(0, sigma^2) (Box-Muller), and He-uniform weights
+    // Fourier matrix B ~ N(0, sigma^2) (Box-Muller), and He-uniform weights
     uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1;
     m->h_fourier.resize(3 * NFREQ);
     for (auto& v : m->h_fourier) {
         const double g = std::sqrt(-2.0 * std::log(u01(s))) * std::cos(2.0 * M_PI * u01(s));
-        v = (float)(2.0 * M_PI * sigma * g);
+        v = (float)(sigma * g);
     }
     std::vector<float> p(NPARAM, 0.0f);
     for (int l = 0; l < 4; ++l) {
