@@ -85,7 +85,8 @@ int snch_context_destroy(snch_context *ctx);
 /* scene<3>(vertices, indices) + compute_silhouettes() (scene.cuh:1128-1204).
  * V: nV x 3 floats, F: nF x 3 int32 (host).  Builds the edge table with the
  * reference's edge ids (first appearance over triangle i, local edge
- * (v_j, v_{j+1})), edge slots (opp0, lo, hi, opp1) and edge ownership. */
+ * (v_j, v_{j+1})), edge slots (opp0, lo, hi, opp1) and edge ownership.
+ * At most 2^24 triangles (segments for scene<2>). */
 int snch_scene_create_tri(snch_context *ctx, const float *V, uint32_t nV, const int32_t *F, uint32_t nF,
                           snch_scene **out);
 /* scene<2>(vertices, indices) + compute_silhouettes() (scene.cuh:624-652).
@@ -96,7 +97,8 @@ int snch_scene_create_seg(snch_context *ctx, const float *V, uint32_t nV, const 
  * counterpart: the reference builds one lbvh::scene<3> per mesh,
  * scene.cuh:1128-1229).  V: vertices of all meshes, concatenated (v_off[m] ..
  * v_off[m+1] - 1 belong to mesh m); F: triangles of all meshes, concatenated
- * (f_off[m] .. f_off[m+1] - 1), with vertex indices LOCAL to the mesh.
+ * (f_off[m] .. f_off[m+1] - 1), with vertex indices LOCAL to the mesh; at
+ * most 2^24 triangles per mesh (as for snch_scene_create_tri).
  * v_off and f_off hold n_meshes + 1 entries starting at 0 (host).  Each mesh
  * gets its own edge table, scene box and tree, identical to those of the
  * mesh built alone; one build and one query launch cover all meshes. */

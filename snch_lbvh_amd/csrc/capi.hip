@@ -199,6 +199,11 @@ snch::TreeView snch_scene::view(uint32_t m) const
     t.tcone = tcone.as<float4>() + 4 * (size_t)(off - m);
     // the trig-free cone test is conservative only with the enclosing radius
     t.flags = (dim == 3 && !(build_flags & SNCH_BUILD_REFERENCE_RADIUS)) ? TREE_FAST_CONE : 0u;
+    uint32_t vd = 0;
+    const float d2 = m < mesh_diag2.size() ? mesh_diag2[m] : 0.0f;
+    std::memcpy(&vd, &d2, sizeof(vd));
+    vd >>= 20;
+    t.qbase = vd > 247u ? vd - 247u : 0u;
     t.tleaf = tleaf.as<float4>() + 3 * (size_t)off;
     t.tedge = tedge.as<float4>() + 4 * (size_t)eoff;
     t.nbox = nbox.as<float4>() + 2 * (size_t)(2 * off - m);
@@ -277,8 +282,8 @@ int snch_scene_create_tri_batch(snch_context* ctx, uint32_t n_meshes, const floa
             return SNCH_ERR_EMPTY;
         }
         SNCH_CHECK_ARG(v_off[m + 1] > v_off[m], "snch_scene_create_tri_batch: mesh without vertices");
-        SNCH_CHECK_ARG(f_off[m + 1] - f_off[m] <= RUN_FIRST_MASK,
-                       "snch_scene_create_tri_batch: too many triangles in one mesh (limit 2^27 - 1)");
+        SNCH_CHECK_ARG(f_off[m + 1] - f_off[m] <= MAX_MESH_PRIMS,
+                       "snch_scene_create_tri_batch: too many triangles in one mesh (limit 2^24)");
     }
     const uint32_t nV = v_off[n_meshes], nF = f_off[n_meshes];
     SNCH_CHECK_ARG(V && F, "snch_scene_create_tri_batch: null vertex or index array");
@@ -301,6 +306,19 @@ int snch_scene_create_tri_batch(snch_context* ctx, uint32_t n_meshes, const floa
     s->mesh_voff.assign(v_off, v_off + n_meshes + 1);
     s->mesh_toff.assign(1, 0);
     s->mesh_eoff.assign(1, 0);
+    s->mesh_diag2.assign(n_meshes, 0.0f);
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t k = v_off[m]; k < v_off[m + 1]; ++k)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], V[3 * k + c]);
+                hi[c] = std::max(hi[c], V[3 * k + c]);
+            }
+        double d2 = 0.0;
+        for (int c = 0; c < 3; ++c)
+            d2 += (double)(hi[c] - lo[c]) * (hi[c] - lo[c]);
+        s->mesh_diag2[m] = (float)d2;
+    }
     s->F.resize(3 * (size_t)nF);
     s->estart.assign(1, 0);
     std::vector<int32_t> slots_g;
@@ -341,7 +359,7 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
         return SNCH_ERR_EMPTY;
     }
     SNCH_CHECK_ARG(V && S && nV > 0, "snch_scene_create_seg: null vertex or index array");
-    SNCH_CHECK_ARG(nS <= RUN_FIRST_MASK, "snch_scene_create_seg: too many segments (limit 2^27 - 1)");
+    SNCH_CHECK_ARG(nS <= MAX_MESH_PRIMS, "snch_scene_create_seg: too many segments (limit 2^24)");
     for (size_t k = 0; k < 2 * (size_t)nV; ++k)
         SNCH_CHECK_ARG(std::isfinite(V[k]), "snch_scene_create_seg: non-finite vertex coordinate");
     for (size_t k = 0; k < 2 * (size_t)nS; ++k)
@@ -387,6 +405,15 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
     s->mesh_voff = {0, nV};
     s->mesh_toff = {0, nV};
     s->mesh_eoff = {0, next};
+    {
+        float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+        for (size_t k = 0; k < nV; ++k)
+            for (int c = 0; c < 2; ++c) {
+                lo[c] = std::min(lo[c], V[2 * k + c]);
+                hi[c] = std::max(hi[c], V[2 * k + c]);
+            }
+        s->mesh_diag2 = {(float)((double)(hi[0] - lo[0]) * (hi[0] - lo[0]) + (double)(hi[1] - lo[1]) * (hi[1] - lo[1]))};
+    }
     std::vector<float> Vv(V, V + 2 * (size_t)nV);
     const hipError_t e = upload_scene(s, 2, Vv, s->F, internal_slots, s->estart);
     if (e != hipSuccess) {

@@ -51,6 +51,7 @@ constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t RUN_SHIFT = 27;
 constexpr uint32_t RUN_FIRST_MASK = (1u << RUN_SHIFT) - 1u;
 constexpr uint32_t MAX_LEAF_SIZE = 8;
+constexpr uint32_t MAX_MESH_PRIMS = 1u << 24; // node ids fit the 24-bit field of a packed stack entry
 
 struct MeshDesc {
     uint32_t off, n, eoff, nb;
@@ -61,6 +62,8 @@ constexpr uint32_t TREE_FAST_CONE = 1u; // TreeView::flags: cone_overlap_fast ma
 struct TreeView {
     uint32_t n_leaves;
     uint32_t flags;
+    uint32_t qbase; // traversal-stack key quantisation base (bits >> 20 of D^2, less 247)
+    uint32_t pad_;
     const float4* __restrict__ tnode;
     const float4* __restrict__ tcone;
     const float4* __restrict__ tleaf;
@@ -117,6 +120,7 @@ struct snch_scene {
     // per-mesh offsets (n_meshes + 1 each): primitives, vertices, silhouette
     // table entries, packed (owned) silhouette records
     std::vector<uint32_t> mesh_off, mesh_voff, mesh_toff, mesh_eoff;
+    std::vector<float> mesh_diag2; // squared diagonal of each mesh's vertex box
     snch::DeviceBuffer dMesh, dPmesh; // MeshDesc per mesh; mesh id per primitive (batches only)
     uint32_t leaf_size = 1;
     uint32_t nV = 0, nF = 0, nE = 0; // nE: silhouette table size (edges in 3D, vertices in 2D)

@@ -45,16 +45,19 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
 
 // per-lane traversal stack: the top SSZ entries in LDS (a column per lane, so
 // the lanes of a wave hit distinct banks), older entries spilled to a private
-// array; the tree depth is at most 64, so the stack never loses an entry
+// array; the tree depth is at most 64, so the stack never loses an entry.
+// An entry is {node, key} (8 B), key being the lower bound used to skip the
+// entry when it is popped.
 template <int BS, int SSZ>
 struct LaneStack {
-    uint2* lds; // this lane's column: entry s at lds[s * BS]
+    using Entry = uint2;
+    Entry* lds; // this lane's column: entry s at lds[s * BS]
     int sp;
     uint32_t ovn[OVF];
     float ovd[OVF];
     bool overflow;
 
-    __device__ __forceinline__ void init(uint2* base)
+    __device__ __forceinline__ void init(Entry* base)
     {
         lds = base;
         sp = 0;
@@ -88,6 +91,65 @@ struct LaneStack {
             const int o = sp - SSZ;
             lds[s * BS] = make_uint2(ovn[o], __float_as_uint(ovd[o]));
         }
+    }
+};
+
+// The same stack with 4-B entries, node << 8 | q, for trees of at most 2^24
+// leaves (internal node ids < 2^24).  q is an 8-bit lower bound of the key (a
+// non-negative squared distance): the key's float bits >> 20 (exponent and 3
+// mantissa bits, i.e. the key truncated to 8 steps per octave) minus a base
+// chosen per tree so that the 255 levels span 31 octaves below the squared
+// scene diagonal; keys below that range store 0 (decoded as 0), keys above
+// saturate at 255, so the decoded value never exceeds the key and a popped
+// entry is never skipped wrongly.  Half the LDS and spill traffic of uint2
+// entries.
+template <int BS, int SSZ>
+struct PackedStack {
+    using Entry = uint32_t;
+    Entry* lds;
+    int sp;
+    uint32_t ov[OVF];
+    bool overflow;
+    uint32_t qbase;
+
+    __device__ __forceinline__ void init(Entry* base, uint32_t qb)
+    {
+        lds = base;
+        sp = 0;
+        overflow = false;
+        qbase = qb;
+    }
+    __device__ __forceinline__ uint32_t enc(uint32_t n, float d) const
+    {
+        const uint32_t v = __float_as_uint(d) >> 20;
+        const uint32_t q = v > qbase ? min(v - qbase, 255u) : 0u;
+        return (n << 8) | q;
+    }
+    __device__ __forceinline__ void push(uint32_t n, float d)
+    {
+        const int s = sp % SSZ;
+        if (sp >= SSZ) {
+            const int o = sp - SSZ;
+            if (o < OVF) {
+                ov[o] = lds[s * BS];
+            } else {
+                overflow = true;
+                return;
+            }
+        }
+        lds[s * BS] = enc(n, d);
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(uint32_t& n, float& d)
+    {
+        --sp;
+        const int s = sp % SSZ;
+        const uint32_t e = lds[s * BS];
+        n = e >> 8;
+        const uint32_t q = e & 255u;
+        d = q ? __uint_as_float((q + qbase) << 20) : 0.0f;
+        if (sp >= SSZ)
+            lds[s * BS] = ov[sp - SSZ];
     }
 };
 
@@ -188,8 +250,8 @@ __device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, flo
 // query_device(nearest) (query.cuh:238-318) for one query point; best is the
 // squared distance
 template <int BS, int SSZ, bool STATS, int DIM>
-__device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint2* stk_col, float& best, uint32_t& best_idx,
-                                         Counts& cnt, bool& overflow)
+__device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint32_t* stk_col, float& best,
+                                         uint32_t& best_idx, Counts& cnt, bool& overflow)
 {
     best = F_INF;
     best_idx = INVALID;
@@ -197,8 +259,8 @@ __device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint2* stk_col
         cp_leaf<DIM>(t, 0, p, best, best_idx);
         return;
     }
-    LaneStack<BS, SSZ> st;
-    st.init(stk_col);
+    PackedStack<BS, SSZ> st;
+    st.init(stk_col, t.qbase);
     uint32_t node = 0;
     // each internal node is visited at most once; the cap and the range
     // checks only guard against a malformed tree
@@ -261,7 +323,7 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
                                                       const uint32_t* __restrict__ perm, uint32_t* __restrict__ err,
                                                       unsigned long long* stats)
 {
-    __shared__ uint2 stk[SSZ * BS];
+    __shared__ uint32_t stk[SSZ * BS];
     const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -351,7 +413,7 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 // initial bound.  With index tie-breaking the result is bit-identical to the
 // unseeded search.
 template <int BS, int SSZ, bool STATS, int DIM, bool SEEDED, bool FALLBACK = true>
-__device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint2* stk_col,
+__device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
                                           float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
 {
     best = F_INF;
@@ -365,9 +427,9 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     }
     if (!SEEDED || !(cap >= 0.0f))
         cap = F_INF;
-    LaneStack<BS, SSZ> st;
+    PackedStack<BS, SSZ> st;
     for (int pass = 0; pass < (SEEDED && FALLBACK ? 2 : 1); ++pass) {
-        st.init(stk_col);
+        st.init(stk_col, t.qbase);
         const float cap2 = cap * cap;
         uint32_t node = 0;
         const uint32_t cap_iters = 2 * t.n_leaves + 64;
@@ -446,7 +508,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
                                                            const uint32_t* __restrict__ perm,
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
-    __shared__ uint2 stk[SSZ * BS];
+    __shared__ uint32_t stk[SSZ * BS];
     const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -629,7 +691,7 @@ __global__ __launch_bounds__(BS) void k_batch(BatchTable bt, const float* __rest
                                               uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
                                               uint32_t* __restrict__ err)
 {
-    __shared__ uint2 stk[SSZ * BS];
+    __shared__ uint32_t stk[SSZ * BS];
     const uint32_t lane = threadIdx.x & 63;
     bool ovf = false;
     Counts cnt;
