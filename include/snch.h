@@ -65,8 +65,8 @@ typedef struct snch_query_opts {
     const uint32_t *order; /* optional DEVICE permutation of [0, n) (e.g. from
                               snch_query_order): the visiting order; overrides coherent */
     uint32_t stack_depth;      /* LDS stack entries per lane: closest point 8 or 16 (0 = 16),
-                                  closest silhouette (3D) 8, 16, 24 or 32 (0 = 8), batched
-                                  queries 8, 16, 24 or 32 (0 = 16) */
+                                  closest silhouette (3D) and batched queries 8, 16, 24 or
+                                  32 (0 = 16) */
     uint32_t queries_per_wave; /* batched queries: queries a wavefront takes from the work
                                   queue at a time, a multiple of 64 up to 4096 (0 = 64) */
     uint32_t seed_is_cap;      /* closest silhouette: 1 = seed_radius is a hard search radius

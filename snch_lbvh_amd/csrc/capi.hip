@@ -564,7 +564,7 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     SNCH_NEED_SINGLE(s, "snch_closest_silhouette");
     SNCH_CHECK_ARG(n == 0 || (q && out_dist), "snch_closest_silhouette: null pointer");
     uint32_t block = opts && opts->block_size ? opts->block_size : 64u;
-    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 8u;
+    const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_silhouette: block_size must be 64, 128, 256 or 512");
     SNCH_CHECK_ARG(depth == 8 || depth == 16 || depth == 24 || depth == 32,
@@ -574,8 +574,14 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     if (!perm && want_order(opts, n))
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, opts && opts->seed_is_cap,
-                                       out_sil, out_dist, block, depth, perm, opts ? opts->stats : nullptr, st),
+    const bool capped = opts && opts->seed_is_cap;
+    uint32_t* fail_buf = nullptr;
+    if (seed && !capped) {
+        SNCH_HIP(s->qscratch.fail.ensure(sizeof(uint32_t) * ((size_t)n + 1)), "snch_closest_silhouette: scratch");
+        fail_buf = s->qscratch.fail.as<uint32_t>();
+    }
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, fail_buf, out_sil, out_dist,
+                                       block, depth, perm, opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
     return SNCH_OK;
 }
@@ -781,11 +787,12 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
     for (int axis = 0; axis < 3 && err == hipSuccess; ++axis) {
         const uint32_t* vals = axis == 0 ? blocks : axis == 1 ? depths : qpws;
         const int nv = axis == 2 ? 3 : 4;
+        const uint32_t start = axis == 0 ? cur.block_size : axis == 1 ? cur.stack_depth : cur.queries_per_wave;
         for (int k = 0; k < nv && err == hipSuccess; ++k) {
+            if (vals[k] == start)
+                continue; // already measured
             snch_tune_result c = cur;
             uint32_t& field = axis == 0 ? c.block_size : axis == 1 ? c.stack_depth : c.queries_per_wave;
-            if (field == vals[k])
-                continue;
             field = vals[k];
             float ms = 0.0f;
             if ((err = measure(c, &ms)) == hipSuccess) {

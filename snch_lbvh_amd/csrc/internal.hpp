@@ -85,7 +85,7 @@ struct DeviceBuffer {
 
 // per-scene scratch of the query reordering (Morton sort of query points)
 struct QueryScratch {
-    DeviceBuffer codes, idx, codes_alt, idx_alt, hist;
+    DeviceBuffer codes, idx, codes_alt, idx_alt, hist, fail;
 };
 
 // per-scene state of batched queries: device copies of the per-mesh views
@@ -164,11 +164,15 @@ hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, Q
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
                                 uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
                                 hipStream_t st);
-// capped: seed is a hard radius (answers beyond it are not searched; the
-// returned distance is then larger than the radius, possibly +inf)
+// seed (nullable): per-query radius.  capped: a hard radius (answers beyond
+// it are not searched; the returned distance is then larger than the
+// radius, possibly +inf).  Otherwise a hint: a capped pass, then an uncapped
+// pass over the queries it could not settle, listed in fail_buf (n + 1
+// entries, the last one the count); results equal the unseeded search.
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
-                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st);
+                                     const float* seed, bool capped, uint32_t* fail_buf, uint32_t* sil, float* dist,
+                                     uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
+                                     hipStream_t st);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

@@ -406,18 +406,15 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 }
 
 // query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
-// A seeded radius `cap` (SEEDED) caps the search: nodes farther than cap are
-// skipped.  A silhouette found at distance <= cap is the global answer,
-// because every node that could hold one at least as close was visited;
-// otherwise the search reruns without the cap, keeping what it found as the
-// initial bound.  With index tie-breaking the result is bit-identical to the
-// unseeded search.
-template <int BS, int SSZ, bool STATS, int DIM, bool SEEDED, bool FALLBACK = true>
+// best / best_e hold the initial bound and its silhouette (+inf / INVALID for
+// a fresh search, or a candidate found earlier) and receive the answer.  A
+// radius `cap` (finite) skips nodes farther than cap; a silhouette found at
+// distance <= cap is then the global answer, because every node that could
+// hold one at least as close was visited.
+template <int BS, int SSZ, bool STATS, int DIM>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
                                           float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
 {
-    best = F_INF;
-    best_e = INVALID;
     if (t.n_leaves == 1) {
         const box3 b = ldbox2(t.nbox, 0);
         const float4 c0 = t.ncone[0], c1 = t.ncone[1];
@@ -425,87 +422,89 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
             sil_leaf<DIM>(t, 0, p, flip, best, best_e, cnt);
         return;
     }
-    if (!SEEDED || !(cap >= 0.0f))
+    if (!(cap >= 0.0f))
         cap = F_INF;
     PackedStack<BS, SSZ> st;
-    for (int pass = 0; pass < (SEEDED && FALLBACK ? 2 : 1); ++pass) {
-        st.init(stk_col, t.qbase);
-        const float cap2 = cap * cap;
-        uint32_t node = 0;
-        const uint32_t cap_iters = 2 * t.n_leaves + 64;
-        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
-            const NodeRec nr = load_node(t.tnode, node);
-            const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
-            const float b2 = best * best;
-            bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
-            if (STATS) {
-                cnt.node += 1;
-                cnt.box += 2;
-            }
-            if (hL || hR) {
-                if (STATS)
-                    cnt.cone += 1;
-                const float4 CL = t.tcone[4 * node], CR = t.tcone[4 * node + 1], Cr = t.tcone[4 * node + 2];
-                // query.cuh:366-367
-                if (DIM == 3 && (t.flags & TREE_FAST_CONE)) {
-                    const float4 SC = t.tcone[4 * node + 3];
-                    hL = hL && cone_valid(CL.w) &&
-                         cone_overlap_fast(mk(CL.x, CL.y, CL.z), CL.w, SC.x, SC.y, Cr.x, p, nr.L, Lmd);
-                    hR = hR && cone_valid(CR.w) &&
-                         cone_overlap_fast(mk(CR.x, CR.y, CR.z), CR.w, SC.z, SC.w, Cr.y, p, nr.R, Rmd);
-                } else {
-                    hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
-                    hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
-                }
-            }
-            uint32_t lf, lc, rf, rc;
-            const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
-            if (hL && lLeaf)
-                for (uint32_t k = 0; k < lc; ++k)
-                    sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
-            if (hR && rLeaf)
-                for (uint32_t k = 0; k < rc; ++k)
-                    sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
-            const float nb2 = best * best;
-            const bool gL = hL && !lLeaf && Lmd <= nb2;
-            const bool gR = hR && !rLeaf && Rmd <= nb2;
-            if (gL && gR) {
-                const bool lfirst = Lmd <= Rmd;
-                st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
-                node = lfirst ? nr.lref : nr.rref;
-                continue;
-            }
-            if (gL || gR) {
-                node = gL ? nr.lref : nr.rref;
-                continue;
-            }
-            bool found = false;
-            while (st.sp > 0) {
-                uint32_t nn;
-                float dd;
-                st.pop(nn, dd);
-                if (dd <= best * best) { // query.cuh:347
-                    node = nn;
-                    found = true;
-                    break;
-                }
-            }
-            if (!found)
-                break;
+    st.init(stk_col, t.qbase);
+    const float cap2 = cap * cap;
+    uint32_t node = 0;
+    const uint32_t cap_iters = 2 * t.n_leaves + 64;
+    for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+        const NodeRec nr = load_node(t.tnode, node);
+        const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
+        const float b2 = best * best;
+        bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
+        if (STATS) {
+            cnt.node += 1;
+            cnt.box += 2;
         }
-        overflow = overflow || st.overflow;
-        // every candidate within `best` was examined when best <= cap
-        if (cap == F_INF || (best_e != INVALID && best <= cap))
+        if (hL || hR) {
+            if (STATS)
+                cnt.cone += 1;
+            const float4 CL = t.tcone[4 * node], CR = t.tcone[4 * node + 1], Cr = t.tcone[4 * node + 2];
+            // query.cuh:366-367
+            if (DIM == 3 && (t.flags & TREE_FAST_CONE)) {
+                const float4 SC = t.tcone[4 * node + 3];
+                hL = hL && cone_valid(CL.w) &&
+                     cone_overlap_fast(mk(CL.x, CL.y, CL.z), CL.w, SC.x, SC.y, Cr.x, p, nr.L, Lmd);
+                hR = hR && cone_valid(CR.w) &&
+                     cone_overlap_fast(mk(CR.x, CR.y, CR.z), CR.w, SC.z, SC.w, Cr.y, p, nr.R, Rmd);
+            } else {
+                hL = hL && cone_valid(CL.w) && overlap_dim<DIM>(CL, Cr.x, p, nr.L, Lmd);
+                hR = hR && cone_valid(CR.w) && overlap_dim<DIM>(CR, Cr.y, p, nr.R, Rmd);
+            }
+        }
+        uint32_t lf, lc, rf, rc;
+        const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
+        if (hL && lLeaf)
+            for (uint32_t k = 0; k < lc; ++k)
+                sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
+        if (hR && rLeaf)
+            for (uint32_t k = 0; k < rc; ++k)
+                sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
+        const float nb2 = best * best;
+        const bool gL = hL && !lLeaf && Lmd <= nb2;
+        const bool gR = hR && !rLeaf && Rmd <= nb2;
+        if (gL && gR) {
+            const bool lfirst = Lmd <= Rmd;
+            st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
+            node = lfirst ? nr.lref : nr.rref;
+            continue;
+        }
+        if (gL || gR) {
+            node = gL ? nr.lref : nr.rref;
+            continue;
+        }
+        bool found = false;
+        while (st.sp > 0) {
+            uint32_t nn;
+            float dd;
+            st.pop(nn, dd);
+            if (dd <= best * best) { // query.cuh:347
+                node = nn;
+                found = true;
+                break;
+            }
+        }
+        if (!found)
             break;
-        cap = F_INF; // nothing within the seeded radius: exact fallback
     }
+    overflow = overflow || st.overflow;
 }
 
-template <int BS, bool STATS, int DIM, bool SEEDED, int SSZ, bool FALLBACK = true>
+// MODE 0: plain search.  MODE 2: search within the radius seed[qi] only
+// (a hard cap; with `fail` set, queries whose answer was not found within the
+// radius are appended to fail[0 .. *fail_count)).  MODE 3: continue a capped
+// search without the cap, starting from the candidate it left in out_d /
+// out_e.  A seeded search is MODE 2 with a fail list, then MODE 3 over that
+// list: bit-identical to MODE 0, and the few queries that need the second
+// pass run together instead of stalling their waves.
+template <int BS, bool STATS, int DIM, int MODE, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
-                                                           int flip, const float* __restrict__ seed,
+                                                           uint32_t nq, int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
                                                            const uint32_t* __restrict__ perm,
+                                                           uint32_t* __restrict__ fail, uint32_t* __restrict__ fail_count,
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
     __shared__ uint32_t stk[SSZ * BS];
@@ -513,19 +512,31 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     Counts cnt;
     if (t.n_dev)
         n = min(n, *t.n_dev);
-    if (slot >= n) {
+    const uint32_t qi = slot < n ? (perm ? perm[slot] : slot) : nq;
+    if (qi >= nq) {
         if (STATS)
             flush_counts(cnt, stats);
         return;
     }
-    const uint32_t qi = perm ? min(perm[slot], n - 1) : slot;
-    float best;
-    uint32_t best_e;
+    float cap = MODE == 2 ? seed[qi] : F_INF;
+    if (!(cap >= 0.0f))
+        cap = F_INF;
+    float best = F_INF;
+    uint32_t best_e = INVALID;
+    if (MODE == 3) {
+        best = out_d[qi];
+        best_e = out_e ? out_e[qi] : INVALID;
+    }
     bool ovf = false;
-    sil_query<BS, SSZ, STATS, DIM, SEEDED, FALLBACK>(t, load_point<DIM>(q, qi), flip != 0, SEEDED ? seed[qi] : F_INF,
-                                          stk + threadIdx.x, best, best_e, cnt, ovf);
+    sil_query<BS, SSZ, STATS, DIM>(t, load_point<DIM>(q, qi), flip != 0, cap, stk + threadIdx.x, best, best_e, cnt,
+                                   ovf);
     if (ovf && err)
         atomicOr(err, 1u);
+    if (MODE == 2 && fail && cap != F_INF && !(best_e != INVALID && best <= cap)) {
+        const uint32_t k = atomicAdd(fail_count, 1u);
+        if (k < nq)
+            fail[k] = qi;
+    }
     if (out_e)
         out_e[qi] = best_e;
     out_d[qi] = best;
@@ -720,13 +731,13 @@ __global__ __launch_bounds__(BS) void k_batch(BatchTable bt, const float* __rest
                 continue;
             const uint32_t qi = bt.perm ? bt.perm[slot] : slot;
             const v3 p = load_point<3>(q, qi);
-            float best;
-            uint32_t bi;
+            float best = F_INF;
+            uint32_t bi = INVALID;
             if (KIND == 0) {
                 cp_query<BS, SSZ, false, 3>(tv, p, stk + threadIdx.x, best, bi, cnt, ovf);
                 best = dm::sqrt_(best);
             } else {
-                sil_query<BS, SSZ, false, 3, false>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
+                sil_query<BS, SSZ, false, 3>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
             }
             if (out_idx)
                 out_idx[qi] = bi;
@@ -985,65 +996,70 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
 
 template <int BS, int DIM, int SSZ>
 static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
-                           uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
-                           hipStream_t st)
+                           uint32_t* fail_buf, uint32_t* sil, float* dist, const uint32_t* perm,
+                           unsigned long long* stats, hipStream_t st)
 {
     const dim3 g(nblocks(n, BS)), b(BS);
     uint32_t* err = traversal_error_word();
-    if (capped && seed) {
+#define SNCH_SIL(S_, M_, ...) \
+    hipLaunchKernelGGL((k_closest_silhouette<BS, S_, DIM, M_, SSZ>), g, b, 0, st, __VA_ARGS__)
+    if (!seed) {
         if (stats)
-            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true, SSZ, false>), g, b, 0, st, t, q, n, flip,
-                               seed, sil, dist, perm, err, stats);
+            SNCH_SIL(true, 0, t, q, n, n, flip, seed, sil, dist, perm, nullptr, nullptr, err, stats);
         else
-            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true, SSZ, false>), g, b, 0, st, t, q, n, flip,
-                               seed, sil, dist, perm, err, stats);
+            SNCH_SIL(false, 0, t, q, n, n, flip, seed, sil, dist, perm, nullptr, nullptr, err, stats);
         return;
     }
-    if (stats) {
-        if (seed)
-            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, true, SSZ>), g, b, 0, st, t, q, n, flip, seed, sil,
-                               dist, perm, err, stats);
-        else
-            hipLaunchKernelGGL((k_closest_silhouette<BS, true, DIM, false, SSZ>), g, b, 0, st, t, q, n, flip, seed,
-                               sil, dist, perm, err, stats);
-    } else {
-        if (seed)
-            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, true, SSZ>), g, b, 0, st, t, q, n, flip, seed,
-                               sil, dist, perm, err, stats);
-        else
-            hipLaunchKernelGGL((k_closest_silhouette<BS, false, DIM, false, SSZ>), g, b, 0, st, t, q, n, flip, seed,
-                               sil, dist, perm, err, stats);
-    }
+    uint32_t* fail = capped ? nullptr : fail_buf;
+    uint32_t* count = fail ? fail_buf + n : nullptr;
+    if (count)
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, count);
+    if (stats)
+        SNCH_SIL(true, 2, t, q, n, n, flip, seed, sil, dist, perm, fail, count, err, stats);
+    else
+        SNCH_SIL(false, 2, t, q, n, n, flip, seed, sil, dist, perm, fail, count, err, stats);
+    if (capped)
+        return;
+    TreeView t3 = t;
+    t3.n_dev = count; // the second pass covers only the queries listed by the first
+    if (stats)
+        SNCH_SIL(true, 3, t3, q, n, n, flip, seed, sil, dist, fail, nullptr, nullptr, err, stats);
+    else
+        SNCH_SIL(false, 3, t3, q, n, n, flip, seed, sil, dist, fail, nullptr, nullptr, err, stats);
+#undef SNCH_SIL
 }
 
 template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
-                          uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
+                          uint32_t* fail_buf, uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
                           unsigned long long* stats, hipStream_t st)
 {
     if (DIM == 2 || depth == 16) // scene<2> uses the default depth only
-        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
     else if (depth == 8)
-        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
     else if (depth == 24)
-        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
     else
-        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
-                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st)
+                                     const float* seed, bool capped, uint32_t* fail_buf, uint32_t* sil, float* dist,
+                                     uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
+                                     hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
+    if (seed && !capped && !fail_buf)
+        return hipErrorInvalidValue;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
+        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
+        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
         }
     });
     return hipGetLastError();
