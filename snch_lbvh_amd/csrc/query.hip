@@ -493,12 +493,13 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
 }
 
 // MODE 0: plain search.  MODE 2: search within the radius seed[qi] only
-// (a hard cap; with `fail` set, queries whose answer was not found within the
-// radius are appended to fail[0 .. *fail_count)).  MODE 3: continue a capped
+// (a hard cap; with `fail` set, fail[slot] = 1 marks a query whose answer
+// was not found within the radius, else 0).  MODE 3: continue a capped
 // search without the cap, starting from the candidate it left in out_d /
-// out_e.  A seeded search is MODE 2 with a fail list, then MODE 3 over that
-// list: bit-identical to MODE 0, and the few queries that need the second
-// pass run together instead of stalling their waves.
+// out_e.  A seeded search is MODE 2 with flags, a compaction of the flagged
+// queries in visiting order, then MODE 3 over that list: bit-identical to
+// MODE 0, and the few queries that need the second pass run together (and
+// still in spatial order) instead of stalling their waves.
 template <int BS, bool STATS, int DIM, int MODE, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            uint32_t nq, int flip, const float* __restrict__ seed,
@@ -518,6 +519,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
             flush_counts(cnt, stats);
         return;
     }
+    (void)fail_count;
     float cap = MODE == 2 ? seed[qi] : F_INF;
     if (!(cap >= 0.0f))
         cap = F_INF;
@@ -532,11 +534,8 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
                                    ovf);
     if (ovf && err)
         atomicOr(err, 1u);
-    if (MODE == 2 && fail && cap != F_INF && !(best_e != INVALID && best <= cap)) {
-        const uint32_t k = atomicAdd(fail_count, 1u);
-        if (k < nq)
-            fail[k] = qi;
-    }
+    if (MODE == 2 && fail)
+        fail[slot] = (cap != F_INF && !(best_e != INVALID && best <= cap)) ? 1u : 0u;
     if (out_e)
         out_e[qi] = best_e;
     out_d[qi] = best;
@@ -765,6 +764,27 @@ __global__ __launch_bounds__(256) void k_batch_codes(const TreeView* __restrict_
 }
 
 __global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
+
+// flags[n .. len) = 0 (the tail of the padded flag array)
+__global__ void k_zero_tail(uint32_t* flags, uint32_t n, uint32_t len)
+{
+    const uint32_t i = n + threadIdx.x;
+    if (i < len)
+        flags[i] = 0u;
+}
+
+// after an exclusive scan of the flags: pos[s + 1] != pos[s] marks slot s;
+// list its query at pos[s]; pos[n] is the count
+__global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ pos, uint32_t n,
+                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ list,
+                                                 uint32_t* __restrict__ count)
+{
+    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+    if (s == 0)
+        *count = pos[n];
+    if (s < n && pos[s + 1] != pos[s])
+        list[pos[s]] = perm ? perm[s] : s;
+}
 
 // mesh of each query of a batch (last m with qoff[m] <= i)
 __global__ __launch_bounds__(256) void k_query_mesh(const uint32_t* __restrict__ qoff, uint32_t M, uint32_t n,
@@ -1010,22 +1030,27 @@ static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int fl
             SNCH_SIL(false, 0, t, q, n, n, flip, seed, sil, dist, perm, nullptr, nullptr, err, stats);
         return;
     }
-    uint32_t* fail = capped ? nullptr : fail_buf;
-    uint32_t* count = fail ? fail_buf + n : nullptr;
-    if (count)
-        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, count);
+    // scratch: list (n) | count | flags / positions (padded) | scan sums
+    uint32_t* list = capped ? nullptr : fail_buf;
+    uint32_t* count = list ? list + n : nullptr;
+    const uint32_t flen = fail_flags_len(n);
+    uint32_t* flags = list ? list + fail_flags_off(n) : nullptr;
+    if (flags)
+        hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(64), 0, st, flags, n, flen);
     if (stats)
-        SNCH_SIL(true, 2, t, q, n, n, flip, seed, sil, dist, perm, fail, count, err, stats);
+        SNCH_SIL(true, 2, t, q, n, n, flip, seed, sil, dist, perm, flags, nullptr, err, stats);
     else
-        SNCH_SIL(false, 2, t, q, n, n, flip, seed, sil, dist, perm, fail, count, err, stats);
+        SNCH_SIL(false, 2, t, q, n, n, flip, seed, sil, dist, perm, flags, nullptr, err, stats);
     if (capped)
         return;
+    (void)exclusive_scan_u32(flags, flen, flags + flen, st);
+    hipLaunchKernelGGL(k_compact, dim3(nblocks(n, 256)), dim3(256), 0, st, flags, n, perm, list, count);
     TreeView t3 = t;
     t3.n_dev = count; // the second pass covers only the queries listed by the first
     if (stats)
-        SNCH_SIL(true, 3, t3, q, n, n, flip, seed, sil, dist, fail, nullptr, nullptr, err, stats);
+        SNCH_SIL(true, 3, t3, q, n, n, flip, seed, sil, dist, list, nullptr, nullptr, err, stats);
     else
-        SNCH_SIL(false, 3, t3, q, n, n, flip, seed, sil, dist, fail, nullptr, nullptr, err, stats);
+        SNCH_SIL(false, 3, t3, q, n, n, flip, seed, sil, dist, list, nullptr, nullptr, err, stats);
 #undef SNCH_SIL
 }
 
