@@ -575,13 +575,8 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
     const bool capped = opts && opts->seed_is_cap;
-    uint32_t* fail_buf = nullptr;
-    if (seed && !capped) {
-        SNCH_HIP(s->qscratch.fail.ensure(sizeof(uint32_t) * fail_buf_entries(n)), "snch_closest_silhouette: scratch");
-        fail_buf = s->qscratch.fail.as<uint32_t>();
-    }
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, fail_buf, out_sil, out_dist,
-                                       block, depth, perm, opts ? opts->stats : nullptr, st),
+    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, out_sil, out_dist, block,
+                                       depth, perm, opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
     return SNCH_OK;
 }

@@ -85,7 +85,7 @@ struct DeviceBuffer {
 
 // per-scene scratch of the query reordering (Morton sort of query points)
 struct QueryScratch {
-    DeviceBuffer codes, idx, codes_alt, idx_alt, hist, fail;
+    DeviceBuffer codes, idx, codes_alt, idx_alt, hist;
 };
 
 // per-scene state of batched queries: device copies of the per-mesh views
@@ -157,14 +157,6 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt,
                             const uint32_t* seg = nullptr, int seg_bits = 0);
-// sort.hip: in-place exclusive scan of a[0, len), len a multiple of 4; sums
-// needs len / 4096 + 8 entries of scratch
-hipError_t exclusive_scan_u32(uint32_t* a, uint32_t len, uint32_t* sums, hipStream_t st);
-// entries of the seeded-search scratch for n queries: list (n), count (1),
-// per-slot flags / positions (fail_flags_len(n)), scan sums
-inline uint32_t fail_flags_len(uint32_t n) { return (n + 1 + 3) & ~3u; }
-inline size_t fail_flags_off(uint32_t n) { return ((size_t)n + 1 + 3) & ~(size_t)3; } // 16-B aligned
-inline size_t fail_buf_entries(uint32_t n) { return fail_flags_off(n) + fail_flags_len(n) + fail_flags_len(n) / 4096 + 8; }
 // query.hip (dim = 2 or 3).  perm (nullable): slot -> query index, the
 // order in which queries are visited (query_order computes a Morton order)
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
@@ -174,14 +166,11 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
                                 hipStream_t st);
 // seed (nullable): per-query radius.  capped: a hard radius (answers beyond
 // it are not searched; the returned distance is then larger than the
-// radius, possibly +inf).  Otherwise a hint: a capped pass, then an uncapped
-// pass over the queries it could not settle, compacted in visiting order
-// into fail_buf (fail_buf_entries(n) entries); results equal the unseeded
-// search.
+// radius, possibly +inf).  Otherwise a hint: queries not settled within it
+// continue without it; results equal the unseeded search.
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, bool capped, uint32_t* fail_buf, uint32_t* sil, float* dist,
-                                     uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
-                                     hipStream_t st);
+                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

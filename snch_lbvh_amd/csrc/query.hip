@@ -492,20 +492,18 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     overflow = overflow || st.overflow;
 }
 
-// MODE 0: plain search.  MODE 2: search within the radius seed[qi] only
-// (a hard cap; with `fail` set, fail[slot] = 1 marks a query whose answer
-// was not found within the radius, else 0).  MODE 3: continue a capped
-// search without the cap, starting from the candidate it left in out_d /
-// out_e.  A seeded search is MODE 2 with flags, a compaction of the flagged
-// queries in visiting order, then MODE 3 over that list: bit-identical to
-// MODE 0, and the few queries that need the second pass run together (and
-// still in spatial order) instead of stalling their waves.
+// MODE 0: plain search.  MODE 1: search within the radius seed[qi]; a
+// query not settled there continues at once without the cap, from the
+// candidate it found (bit-identical to MODE 0).  MODE 2: search within the
+// radius only (a hard cap).  Keeping the continuation in the same launch
+// lets the rare unsettled queries (the costliest ones) overlap with the rest
+// of the grid; a separate second launch over them measured ~30 ms slower on
+// the 1M-triangle torus, all of it the tail of those few queries.
 template <int BS, bool STATS, int DIM, int MODE, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            uint32_t nq, int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
                                                            const uint32_t* __restrict__ perm,
-                                                           uint32_t* __restrict__ fail, uint32_t* __restrict__ fail_count,
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
     __shared__ uint32_t stk[SSZ * BS];
@@ -519,23 +517,19 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
             flush_counts(cnt, stats);
         return;
     }
-    (void)fail_count;
-    float cap = MODE == 2 ? seed[qi] : F_INF;
+    float cap = MODE != 0 ? seed[qi] : F_INF;
     if (!(cap >= 0.0f))
         cap = F_INF;
     float best = F_INF;
     uint32_t best_e = INVALID;
-    if (MODE == 3) {
-        best = out_d[qi];
-        best_e = out_e ? out_e[qi] : INVALID;
-    }
     bool ovf = false;
-    sil_query<BS, SSZ, STATS, DIM>(t, load_point<DIM>(q, qi), flip != 0, cap, stk + threadIdx.x, best, best_e, cnt,
-                                   ovf);
+    const v3 p = load_point<DIM>(q, qi);
+    sil_query<BS, SSZ, STATS, DIM>(t, p, flip != 0, cap, stk + threadIdx.x, best, best_e, cnt, ovf);
+    // every candidate within `best` was examined when best <= cap
+    if (MODE == 1 && cap != F_INF && !(best_e != INVALID && best <= cap))
+        sil_query<BS, SSZ, STATS, DIM>(t, p, flip != 0, F_INF, stk + threadIdx.x, best, best_e, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
-    if (MODE == 2 && fail)
-        fail[slot] = (cap != F_INF && !(best_e != INVALID && best <= cap)) ? 1u : 0u;
     if (out_e)
         out_e[qi] = best_e;
     out_d[qi] = best;
@@ -764,27 +758,6 @@ __global__ __launch_bounds__(256) void k_batch_codes(const TreeView* __restrict_
 }
 
 __global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
-
-// flags[n .. len) = 0 (the tail of the padded flag array)
-__global__ void k_zero_tail(uint32_t* flags, uint32_t n, uint32_t len)
-{
-    const uint32_t i = n + threadIdx.x;
-    if (i < len)
-        flags[i] = 0u;
-}
-
-// after an exclusive scan of the flags: pos[s + 1] != pos[s] marks slot s;
-// list its query at pos[s]; pos[n] is the count
-__global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ pos, uint32_t n,
-                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ list,
-                                                 uint32_t* __restrict__ count)
-{
-    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-    if (s == 0)
-        *count = pos[n];
-    if (s < n && pos[s + 1] != pos[s])
-        list[pos[s]] = perm ? perm[s] : s;
-}
 
 // mesh of each query of a batch (last m with qoff[m] <= i)
 __global__ __launch_bounds__(256) void k_query_mesh(const uint32_t* __restrict__ qoff, uint32_t M, uint32_t n,
@@ -1016,8 +989,8 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
 
 template <int BS, int DIM, int SSZ>
 static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
-                           uint32_t* fail_buf, uint32_t* sil, float* dist, const uint32_t* perm,
-                           unsigned long long* stats, hipStream_t st)
+                           uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
+                           hipStream_t st)
 {
     const dim3 g(nblocks(n, BS)), b(BS);
     uint32_t* err = traversal_error_word();
@@ -1025,66 +998,51 @@ static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int fl
     hipLaunchKernelGGL((k_closest_silhouette<BS, S_, DIM, M_, SSZ>), g, b, 0, st, __VA_ARGS__)
     if (!seed) {
         if (stats)
-            SNCH_SIL(true, 0, t, q, n, n, flip, seed, sil, dist, perm, nullptr, nullptr, err, stats);
+            SNCH_SIL(true, 0, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
         else
-            SNCH_SIL(false, 0, t, q, n, n, flip, seed, sil, dist, perm, nullptr, nullptr, err, stats);
-        return;
+            SNCH_SIL(false, 0, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
+    } else if (capped) {
+        if (stats)
+            SNCH_SIL(true, 2, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
+        else
+            SNCH_SIL(false, 2, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
+    } else {
+        if (stats)
+            SNCH_SIL(true, 1, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
+        else
+            SNCH_SIL(false, 1, t, q, n, n, flip, seed, sil, dist, perm, err, stats);
     }
-    // scratch: list (n) | count | flags / positions (padded) | scan sums
-    uint32_t* list = capped ? nullptr : fail_buf;
-    uint32_t* count = list ? list + n : nullptr;
-    const uint32_t flen = fail_flags_len(n);
-    uint32_t* flags = list ? list + fail_flags_off(n) : nullptr;
-    if (flags)
-        hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(64), 0, st, flags, n, flen);
-    if (stats)
-        SNCH_SIL(true, 2, t, q, n, n, flip, seed, sil, dist, perm, flags, nullptr, err, stats);
-    else
-        SNCH_SIL(false, 2, t, q, n, n, flip, seed, sil, dist, perm, flags, nullptr, err, stats);
-    if (capped)
-        return;
-    (void)exclusive_scan_u32(flags, flen, flags + flen, st);
-    hipLaunchKernelGGL(k_compact, dim3(nblocks(n, 256)), dim3(256), 0, st, flags, n, perm, list, count);
-    TreeView t3 = t;
-    t3.n_dev = count; // the second pass covers only the queries listed by the first
-    if (stats)
-        SNCH_SIL(true, 3, t3, q, n, n, flip, seed, sil, dist, list, nullptr, nullptr, err, stats);
-    else
-        SNCH_SIL(false, 3, t3, q, n, n, flip, seed, sil, dist, list, nullptr, nullptr, err, stats);
 #undef SNCH_SIL
 }
 
 template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
-                          uint32_t* fail_buf, uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
+                          uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
                           unsigned long long* stats, hipStream_t st)
 {
     if (DIM == 2 || depth == 16) // scene<2> uses the default depth only
-        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
     else if (depth == 8)
-        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
     else if (depth == 24)
-        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
     else
-        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, fail_buf, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                     const float* seed, bool capped, uint32_t* fail_buf, uint32_t* sil, float* dist,
-                                     uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
-                                     hipStream_t st)
+                                     const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
-    if (seed && !capped && !fail_buf)
-        return hipErrorInvalidValue;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
-        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
-        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, fail_buf, sil, dist, depth, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
         }
     });
     return hipGetLastError();

@@ -288,19 +288,4 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     return hipGetLastError();
 }
 
-hipError_t exclusive_scan_u32(uint32_t* a, uint32_t len, uint32_t* sums, hipStream_t st)
-{
-    if (len == 0)
-        return hipSuccess;
-    const uint32_t nchunks = (len + 4095) / 4096;
-    if (nchunks <= 4) {
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, a, len);
-    } else {
-        hipLaunchKernelGGL(k_chunk_sums, dim3(nchunks), dim3(1024), 0, st, a, len, sums);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, sums, (nchunks + 3) & ~3u);
-        hipLaunchKernelGGL(k_chunk_scan, dim3(nchunks), dim3(1024), 0, st, a, len, sums);
-    }
-    return hipGetLastError();
-}
-
 } // namespace snch

@@ -277,8 +277,7 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
         if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, 64, 16, nullptr, nullptr, st)) != hipSuccess)
             return e;
         if (has_n &&
-            (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, s->w.cp_dist, true, nullptr, s->w.sil_idx, s->w.sil_dist,
-                                           64, 16,
+            (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, s->w.cp_dist, true, s->w.sil_idx, s->w.sil_dist, 64, 16,
                                            nullptr, nullptr, st)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_star, dim3(nb), dim3(256), 0, st, s->w, cur, cap, s->dir->dV.as<float4>(),
