@@ -406,15 +406,24 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 }
 
 // query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
-// best / best_e hold the initial bound and its silhouette (+inf / INVALID for
-// a fresh search, or a candidate found earlier) and receive the answer.  A
-// radius `cap` (finite) skips nodes farther than cap; a silhouette found at
-// distance <= cap is then the global answer, because every node that could
-// hold one at least as close was visited.
-template <int BS, int SSZ, bool STATS, int DIM>
+// best / best_e receive the answer.  A finite radius `cap` changes only the
+// visiting order: internal nodes farther than cap (that pass the reference's
+// tests) are set aside on a short deferred list instead of being descended
+// into.  When the search within cap ends with best <= cap, every node that
+// could hold a silhouette at least as close has been visited and the
+// deferred nodes are dropped; otherwise (HINT) the search resumes from the
+// deferred nodes with no cap, so nothing is visited twice, or (hard cap,
+// !HINT) it stops.  If the deferred list overflows, the resumed search
+// restarts from the root with the bound found so far.  With index
+// tie-breaking the answer equals the uncapped search.
+constexpr int DEFER = 24; // deferred nodes per lane (private)
+
+template <int BS, int SSZ, bool STATS, int DIM, bool HINT>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
                                           float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
 {
+    best = F_INF;
+    best_e = INVALID;
     if (t.n_leaves == 1) {
         const box3 b = ldbox2(t.nbox, 0);
         const float4 c0 = t.ncone[0], c1 = t.ncone[1];
@@ -424,16 +433,20 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     }
     if (!(cap >= 0.0f))
         cap = F_INF;
+    float cap2 = cap * cap;
     PackedStack<BS, SSZ> st;
     st.init(stk_col, t.qbase);
-    const float cap2 = cap * cap;
+    uint32_t dnode[DEFER];
+    float dkey[DEFER];
+    int nd = 0;
+    bool dlost = false;
     uint32_t node = 0;
-    const uint32_t cap_iters = 2 * t.n_leaves + 64;
+    const uint32_t cap_iters = 4 * t.n_leaves + 64;
     for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
         const NodeRec nr = load_node(t.tnode, node);
         const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
         const float b2 = best * best;
-        bool hL = Lmd <= b2 && Lmd <= cap2, hR = Rmd <= b2 && Rmd <= cap2;
+        bool hL = Lmd <= b2, hR = Rmd <= b2;
         if (STATS) {
             cnt.node += 1;
             cnt.box += 2;
@@ -456,15 +469,38 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
         }
         uint32_t lf, lc, rf, rc;
         const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
-        if (hL && lLeaf)
+        if (hL && lLeaf && Lmd <= cap2)
             for (uint32_t k = 0; k < lc; ++k)
                 sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
-        if (hR && rLeaf)
+        if (hR && rLeaf && Rmd <= cap2)
             for (uint32_t k = 0; k < rc; ++k)
                 sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
         const float nb2 = best * best;
-        const bool gL = hL && !lLeaf && Lmd <= nb2;
-        const bool gR = hR && !rLeaf && Rmd <= nb2;
+        bool gL = hL && !lLeaf && Lmd <= nb2;
+        bool gR = hR && !rLeaf && Rmd <= nb2;
+        // beyond the cap: set aside (leaves too, as their parent's half)
+        if (cap2 != F_INF) {
+            if (hL && Lmd > cap2 && Lmd <= nb2) {
+                if (nd < DEFER) {
+                    dnode[nd] = node | (lLeaf ? LEAF_BIT : 0u);
+                    dkey[nd] = Lmd;
+                    ++nd;
+                } else {
+                    dlost = true;
+                }
+                gL = false;
+            }
+            if (hR && Rmd > cap2 && Rmd <= nb2) {
+                if (nd < DEFER) {
+                    dnode[nd] = node | (rLeaf ? LEAF_BIT : 0u) | 0x40000000u;
+                    dkey[nd] = Rmd;
+                    ++nd;
+                } else {
+                    dlost = true;
+                }
+                gR = false;
+            }
+        }
         if (gL && gR) {
             const bool lfirst = Lmd <= Rmd;
             st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
@@ -486,19 +522,59 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
                 break;
             }
         }
+        if (found)
+            continue;
+        // the search within the cap is over
+        if (cap2 == F_INF || (best_e != INVALID && best <= cap) || !HINT)
+            break;
+        cap2 = F_INF;
+        if (dlost) { // some set-aside nodes were not kept: restart with the bound found so far
+            node = 0;
+            continue;
+        }
+        // resume from the set-aside children: a deferred record names the
+        // parent and the side; leaves are evaluated now, internal nodes stacked
+        while (nd > 0) {
+            --nd;
+            if (!(dkey[nd] <= best * best))
+                continue;
+            const uint32_t rec = dnode[nd];
+            const uint32_t par = rec & 0x3FFFFFFFu & ~LEAF_BIT;
+            const NodeRec pr = load_node(t.tnode, par & 0x3FFFFFFFu);
+            const bool right = (rec & 0x40000000u) != 0;
+            const uint32_t ref = right ? pr.rref : pr.lref, run = right ? pr.rrun : pr.lrun;
+            uint32_t f0, c0;
+            if (as_leaf(ref, run, f0, c0)) {
+                for (uint32_t k = 0; k < c0; ++k)
+                    sil_leaf<DIM>(t, f0 + k, p, flip, best, best_e, cnt);
+            } else {
+                st.push(ref, dkey[nd]);
+            }
+        }
+        found = false;
+        while (st.sp > 0) {
+            uint32_t nn;
+            float dd;
+            st.pop(nn, dd);
+            if (dd <= best * best) {
+                node = nn;
+                found = true;
+                break;
+            }
+        }
         if (!found)
             break;
     }
     overflow = overflow || st.overflow;
 }
 
-// MODE 0: plain search.  MODE 1: search within the radius seed[qi]; a
-// query not settled there continues at once without the cap, from the
-// candidate it found (bit-identical to MODE 0).  MODE 2: search within the
-// radius only (a hard cap).  Keeping the continuation in the same launch
-// lets the rare unsettled queries (the costliest ones) overlap with the rest
-// of the grid; a separate second launch over them measured ~30 ms slower on
-// the 1M-triangle torus, all of it the tail of those few queries.
+// MODE 0: plain search.  MODE 1: search within the radius seed[qi] first; a
+// query not settled there resumes from the nodes it set aside (bit-identical
+// to MODE 0).  MODE 2: search within the radius only (a hard cap).  Keeping
+// the resumption in the same launch lets the rare unsettled queries (the
+// costliest ones) overlap with the rest of the grid; a separate second
+// launch over them measured ~30 ms slower on the 1M-triangle torus, all of
+// it the tail of those few queries.
 template <int BS, bool STATS, int DIM, int MODE, int SSZ>
 __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            uint32_t nq, int flip, const float* __restrict__ seed,
@@ -520,14 +596,11 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     float cap = MODE != 0 ? seed[qi] : F_INF;
     if (!(cap >= 0.0f))
         cap = F_INF;
-    float best = F_INF;
-    uint32_t best_e = INVALID;
+    float best;
+    uint32_t best_e;
     bool ovf = false;
     const v3 p = load_point<DIM>(q, qi);
-    sil_query<BS, SSZ, STATS, DIM>(t, p, flip != 0, cap, stk + threadIdx.x, best, best_e, cnt, ovf);
-    // every candidate within `best` was examined when best <= cap
-    if (MODE == 1 && cap != F_INF && !(best_e != INVALID && best <= cap))
-        sil_query<BS, SSZ, STATS, DIM>(t, p, flip != 0, F_INF, stk + threadIdx.x, best, best_e, cnt, ovf);
+    sil_query<BS, SSZ, STATS, DIM, MODE != 2>(t, p, flip != 0, cap, stk + threadIdx.x, best, best_e, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
     if (out_e)
@@ -730,7 +803,7 @@ __global__ __launch_bounds__(BS) void k_batch(BatchTable bt, const float* __rest
                 cp_query<BS, SSZ, false, 3>(tv, p, stk + threadIdx.x, best, bi, cnt, ovf);
                 best = dm::sqrt_(best);
             } else {
-                sil_query<BS, SSZ, false, 3>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
+                sil_query<BS, SSZ, false, 3, false>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
             }
             if (out_idx)
                 out_idx[qi] = bi;
