@@ -103,12 +103,18 @@ struct LaneStack {
 // saturate at 255, so the decoded value never exceeds the key and a popped
 // entry is never skipped wrongly.  Half the LDS and spill traffic of uint2
 // entries.
+// The private array also holds, from its top end down, up to DEFER entries
+// set aside by a capped search (see sil_query).
+constexpr int DEFER = 24;
+
 template <int BS, int SSZ>
 struct PackedStack {
     using Entry = uint32_t;
     Entry* lds;
     int sp;
-    uint32_t ov[OVF];
+    int nd; // set-aside entries at ov[OVF2 - 1], ov[OVF2 - 2], ...
+    static constexpr int OVF2 = OVF + DEFER;
+    uint32_t ov[OVF2];
     bool overflow;
     uint32_t qbase;
 
@@ -116,8 +122,25 @@ struct PackedStack {
     {
         lds = base;
         sp = 0;
+        nd = 0;
         overflow = false;
         qbase = qb;
+    }
+    __device__ __forceinline__ bool defer(uint32_t n, float d)
+    {
+        if (nd >= DEFER)
+            return false;
+        ov[OVF2 - 1 - nd] = enc(n, d);
+        ++nd;
+        return true;
+    }
+    __device__ __forceinline__ void undefer(uint32_t& n, float& d)
+    {
+        --nd;
+        const uint32_t e = ov[OVF2 - 1 - nd];
+        n = e >> 8;
+        const uint32_t q = e & 255u;
+        d = q ? __uint_as_float((q + qbase) << 20) : 0.0f;
     }
     __device__ __forceinline__ uint32_t enc(uint32_t n, float d) const
     {
@@ -408,16 +431,14 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 // query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
 // best / best_e receive the answer.  A finite radius `cap` changes only the
 // visiting order: internal nodes farther than cap (that pass the reference's
-// tests) are set aside on a short deferred list instead of being descended
-// into.  When the search within cap ends with best <= cap, every node that
-// could hold a silhouette at least as close has been visited and the
-// deferred nodes are dropped; otherwise (HINT) the search resumes from the
-// deferred nodes with no cap, so nothing is visited twice, or (hard cap,
-// !HINT) it stops.  If the deferred list overflows, the resumed search
-// restarts from the root with the bound found so far.  With index
-// tie-breaking the answer equals the uncapped search.
-constexpr int DEFER = 24; // deferred nodes per lane (private)
-
+// tests) are set aside instead of being descended into.  When the search
+// within cap ends with best <= cap, every node that could hold a silhouette
+// at least as close has been visited and the set-aside nodes are dropped;
+// otherwise (HINT) the search resumes from them with no cap, so nothing is
+// visited twice, or (hard cap, !HINT) it stops.  If more nodes are set aside
+// than the stack keeps, the resumed search restarts from the root with the
+// bound found so far.  With index tie-breaking the answer equals the
+// uncapped search.
 template <int BS, int SSZ, bool STATS, int DIM, bool HINT>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
                                           float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
@@ -436,9 +457,6 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     float cap2 = cap * cap;
     PackedStack<BS, SSZ> st;
     st.init(stk_col, t.qbase);
-    uint32_t dnode[DEFER];
-    float dkey[DEFER];
-    int nd = 0;
     bool dlost = false;
     uint32_t node = 0;
     const uint32_t cap_iters = 4 * t.n_leaves + 64;
@@ -469,37 +487,22 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
         }
         uint32_t lf, lc, rf, rc;
         const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
-        if (hL && lLeaf && Lmd <= cap2)
+        if (hL && lLeaf)
             for (uint32_t k = 0; k < lc; ++k)
                 sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
-        if (hR && rLeaf && Rmd <= cap2)
+        if (hR && rLeaf)
             for (uint32_t k = 0; k < rc; ++k)
                 sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
         const float nb2 = best * best;
         bool gL = hL && !lLeaf && Lmd <= nb2;
         bool gR = hR && !rLeaf && Rmd <= nb2;
-        // beyond the cap: set aside (leaves too, as their parent's half)
-        if (cap2 != F_INF) {
-            if (hL && Lmd > cap2 && Lmd <= nb2) {
-                if (nd < DEFER) {
-                    dnode[nd] = node | (lLeaf ? LEAF_BIT : 0u);
-                    dkey[nd] = Lmd;
-                    ++nd;
-                } else {
-                    dlost = true;
-                }
-                gL = false;
-            }
-            if (hR && Rmd > cap2 && Rmd <= nb2) {
-                if (nd < DEFER) {
-                    dnode[nd] = node | (rLeaf ? LEAF_BIT : 0u) | 0x40000000u;
-                    dkey[nd] = Rmd;
-                    ++nd;
-                } else {
-                    dlost = true;
-                }
-                gR = false;
-            }
+        if (gL && Lmd > cap2) { // beyond the cap: set aside
+            dlost = dlost || !st.defer(nr.lref, Lmd);
+            gL = false;
+        }
+        if (gR && Rmd > cap2) {
+            dlost = dlost || !st.defer(nr.rref, Rmd);
+            gR = false;
         }
         if (gL && gR) {
             const bool lfirst = Lmd <= Rmd;
@@ -512,54 +515,34 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
             continue;
         }
         bool found = false;
-        while (st.sp > 0) {
-            uint32_t nn;
-            float dd;
-            st.pop(nn, dd);
-            if (dd <= best * best) { // query.cuh:347
-                node = nn;
+        for (int phase = 0; phase < 2 && !found; ++phase) {
+            while (st.sp > 0) {
+                uint32_t nn;
+                float dd;
+                st.pop(nn, dd);
+                if (dd <= best * best) { // query.cuh:347
+                    node = nn;
+                    found = true;
+                    break;
+                }
+            }
+            if (found || phase == 1)
+                break;
+            // the search within the cap is over
+            if (cap2 == F_INF || (best_e != INVALID && best <= cap) || !HINT)
+                break;
+            cap2 = F_INF;
+            if (dlost) { // not every set-aside node was kept: restart with the bound found so far
+                node = 0;
                 found = true;
                 break;
             }
-        }
-        if (found)
-            continue;
-        // the search within the cap is over
-        if (cap2 == F_INF || (best_e != INVALID && best <= cap) || !HINT)
-            break;
-        cap2 = F_INF;
-        if (dlost) { // some set-aside nodes were not kept: restart with the bound found so far
-            node = 0;
-            continue;
-        }
-        // resume from the set-aside children: a deferred record names the
-        // parent and the side; leaves are evaluated now, internal nodes stacked
-        while (nd > 0) {
-            --nd;
-            if (!(dkey[nd] <= best * best))
-                continue;
-            const uint32_t rec = dnode[nd];
-            const uint32_t par = rec & 0x3FFFFFFFu & ~LEAF_BIT;
-            const NodeRec pr = load_node(t.tnode, par & 0x3FFFFFFFu);
-            const bool right = (rec & 0x40000000u) != 0;
-            const uint32_t ref = right ? pr.rref : pr.lref, run = right ? pr.rrun : pr.lrun;
-            uint32_t f0, c0;
-            if (as_leaf(ref, run, f0, c0)) {
-                for (uint32_t k = 0; k < c0; ++k)
-                    sil_leaf<DIM>(t, f0 + k, p, flip, best, best_e, cnt);
-            } else {
-                st.push(ref, dkey[nd]);
-            }
-        }
-        found = false;
-        while (st.sp > 0) {
-            uint32_t nn;
-            float dd;
-            st.pop(nn, dd);
-            if (dd <= best * best) {
-                node = nn;
-                found = true;
-                break;
+            while (st.nd > 0) { // resume from the set-aside nodes
+                uint32_t nn;
+                float dd;
+                st.undefer(nn, dd);
+                if (dd <= best * best)
+                    st.push(nn, dd);
             }
         }
         if (!found)
