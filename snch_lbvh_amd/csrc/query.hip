@@ -747,7 +747,7 @@ struct BatchTable {
 };
 
 template <int BS, int SSZ, int KIND>
-__global__ __launch_bounds__(BS) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
+__global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
                                               uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
                                               uint32_t* __restrict__ err)
 {
