@@ -457,7 +457,7 @@ def main():
     achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(), "bytes_per_query": per_q,
-                "kernel": "k_closest_silhouette<64, false, 3, false, 8, true>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
+                "kernel": "k_closest_silhouette<64, false, 3, 0, 16>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
                 "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms, "shape_sweep_ms": sweep,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
 
