@@ -16,7 +16,7 @@
 //                     (bvh.cuh:459-476), so no uniqueness pass or host
 //                     round trip is needed.
 //   k_refit           bottom-up AABB + cone merge over subtrees of at most
-//                     1024 leaves in LDS, then k_refit_top above them (the
+//                     256 leaves in LDS, then k_refit_top above them (the
 //                     parent box centroid feeds the cone merge, as in
 //                     bvh.cuh:598); writes the reference-layout node data and
 //                     the packed traversal records.
@@ -409,7 +409,7 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const 
     return 32 + __clz((idx[a] - off) ^ (idx[b] - off));
 }
 
-constexpr uint32_t RT = 1024; // leaves of one refit subtree (threads of one refit workgroup)
+constexpr uint32_t RT = 256; // leaves of one refit subtree (threads of one refit workgroup)
 
 __device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_t node)
 {
@@ -418,19 +418,17 @@ __device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_
         cuts[k] = node;
 }
 
-// bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes).
-// One thread per internal node of every mesh (internal-record slot t);
-// nodes are written with global (mesh-major) ids.
-__global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes_all,
-                                               const uint32_t* __restrict__ idx_all, uint32_t n_internal,
-                                               const MeshDesc* __restrict__ desc, uint32_t M,
-                                               uint32_t* __restrict__ parent, uint2* __restrict__ child,
-                                               uint2* __restrict__ range, uint32_t* __restrict__ flags,
-                                               uint32_t* __restrict__ cuts, uint32_t cut_cap)
+// bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
+// for one internal node: children, leaf range and parent links; the cuts it
+// roots (maximal subtrees of at most RT leaves below a node of more) and
+// whether it is a top (a node of more than RT leaves)
+__device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_all,
+                                            const uint32_t* __restrict__ idx_all, uint32_t t,
+                                            const MeshDesc* __restrict__ desc, uint32_t M,
+                                            uint32_t* __restrict__ parent, uint2* __restrict__ child,
+                                            uint2* __restrict__ range, uint32_t& n_cut, uint32_t* cut,
+                                            uint32_t& top)
 {
-    const uint32_t t = blockIdx.x * BT + threadIdx.x;
-    if (t >= n_internal)
-        return;
     const uint32_t m = mesh_search<false>(desc, M, t);
     const MeshDesc D = desc[m];
     const uint32_t i = t - (D.off - m), N = D.n, off = D.off, nb = D.nb;
@@ -489,17 +487,59 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     range[self] = make_uint2(off + first, off + last);
     parent[left] = self;
     parent[right] = self;
-    flags[self] = 0u;
-    // refit subtrees ("cuts"): maximal subtrees of at most RT leaves
     if (last - first + 1 > RT) {
+        top = self;
         if (g - first + 1 <= RT)
-            append_cut(cuts, cut_cap, left);
+            cut[n_cut++] = left;
         if (last - g <= RT)
-            append_cut(cuts, cut_cap, right);
+            cut[n_cut++] = right;
     } else if (i == 0) {
-        append_cut(cuts, cut_cap, self);
+        cut[n_cut++] = self;
     }
 }
+
+// One thread per internal node of every mesh (internal-record slot t);
+// nodes are written with global (mesh-major) ids.  Lists the refit subtrees
+// ("cuts") and the nodes above them ("tops"); flags[top] = its list slot.
+__global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes_all,
+                                               const uint32_t* __restrict__ idx_all, uint32_t n_internal,
+                                               const MeshDesc* __restrict__ desc, uint32_t M,
+                                               uint32_t* __restrict__ parent, uint2* __restrict__ child,
+                                               uint2* __restrict__ range, uint32_t* __restrict__ flags,
+                                               uint32_t* __restrict__ cuts, uint32_t cut_cap,
+                                               uint32_t* __restrict__ tops, uint32_t top_cap)
+{
+    __shared__ uint32_t s_n[2], s_base[2]; // cuts and tops of this block: counts, then list offsets
+    if (threadIdx.x < 2)
+        s_n[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t t = blockIdx.x * BT + threadIdx.x;
+    uint32_t n_cut = 0, cut[2] = {INVALID, INVALID}, self = INVALID; // self: set for a top
+    if (t < n_internal)
+        karras_node(codes_all, idx_all, t, desc, M, parent, child, range, n_cut, cut, self);
+    // one list append per block and list (the counters are shared by all blocks)
+    const uint32_t cut_k = n_cut ? atomicAdd(&s_n[0], n_cut) : 0u;
+    const uint32_t top_k = self != INVALID ? atomicAdd(&s_n[1], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const uint32_t n = s_n[threadIdx.x];
+        s_base[threadIdx.x] = n ? atomicAdd(threadIdx.x == 0 ? cuts + cut_cap : tops + top_cap, n) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < n_cut; ++j) {
+        const uint32_t k = s_base[0] + cut_k + j;
+        if (k < cut_cap)
+            cuts[k] = cut[j];
+    }
+    if (self != INVALID) {
+        const uint32_t k = s_base[1] + top_k;
+        if (k < top_cap)
+            tops[k] = self;
+        flags[self] = k;
+    }
+}
+
+
 
 // meshes of one primitive: the leaf is the root and its own cut
 __global__ void k_mesh_roots(const MeshDesc* __restrict__ desc, uint32_t M, uint32_t* __restrict__ parent,
@@ -538,11 +578,14 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
 // counter of an internal node lives in LDS at (Karras position - first), and
 // the box and cone of every node of the cut are kept in LDS too, so the
 // second arrival merges its children without a trip to memory.
-// Workgroup-scope fences order it.  k_refit_top: one workgroup climbs from
-// the cut roots through the nodes above them (about 2N/RT of them) with
-// arrival counters in global memory; all of them are touched by this one
-// workgroup, so workgroup scope orders them too, and the launch boundary
-// publishes the first launch's results.
+// Workgroup-scope fences order it.  k_refit_top: one workgroup writes the
+// nodes above the cuts ("tops", about 2N/RT of them, listed by k_karras).
+// With their parent links, children and pending-children counts in LDS it
+// goes level by level: each round writes, in parallel, every top whose top
+// children are written, and a barrier separates the rounds.  With more than
+// TOP_LDS tops it climbs from the cut roots with arrival counters in memory
+// instead.  All of it is touched by this one workgroup, so workgroup scope
+// orders it, and the launch boundary publishes the first launch's results.
 
 // ids of one mesh: global node id <-> local id / sorted position
 struct MeshIds {
@@ -602,11 +645,12 @@ __device__ __forceinline__ void refit_write(const MeshIds& id, uint32_t p, uint2
     tcone[4 * r + 3] = make_float4(ls, lcs, rs, rcs);
 }
 
-// box + cone of one node in LDS, structure of arrays over 2 * RT slots (slot
-// t: leaf first + t of the cut; slot RT + k: internal node of Karras position
-// first + k)
+// box + cone of live nodes in LDS, structure of arrays over RT slots.  A node
+// lives in the slot of its first leaf (range.x - first): a parent takes over
+// its left child's slot once both children are merged, so live nodes (those
+// whose parent is not merged yet) always hold distinct slots.
 struct NodeLds {
-    float v[11][2 * RT];
+    float v[11][RT];
     __device__ __forceinline__ void put(uint32_t s, const box3& b, const cone3& c)
     {
         v[0][s] = b.lo.x; v[1][s] = b.lo.y; v[2][s] = b.lo.z;
@@ -624,6 +668,29 @@ struct NodeLds {
     }
 };
 
+// Work queue of the nodes to merge in one round.  Each node is queued once,
+// so the queue is one array; round r merges queue[qs, qe) and appends the
+// parents whose second child it merged at qe + n[r % 3].  Three counters
+// rotate: the one a round appends to is cleared two rounds earlier, after
+// every thread has read it (the barrier between rounds orders both).
+struct RoundQueue {
+    uint32_t n[3];
+    __device__ __forceinline__ void push(uint16_t* q, uint32_t qe, uint32_t round, uint32_t item)
+    {
+        q[qe + atomicAdd(&n[round % 3], 1u)] = (uint16_t)item;
+    }
+};
+
+constexpr uint16_t NO_PARENT = 0xFFFFu;
+
+// parent p of a node of the cut [first, last], as a Karras position - first
+// (NO_PARENT unless it is an internal node of the cut's range: never the case
+// below the cut root of a Karras tree)
+__device__ __forceinline__ uint16_t local_parent(const MeshIds& id, uint32_t p, uint32_t first, uint32_t last)
+{
+    return id.internal(p) && id.pos(p) - first <= last - first ? (uint16_t)(id.pos(p) - first) : NO_PARENT;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
@@ -636,8 +703,14 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                                               float4* __restrict__ tleaf, const uint32_t* __restrict__ cuts,
                                               uint32_t cut_cap)
 {
-    __shared__ uint32_t lflag[RT];
     __shared__ NodeLds nd;
+    __shared__ uint2 lch[RT];         // children of the internal node at Karras position first + k
+    __shared__ uint16_t lslot[RT];    // its slot (range.x - first)
+    __shared__ uint16_t lpar[RT];     // its parent (Karras position - first), or NO_PARENT for the cut root
+    __shared__ uint16_t lpl[RT];      // parent of leaf first + t, or NO_PARENT
+    __shared__ uint32_t lflag[RT];    // arrivals at internal node k
+    __shared__ uint16_t queue[RT];
+    __shared__ RoundQueue rq;
     const uint32_t t = threadIdx.x;
     const uint32_t count = min(cuts[cut_cap], cut_cap);
     for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
@@ -653,13 +726,31 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
         } else {
             first = last = id.pos(root);
         }
-        lflag[t] = 0u;
-        __syncthreads();
+        const bool ok = last - first < RT && last < N_all; // holds for every cut of a Karras tree
         const uint32_t j = first + t;
-        if (last - first < RT && j <= last && last < N_all) {
+        lflag[t] = 0u;
+        if (t < 3)
+            rq.n[t] = 0u;
+        // the cut's topology, one coalesced load per array (an internal node of
+        // the cut sits at a Karras position in [first, last]; the position not
+        // used by the cut may name a node outside it, which is never read back)
+        if (ok && j <= last) {
+            const uint32_t leaf = D.nb + (D.n - 1) + (j - D.off);
+            lpl[t] = leaf == root ? NO_PARENT : local_parent(id, parent[leaf], first, last);
+            if (j - D.off < D.n - 1) {
+                const uint32_t ni = D.nb + (j - D.off);
+                const uint2 ch = child[ni];
+                lch[t] = ch;
+                lslot[t] = (uint16_t)min(range[ni].x - first, RT - 1);
+                lpar[t] = ni == root ? NO_PARENT : local_parent(id, parent[ni], first, last);
+            }
+        }
+        bool leaf_ok = false;
+        if (ok && j <= last) {
             const uint32_t orig = sidx[j];
             if (orig - D.off < D.n) { // a permutation of the mesh's primitives; else the node stays unwritten
-                uint32_t node = D.nb + (D.n - 1) + (j - D.off);
+                leaf_ok = true;
+                const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
                 const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
                 const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
                 nbox[2 * node] = b0;
@@ -689,51 +780,138 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                     tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(les));
                     tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
                 }
-                // climb inside the cut (at most its depth, < RT levels)
-                for (uint32_t guard = 0; node != root && guard < RT; ++guard) {
-                    const uint32_t p = parent[node];
-                    if (!id.internal(p) || id.pos(p) < first || id.pos(p) - first >= RT)
-                        break; // malformed tree: never taken for a Karras tree
-                    const uint32_t k = id.pos(p) - first;
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    const uint32_t old = atomicAdd(&lflag[k], 1u);
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    if (old == 0u)
-                        break; // the sibling subtree is not finished; its last thread continues
-                    const uint2 ch = child[p];
-                    const uint32_t sl = (id.internal(ch.x) ? RT : 0u) + (id.pos(ch.x) - first);
-                    const uint32_t sr = (id.internal(ch.y) ? RT : 0u) + (id.pos(ch.y) - first);
-                    box3 lb, rb, pb;
-                    cone3 lc, rc, pc;
-                    nd.get(sl, lb, lc);
-                    nd.get(sr, rb, rc);
-                    refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
-                                     tnode, tcone);
-                    nd.put(RT + k, pb, pc);
-                    node = p;
-                }
             }
         }
+        __syncthreads(); // topology, leaf slots and cleared counters
+        // round 0: the leaves arrive at their parents
+        if (leaf_ok && lpl[t] != NO_PARENT && atomicAdd(&lflag[lpl[t]], 1u) == 1u)
+            rq.push(queue, 0u, 0u, lpl[t]);
         __syncthreads();
+        uint32_t qs = 0, qe = rq.n[0];
+        for (uint32_t round = 1; qs < qe && round < RT; ++round) {
+            if (t == 0)
+                rq.n[(round + 1) % 3] = 0u;
+            for (uint32_t i = qs + t; i < qe; i += RT) {
+                const uint32_t k = queue[i];
+                const uint2 ch = lch[k];
+                const uint32_t ls = lslot[k], rs = min(id.pos(ch.x) + 1u - first, RT - 1);
+                box3 lb, rb, pb;
+                cone3 lc, rc, pc;
+                nd.get(ls, lb, lc);
+                nd.get(rs, rb, rc);
+                refit_write<DIM>(id, D.nb + (first + k - D.off), ch, lb, rb, lc, rc, conservative, range, leaf_size,
+                                 pb, pc, nbox, ncone, tnode, tcone);
+                nd.put(ls, pb, pc);
+                const uint32_t kp = lpar[k];
+                if (kp != NO_PARENT && atomicAdd(&lflag[kp], 1u) == 1u)
+                    rq.push(queue, qe, round, kp);
+            }
+            __syncthreads();
+            const uint32_t added = rq.n[round % 3];
+            qs = qe;
+            qe += added;
+        }
+        __syncthreads(); // the next cut reuses the LDS
     }
 }
+
+// tops held in LDS by k_refit_top, 14 B each: the children, a word (parent
+// slot << 16 | top children not yet written) and a queue entry
+constexpr uint32_t TOP_LDS = 11264;
+constexpr uint32_t TOP_ROUNDS = 128; // a Karras tree over 64-bit keys is at most 64 levels deep
 
 template <int DIM>
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ parent,
                                                     const uint2* __restrict__ child, const uint2* __restrict__ range,
-                                                    uint32_t* flags, const MeshDesc* __restrict__ desc, uint32_t M,
+                                                    uint32_t* flags, const uint32_t* __restrict__ tops,
+                                                    uint32_t top_cap, const MeshDesc* __restrict__ desc, uint32_t M,
                                                     uint32_t n_nodes, int conservative, uint32_t leaf_size,
                                                     float4* nbox, float4* ncone, float4* __restrict__ tnode,
                                                     float4* __restrict__ tcone, const uint32_t* __restrict__ cuts,
                                                     uint32_t cut_cap)
 {
+    __shared__ uint32_t tword[TOP_LDS];
+    __shared__ uint2 tch[TOP_LDS];
+    __shared__ uint16_t queue[TOP_LDS];
+    __shared__ RoundQueue rq;
+    const uint32_t t = threadIdx.x;
+    const uint32_t n_top = min(tops[top_cap], top_cap);
+    if (n_top <= TOP_LDS) {
+        // level by level: a round writes the tops whose top children are
+        // written (the cut roots below them were written by k_refit)
+        if (t < 3)
+            rq.n[t] = 0u;
+        for (uint32_t base = t; base < n_top; base += 4 * 1024) {
+            uint32_t p[4], pp[4];
+            uint2 ch[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t k = base + 1024u * u;
+                p[u] = k < n_top ? tops[k] : INVALID;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pp[u] = p[u] < n_nodes ? parent[p[u]] : INVALID;
+                ch[u] = p[u] < n_nodes ? child[p[u]] : make_uint2(INVALID, INVALID);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t k = base + 1024u * u;
+                if (k < n_top) {
+                    const uint32_t ps = pp[u] < n_nodes ? flags[pp[u]] : 0xFFFFu;
+                    tword[k] = min(ps, 0xFFFFu) << 16;
+                    tch[k] = ch[u];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = t; k < n_top; k += 1024) {
+            const uint32_t ps = tword[k] >> 16;
+            if (ps < n_top)
+                atomicAdd(&tword[ps], 1u);
+        }
+        __syncthreads();
+        for (uint32_t k = t; k < n_top; k += 1024)
+            if ((tword[k] & 0xFFFFu) == 0u)
+                rq.push(queue, 0u, 0u, k);
+        __syncthreads();
+        uint32_t qs = 0, qe = rq.n[0];
+        for (uint32_t round = 1; qs < qe && round < TOP_ROUNDS; ++round) {
+            if (t == 0)
+                rq.n[(round + 1) % 3] = 0u;
+            for (uint32_t i = qs + t; i < qe; i += 1024) {
+                const uint32_t k = queue[i];
+                const uint32_t p = tops[k];
+                const uint2 ch = tch[k];
+                const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
+                const MeshIds id{D.off, D.n, D.nb};
+                box3 pb;
+                cone3 pc;
+                refit_write<DIM>(id, p, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
+                                 ldcone(ncone, ch.y), conservative, range, leaf_size, pb, pc, nbox, ncone, tnode,
+                                 tcone);
+                const uint32_t ps = tword[k] >> 16;
+                if (ps < n_top && (atomicSub(&tword[ps], 1u) & 0xFFFFu) == 1u)
+                    rq.push(queue, qe, round, ps);
+            }
+            __syncthreads(); // this round's node writes precede the next round's reads
+            const uint32_t added = rq.n[round % 3];
+            qs = qe;
+            qe += added;
+        }
+        return;
+    }
+    // more tops than LDS holds: climb from the cut roots with arrival
+    // counters in memory
+    for (uint32_t k = t; k < n_top; k += 1024)
+        flags[tops[k]] = 0u;
+    __syncthreads();
     const uint32_t count = min(cuts[cut_cap], cut_cap);
-    for (uint32_t k = threadIdx.x; k < count; k += 1024) {
+    for (uint32_t k = t; k < count; k += 1024) {
         uint32_t node = cuts[k];
         if (node >= n_nodes)
             continue;
-        const uint32_t m = mesh_search<true>(desc, M, node);
-        const MeshDesc D = desc[m];
+        const MeshDesc D = desc[mesh_search<true>(desc, M, node)];
         const MeshIds id{D.off, D.n, D.nb};
         uint32_t p = parent[node];
         for (int guard = 0; p != INVALID && id.internal(p) && guard < 96; ++guard) {
@@ -751,7 +929,11 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
     }
 }
 
-__global__ void k_zero_word(uint32_t* w) { *w = 0u; }
+__global__ void k_zero_words(uint32_t* a, uint32_t* b)
+{
+    *a = 0u;
+    *b = 0u;
+}
 
 inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
 
@@ -796,6 +978,9 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     const uint32_t cut_cap = N; // cuts are disjoint subtrees of at least one leaf
     SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 1)));
     uint32_t* cuts = w.top.as<uint32_t>();
+    const uint32_t top_cap = NI > 0 ? NI : 1u; // tops are internal nodes
+    SNCH_TRY(w.tops.ensure(sizeof(uint32_t) * (top_cap + 1)));
+    uint32_t* tops = w.tops.as<uint32_t>();
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tcone.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
@@ -826,21 +1011,27 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                               w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt, pmesh, pmesh ? mesh_bits : 0));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, st, cuts + cut_cap);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(1), 0, st, cuts + cut_cap, tops + top_cap);
     if (NI > 0)
         hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, desc, M,
                            w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(),
-                           cuts, cut_cap);
+                           cuts, cut_cap, tops, top_cap);
     hipLaunchKernelGGL(k_mesh_roots, dim3((M + 255) / 256), dim3(256), 0, st, desc, M, w.parent.as<uint32_t>(), cuts,
                        cut_cap);
     // about 2N / RT cuts (each holds at least RT / 2 leaves but for one per
     // side of a large node, plus one per small mesh); workgroups loop over them
     const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
-    const uint32_t refit_blocks = std::max(1u, std::min(2u * ((N + RT - 1) / RT) + M, 2u * n_cu));
     bool any_big = false;
     for (uint32_t m = 0; m < M && !any_big; ++m)
         any_big = s->mesh_off[m + 1] - s->mesh_off[m] > RT;
     SNCH_DISPATCH_REFIT(s->dim, {
+        // as many workgroups as are resident at once (LDS-bound), or one per cut
+        static int per_cu = 0;
+        if (per_cu == 0 &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_refit<DIM>, (int)RT, 0) != hipSuccess)
+            per_cu = 1;
+        const uint32_t refit_blocks =
+            std::max(1u, std::min(2u * ((N + RT - 1) / RT) + M, (uint32_t)std::max(per_cu, 1) * n_cu));
         hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RT), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
@@ -848,7 +1039,8 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                            s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
         if (any_big)
             hipLaunchKernelGGL(k_refit_top<DIM>, dim3(1), dim3(1024), 0, st, w.parent.as<uint32_t>(),
-                               w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), desc, M, NN,
+                               w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), tops, top_cap,
+                               desc, M, NN,
                                conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
                                s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
     });

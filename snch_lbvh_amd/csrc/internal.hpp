@@ -103,7 +103,7 @@ struct BatchConfig {
 
 struct BuildScratch {
     DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
-        top, status;
+        top, tops, status;
 };
 
 } // namespace snch

@@ -126,6 +126,35 @@ __global__ void k_init(Walks w, const float* __restrict__ x0, uint32_t n)
     w.live[0][i] = i;
 }
 
+// Morton code of each live walk's position in the Dirichlet scene's root box
+// (entries past the live count sort last), so that the step's queries visit
+// the walks in spatial order
+__global__ void k_walk_codes(Walks w, int cur, uint32_t cap, const float4* __restrict__ nbox,
+                             uint32_t* __restrict__ codes, uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= cap)
+        return;
+    const uint32_t n = min(w.count[cur], cap);
+    if (i >= n) {
+        codes[i] = 0xFFFFFFFFu;
+        idx[i] = 0xFFFFFFFFu;
+        return;
+    }
+    const uint32_t wk = w.live[cur][i];
+    const float4 p = w.pos[wk], lo = nbox[0], hi = nbox[1];
+    codes[i] = morton3(mk((p.x - lo.x) / (hi.x - lo.x), (p.y - lo.y) / (hi.y - lo.y), (p.z - lo.z) / (hi.z - lo.z)));
+    idx[i] = wk;
+}
+
+__global__ void k_apply_order(Walks w, int cur, uint32_t cap, const uint32_t* __restrict__ sorted)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = min(w.count[cur], cap);
+    if (i < n)
+        w.live[cur][i] = sorted[i];
+}
+
 __global__ void k_gather(Walks w, int cur, uint32_t cap)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -243,7 +272,7 @@ struct snch_wost {
     const snch_scene* neu = nullptr;
     uint32_t cap = 0;
     DeviceBuffer pos, nrm, value, steps, live0, live1, count, qpos, cp_idx, cp_dist, sil_idx, sil_dist, ray_o, ray_d,
-        ray_t, ray_hit, ray_idx, sph_r, sph_u, smp_idx, smp_pdf, star_r;
+        ray_t, ray_hit, ray_idx, sph_r, sph_u, smp_idx, smp_pdf, star_r, ocodes, oidx, ocodes_alt, oidx_alt, ohist;
     Walks w{};
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -273,6 +302,16 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
         const int cur = step & 1;
         vd.n_dev = s->w.count + cur;
         vn.n_dev = s->w.count + cur;
+        // live walks in Morton order (all buffers were sized at creation: no
+        // allocation happens inside a graph capture)
+        hipLaunchKernelGGL(k_walk_codes, dim3(nb), dim3(256), 0, st, s->w, cur, cap, vd.nbox, s->ocodes.as<uint32_t>(),
+                           s->oidx.as<uint32_t>());
+        bool in_alt = false;
+        if ((e = radix_sort_pairs(s->ocodes.as<uint32_t>(), s->oidx.as<uint32_t>(), s->ocodes_alt.as<uint32_t>(),
+                                  s->oidx_alt.as<uint32_t>(), cap, 32, s->ohist, st, &in_alt)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_apply_order, dim3(nb), dim3(256), 0, st, s->w, cur, cap,
+                           in_alt ? s->oidx_alt.as<uint32_t>() : s->oidx.as<uint32_t>());
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, s->w, cur, cap);
         if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, 64, 16, nullptr, nullptr, st)) != hipSuccess)
             return e;
@@ -330,7 +369,10 @@ int snch_wost_create(snch_context* ctx, const snch_scene* dirichlet, const snch_
                 {&s->cp_idx, 4 * n}, {&s->cp_dist, 4 * n}, {&s->sil_idx, 4 * n}, {&s->sil_dist, 4 * n},
                 {&s->ray_o, 12 * n}, {&s->ray_d, 12 * n},  {&s->ray_t, 4 * n},  {&s->ray_hit, n},
                 {&s->ray_idx, 4 * n}, {&s->sph_r, 4 * n},  {&s->sph_u, 12 * n}, {&s->smp_idx, 4 * n},
-                {&s->smp_pdf, 4 * n}, {&s->star_r, 4 * n}};
+                {&s->smp_pdf, 4 * n}, {&s->star_r, 4 * n}, {&s->ocodes, 4 * n}, {&s->oidx, 4 * n},
+                {&s->ocodes_alt, 4 * n}, {&s->oidx_alt, 4 * n},
+                // radix-sort histograms: 256 per 4096-key tile, plus the scan's chunk sums
+                {&s->ohist, 4 * (256 * ((n + 4095) / 4096) + (256 * ((n + 4095) / 4096) + 4095) / 4096 + 8)}};
     for (auto& b : bufs) {
         if ((e = b.b->ensure(b.bytes)) != hipSuccess) {
             delete s;
