@@ -16,7 +16,7 @@
 //                     (bvh.cuh:459-476), so no uniqueness pass or host
 //                     round trip is needed.
 //   k_refit           bottom-up AABB + cone merge over subtrees of at most
-//                     256 leaves in LDS, then k_refit_top above them (the
+//                     1024 leaves in LDS, then k_refit_top above them (the
 //                     parent box centroid feeds the cone merge, as in
 //                     bvh.cuh:598); writes the reference-layout node data and
 //                     the packed traversal records.
@@ -409,7 +409,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const 
     return 32 + __clz((idx[a] - off) ^ (idx[b] - off));
 }
 
-constexpr uint32_t RT = 256; // leaves of one refit subtree (threads of one refit workgroup)
+constexpr uint32_t RT = 1024; // leaves of one refit subtree ("cut")
+constexpr uint32_t RB = 512;  // threads of one k_refit workgroup (RT / RB leaves each)
 
 __device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_t node)
 {
@@ -574,18 +575,17 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
 //
 // k_karras lists the maximal subtrees of at most RT leaves ("cuts"); they are
 // disjoint and cover every leaf of every mesh.  k_refit gives each cut to one
-// workgroup: thread t initialises leaf first + t, then climbs; the arrival
-// counter of an internal node lives in LDS at (Karras position - first), and
-// the box and cone of every node of the cut are kept in LDS too, so the
-// second arrival merges its children without a trip to memory.
-// Workgroup-scope fences order it.  k_refit_top: one workgroup writes the
-// nodes above the cuts ("tops", about 2N/RT of them, listed by k_karras).
-// With their parent links, children and pending-children counts in LDS it
-// goes level by level: each round writes, in parallel, every top whose top
-// children are written, and a barrier separates the rounds.  With more than
-// TOP_LDS tops it climbs from the cut roots with arrival counters in memory
-// instead.  All of it is touched by this one workgroup, so workgroup scope
-// orders it, and the launch boundary publishes the first launch's results.
+// workgroup, which keeps the topology of the cut and the box and cone of its
+// live nodes in LDS.  It merges level by level: the leaves arrive at their
+// parents (an LDS arrival counter per internal node), the second arrival
+// queues the parent, and each round merges the queued nodes with consecutive
+// threads, so the merge math runs on full wavefronts; a barrier separates the
+// rounds.  k_refit_top does the same for the nodes above the cuts ("tops",
+// about 2N/RT of them, listed by k_karras) in one workgroup, with the boxes
+// in LDS when they fit and in memory otherwise (and, for very large scenes,
+// a climb from the cut roots with arrival counters in memory).  All of it
+// is touched by one workgroup, so workgroup scope orders it, and the launch
+// boundary publishes the first launch's results.
 
 // ids of one mesh: global node id <-> local id / sorted position
 struct MeshIds {
@@ -692,7 +692,7 @@ __device__ __forceinline__ uint16_t local_parent(const MeshIds& id, uint32_t p, 
 }
 
 template <int DIM>
-__global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
+__global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
                                               const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
                                               const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
@@ -703,11 +703,12 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
                                               float4* __restrict__ tleaf, const uint32_t* __restrict__ cuts,
                                               uint32_t cut_cap)
 {
+    constexpr uint32_t LPT = RT / RB; // leaves per thread
     __shared__ NodeLds nd;
     __shared__ uint2 lch[RT];         // children of the internal node at Karras position first + k
     __shared__ uint16_t lslot[RT];    // its slot (range.x - first)
     __shared__ uint16_t lpar[RT];     // its parent (Karras position - first), or NO_PARENT for the cut root
-    __shared__ uint16_t lpl[RT];      // parent of leaf first + t, or NO_PARENT
+    __shared__ uint16_t lpl[RT];      // parent of leaf first + u, or NO_PARENT
     __shared__ uint32_t lflag[RT];    // arrivals at internal node k
     __shared__ uint16_t queue[RT];
     __shared__ RoundQueue rq;
@@ -727,71 +728,82 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
             first = last = id.pos(root);
         }
         const bool ok = last - first < RT && last < N_all; // holds for every cut of a Karras tree
-        const uint32_t j = first + t;
-        lflag[t] = 0u;
         if (t < 3)
             rq.n[t] = 0u;
         // the cut's topology, one coalesced load per array (an internal node of
         // the cut sits at a Karras position in [first, last]; the position not
         // used by the cut may name a node outside it, which is never read back)
-        if (ok && j <= last) {
-            const uint32_t leaf = D.nb + (D.n - 1) + (j - D.off);
-            lpl[t] = leaf == root ? NO_PARENT : local_parent(id, parent[leaf], first, last);
-            if (j - D.off < D.n - 1) {
-                const uint32_t ni = D.nb + (j - D.off);
-                const uint2 ch = child[ni];
-                lch[t] = ch;
-                lslot[t] = (uint16_t)min(range[ni].x - first, RT - 1);
-                lpar[t] = ni == root ? NO_PARENT : local_parent(id, parent[ni], first, last);
+#pragma unroll
+        for (uint32_t h = 0; h < LPT; ++h) {
+            const uint32_t u = t + h * RB, j = first + u;
+            lflag[u] = 0u;
+            if (ok && j <= last) {
+                const uint32_t leaf = D.nb + (D.n - 1) + (j - D.off);
+                lpl[u] = leaf == root ? NO_PARENT : local_parent(id, parent[leaf], first, last);
+                if (j - D.off < D.n - 1) {
+                    const uint32_t ni = D.nb + (j - D.off);
+                    lch[u] = child[ni];
+                    lslot[u] = (uint16_t)min(range[ni].x - first, RT - 1);
+                    lpar[u] = ni == root ? NO_PARENT : local_parent(id, parent[ni], first, last);
+                }
             }
         }
-        bool leaf_ok = false;
-        if (ok && j <= last) {
+        bool leaf_ok[LPT];
+#pragma unroll
+        for (uint32_t h = 0; h < LPT; ++h) {
+            const uint32_t u = t + h * RB, j = first + u;
+            leaf_ok[h] = false;
+            if (!(ok && j <= last))
+                continue;
             const uint32_t orig = sidx[j];
-            if (orig - D.off < D.n) { // a permutation of the mesh's primitives; else the node stays unwritten
-                leaf_ok = true;
-                const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
-                const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
-                const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
-                nbox[2 * node] = b0;
-                nbox[2 * node + 1] = b1;
-                ncone[2 * node] = c0;
-                ncone[2 * node + 1] = c1;
-                {
-                    box3 b;
-                    b.lo = mk(b0.x, b0.y, b0.z);
-                    b.hi = mk(b1.x, b1.y, b1.z);
-                    cone3 cc;
-                    cc.axis = mk(c0.x, c0.y, c0.z);
-                    cc.half_angle = c0.w;
-                    cc.radius = c1.x;
-                    nd.put(t, b, cc);
-                }
-                const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
-                const uint32_t lo = orig - D.off, les = es - D.eoff;
-                if (DIM == 3) {
-                    const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], d = V[F[3 * orig + 2]];
-                    tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(lo));
-                    tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(les));
-                    tleaf[3 * j + 2] = make_float4(d.x, d.y, d.z, __uint_as_float(ec));
-                } else {
-                    const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
-                    tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(lo));
-                    tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(les));
-                    tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
-                }
+            if (orig - D.off >= D.n)
+                continue; // not a permutation of the mesh's primitives: the node stays unwritten
+            leaf_ok[h] = true;
+            const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
+            const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
+            const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
+            nbox[2 * node] = b0;
+            nbox[2 * node + 1] = b1;
+            ncone[2 * node] = c0;
+            ncone[2 * node + 1] = c1;
+            {
+                box3 b;
+                b.lo = mk(b0.x, b0.y, b0.z);
+                b.hi = mk(b1.x, b1.y, b1.z);
+                cone3 cc;
+                cc.axis = mk(c0.x, c0.y, c0.z);
+                cc.half_angle = c0.w;
+                cc.radius = c1.x;
+                nd.put(u, b, cc);
+            }
+            const uint32_t es = estart[orig], ec = estart[orig + 1] - es;
+            const uint32_t lo = orig - D.off, les = es - D.eoff;
+            if (DIM == 3) {
+                const float4 a = V[F[3 * orig]], b = V[F[3 * orig + 1]], d = V[F[3 * orig + 2]];
+                tleaf[3 * j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(lo));
+                tleaf[3 * j + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(les));
+                tleaf[3 * j + 2] = make_float4(d.x, d.y, d.z, __uint_as_float(ec));
+            } else {
+                const float4 a = V[F[2 * orig]], b = V[F[2 * orig + 1]];
+                tleaf[3 * j + 0] = make_float4(a.x, a.y, 0.0f, __uint_as_float(lo));
+                tleaf[3 * j + 1] = make_float4(b.x, b.y, 0.0f, __uint_as_float(les));
+                tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
             }
         }
         __syncthreads(); // topology, leaf slots and cleared counters
         // round 0: the leaves arrive at their parents
-        if (leaf_ok && lpl[t] != NO_PARENT && atomicAdd(&lflag[lpl[t]], 1u) == 1u)
-            rq.push(queue, 0u, 0u, lpl[t]);
+#pragma unroll
+        for (uint32_t h = 0; h < LPT; ++h) {
+            const uint32_t u = t + h * RB;
+            if (leaf_ok[h] && lpl[u] != NO_PARENT && atomicAdd(&lflag[lpl[u]], 1u) == 1u)
+                rq.push(queue, 0u, 0u, lpl[u]);
+        }
         __syncthreads();
         uint32_t qs = 0, qe = rq.n[0];
         for (uint32_t round = 1; qs < qe && round < RT; ++round) {
             if (t == 0)
                 rq.n[(round + 1) % 3] = 0u;
-            for (uint32_t i = qs + t; i < qe; i += RT) {
+            for (uint32_t i = qs + t; i < qe; i += RB) {
                 const uint32_t k = queue[i];
                 const uint2 ch = lch[k];
                 const uint32_t ls = lslot[k], rs = min(id.pos(ch.x) + 1u - first, RT - 1);
@@ -815,10 +827,49 @@ __global__ __launch_bounds__(RT) void k_refit(const float4* __restrict__ V, cons
     }
 }
 
-// tops held in LDS by k_refit_top, 14 B each: the children, a word (parent
-// slot << 16 | top children not yet written) and a queue entry
+// k_refit_top keeps, when the scene has at most TOP_BOX cuts and tops, the
+// box and cone of every live node in LDS (slot: the cut's index; a top takes
+// over its left child's slot) with each top's parent slot and arrival count,
+// children and children's slots and a queue entry, 18 B per top.  Up to
+// TOP_LDS tops, the topology alone (14 B per top) sits in LDS and the boxes
+// are read from memory.  Beyond that it climbs from the cut roots with
+// arrival counters in memory.
+constexpr uint32_t TOP_BOX = 2048;
 constexpr uint32_t TOP_LDS = 11264;
 constexpr uint32_t TOP_ROUNDS = 128; // a Karras tree over 64-bit keys is at most 64 levels deep
+constexpr uint32_t TOP_SMEM = TOP_LDS * 14;
+static_assert(TOP_BOX * (44 + 18) <= TOP_SMEM, "LDS layouts of k_refit_top");
+
+// topology of the tops: tword[k] = parent slot << 16 (0xFFFF: a mesh root),
+// tch[k] = children
+__device__ __forceinline__ void load_tops(const uint32_t* __restrict__ parent, const uint2* __restrict__ child,
+                                          const uint32_t* flags, const uint32_t* __restrict__ tops, uint32_t n_top,
+                                          uint32_t n_nodes, uint32_t* tword, uint2* tch)
+{
+    for (uint32_t base = threadIdx.x; base < n_top; base += 4 * 1024) {
+        uint32_t p[4], pp[4];
+        uint2 ch[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t k = base + 1024u * u;
+            p[u] = k < n_top ? tops[k] : INVALID;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pp[u] = p[u] < n_nodes ? parent[p[u]] : INVALID;
+            ch[u] = p[u] < n_nodes ? child[p[u]] : make_uint2(INVALID, INVALID);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t k = base + 1024u * u;
+            if (k < n_top) {
+                const uint32_t ps = pp[u] < n_nodes ? flags[pp[u]] : 0xFFFFu;
+                tword[k] = min(ps, 0xFFFFu) << 16;
+                tch[k] = ch[u];
+            }
+        }
+    }
+}
 
 template <int DIM>
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ parent,
@@ -830,40 +881,91 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
                                                     float4* __restrict__ tcone, const uint32_t* __restrict__ cuts,
                                                     uint32_t cut_cap)
 {
-    __shared__ uint32_t tword[TOP_LDS];
-    __shared__ uint2 tch[TOP_LDS];
-    __shared__ uint16_t queue[TOP_LDS];
+    __shared__ __align__(16) unsigned char smem[TOP_SMEM];
     __shared__ RoundQueue rq;
     const uint32_t t = threadIdx.x;
     const uint32_t n_top = min(tops[top_cap], top_cap);
-    if (n_top <= TOP_LDS) {
-        // level by level: a round writes the tops whose top children are
-        // written (the cut roots below them were written by k_refit)
-        if (t < 3)
-            rq.n[t] = 0u;
-        for (uint32_t base = t; base < n_top; base += 4 * 1024) {
-            uint32_t p[4], pp[4];
-            uint2 ch[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t k = base + 1024u * u;
-                p[u] = k < n_top ? tops[k] : INVALID;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                pp[u] = p[u] < n_nodes ? parent[p[u]] : INVALID;
-                ch[u] = p[u] < n_nodes ? child[p[u]] : make_uint2(INVALID, INVALID);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t k = base + 1024u * u;
-                if (k < n_top) {
-                    const uint32_t ps = pp[u] < n_nodes ? flags[pp[u]] : 0xFFFFu;
-                    tword[k] = min(ps, 0xFFFFu) << 16;
-                    tch[k] = ch[u];
+    const uint32_t n_cut = min(cuts[cut_cap], cut_cap);
+    if (t < 3)
+        rq.n[t] = 0u;
+    if (n_top <= TOP_BOX && n_cut <= TOP_BOX) {
+        float(*bx)[TOP_BOX] = reinterpret_cast<float(*)[TOP_BOX]>(smem); // 11 rows: box, cone
+        uint2* tch = reinterpret_cast<uint2*>(smem + 44 * TOP_BOX);
+        uint32_t* tword = reinterpret_cast<uint32_t*>(smem + 52 * TOP_BOX);
+        uint32_t* tslot = reinterpret_cast<uint32_t*>(smem + 56 * TOP_BOX); // children's slots, left | right << 16
+        uint16_t* queue = reinterpret_cast<uint16_t*>(smem + 60 * TOP_BOX);
+        auto put = [&](uint32_t sl, const box3& b, const cone3& c) {
+            bx[0][sl] = b.lo.x; bx[1][sl] = b.lo.y; bx[2][sl] = b.lo.z;
+            bx[3][sl] = b.hi.x; bx[4][sl] = b.hi.y; bx[5][sl] = b.hi.z;
+            bx[6][sl] = c.axis.x; bx[7][sl] = c.axis.y; bx[8][sl] = c.axis.z;
+            bx[9][sl] = c.half_angle; bx[10][sl] = c.radius;
+        };
+        auto get = [&](uint32_t sl, box3& b, cone3& c) {
+            b.lo = mk(bx[0][sl], bx[1][sl], bx[2][sl]);
+            b.hi = mk(bx[3][sl], bx[4][sl], bx[5][sl]);
+            c.axis = mk(bx[6][sl], bx[7][sl], bx[8][sl]);
+            c.half_angle = bx[9][sl];
+            c.radius = bx[10][sl];
+        };
+        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch);
+        for (uint32_t k = t; k < n_top; k += 1024)
+            tslot[k] = 0u;
+        __syncthreads();
+        // the cut roots (written by k_refit) take their slots and arrive
+        for (uint32_t c = t; c < n_cut; c += 1024) {
+            const uint32_t r = cuts[c];
+            if (r >= n_nodes)
+                continue;
+            put(c, ldbox(nbox, r), ldcone(ncone, r));
+            const uint32_t pp = parent[r];
+            const uint32_t kp = pp < n_nodes ? flags[pp] : INVALID;
+            if (kp >= n_top)
+                continue; // a mesh that is one cut
+            atomicOr(&tslot[kp], tch[kp].x == r ? c : c << 16);
+            if ((atomicAdd(&tword[kp], 1u) & 0xFFFFu) == 1u)
+                rq.push(queue, 0u, 0u, kp);
+        }
+        __syncthreads();
+        uint32_t qs = 0, qe = rq.n[0];
+        for (uint32_t round = 1; qs < qe && round < TOP_ROUNDS; ++round) {
+            if (t == 0)
+                rq.n[(round + 1) % 3] = 0u;
+            for (uint32_t i = qs + t; i < qe; i += 1024) {
+                const uint32_t k = queue[i];
+                const uint32_t p = tops[k];
+                const uint2 ch = tch[k];
+                const uint32_t ls = min(tslot[k] & 0xFFFFu, TOP_BOX - 1), rs = min(tslot[k] >> 16, TOP_BOX - 1);
+                const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
+                const MeshIds id{D.off, D.n, D.nb};
+                box3 lb, rb, pb;
+                cone3 lc, rc, pc;
+                get(ls, lb, lc);
+                get(rs, rb, rc);
+                refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
+                                 tnode, tcone);
+                put(ls, pb, pc);
+                const uint32_t kp = tword[k] >> 16;
+                if (kp < n_top) {
+                    atomicOr(&tslot[kp], tch[kp].x == p ? ls : ls << 16);
+                    if ((atomicAdd(&tword[kp], 1u) & 0xFFFFu) == 1u)
+                        rq.push(queue, qe, round, kp);
                 }
             }
+            __syncthreads(); // this round's slots precede the next round's reads
+            const uint32_t added = rq.n[round % 3];
+            qs = qe;
+            qe += added;
         }
+        return;
+    }
+    if (n_top <= TOP_LDS) {
+        // level by level with the boxes in memory: a round writes the tops
+        // whose top children are written (the cut roots below them were
+        // written by k_refit); tword counts the top children not yet written
+        uint32_t* tword = reinterpret_cast<uint32_t*>(smem);
+        uint2* tch = reinterpret_cast<uint2*>(smem + 4 * TOP_LDS);
+        uint16_t* queue = reinterpret_cast<uint16_t*>(smem + 12 * TOP_LDS);
+        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch);
         __syncthreads();
         for (uint32_t k = t; k < n_top; k += 1024) {
             const uint32_t ps = tword[k] >> 16;
@@ -906,8 +1008,7 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
     for (uint32_t k = t; k < n_top; k += 1024)
         flags[tops[k]] = 0u;
     __syncthreads();
-    const uint32_t count = min(cuts[cut_cap], cut_cap);
-    for (uint32_t k = t; k < count; k += 1024) {
+    for (uint32_t k = t; k < n_cut; k += 1024) {
         uint32_t node = cuts[k];
         if (node >= n_nodes)
             continue;
@@ -1028,11 +1129,11 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
         // as many workgroups as are resident at once (LDS-bound), or one per cut
         static int per_cu = 0;
         if (per_cu == 0 &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_refit<DIM>, (int)RT, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_refit<DIM>, (int)RB, 0) != hipSuccess)
             per_cu = 1;
         const uint32_t refit_blocks =
             std::max(1u, std::min(2u * ((N + RT - 1) / RT) + M, (uint32_t)std::max(per_cu, 1) * n_cu));
-        hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RT), 0, st, s->dV.as<float4>(),
+        hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RB), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
                            desc, M, N, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),

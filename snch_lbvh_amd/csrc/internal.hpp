@@ -157,6 +157,8 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt,
                             const uint32_t* seg = nullptr, int seg_bits = 0);
+// bytes of `hist` a sort of n pairs over `passes` 8-bit digits uses
+size_t radix_sort_scratch_bytes(uint32_t n, int passes);
 // query.hip (dim = 2 or 3).  perm (nullable): slot -> query index, the
 // order in which queries are visited (query_order computes a Morton order)
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,

@@ -1,18 +1,27 @@
 // Stable LSD radix sort of (uint32 key, uint32 value) pairs for gfx950.
 //
-// Replaces thrust::stable_sort_by_key in bvh.cuh:452-454.  Each 8-bit pass is
-// reduce-then-scan:
-//   upsweep    one tile (256 threads x 16 keys) -> 256-bin digit histogram,
-//              stored digit-major so one exclusive scan gives every tile's
-//              global write offset per digit;
-//   scan       single-block exclusive scan of the digit-major histograms;
-//   downsweep  each wave ranks its 64-key rows with eight wave64 ballots (the
-//              lanes holding the same digit form a peer mask), keeps per-wave
-//              running digit counts in LDS, adds the per-wave prefix and the
-//              tile offset, and scatters.
+// Replaces thrust::stable_sort_by_key in bvh.cuh:452-454.  One sweep per
+// 8-bit digit, after one histogram pass:
+//   k_sort_init   clears the per-pass digit totals and tile counters;
+//   k_hist_all    one read of the keys (and segment ids) counts the digits
+//                 of every pass at once, and clears the look-back words;
+//   k_onesweep    per pass: a workgroup takes the next tile (4096 pairs) in
+//                 launch order, ranks its keys with eight wave64 ballots per
+//                 64-key row (the lanes holding the same digit form a peer
+//                 mask) and per-wave running counts in LDS, publishes the
+//                 tile's digit counts, and finds the counts of all tiles
+//                 before it by decoupled look-back (Merrill and Garland,
+//                 "Single-pass Parallel Prefix Scan with Decoupled
+//                 Look-back", 2016): a predecessor's word holds either its
+//                 own count (aggregate) or the count of every tile up to it
+//                 (inclusive), flag and count in one 32-bit word.  The digit
+//                 base (an exclusive scan of the pass's totals) plus that
+//                 prefix plus the in-tile rank is each pair's destination.
 // Ranks follow tile order (wave, row, lane), so equal keys keep their input
 // order: the sort is stable, which the reference relies on (objects with
-// equal Morton codes stay in index order).
+// equal Morton codes stay in index order).  Tiles are numbered by an atomic
+// counter as workgroups start, so every tile a workgroup waits on has
+// started before it; the wait is bounded all the same.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -21,49 +30,84 @@ namespace snch {
 
 namespace {
 
-constexpr int ST = 256;          // threads per tile
-constexpr int SI = 16;           // keys per thread
+constexpr int ST = 512;          // threads per tile
+constexpr int SI = 8;            // keys per thread
 constexpr int TILE = ST * SI;    // 4096 keys
 constexpr int RADIX = 256;
 constexpr int WAVES = ST / 64;
+constexpr int MAX_PASSES = 8;    // up to 4 key digits and 4 segment digits
+constexpr uint32_t LB_AGG = 1u << 30, LB_INC = 2u << 30, LB_COUNT = (1u << 30) - 1u;
+constexpr uint32_t LB_SPIN = 1u << 22; // bound on polls of one look-back word
+constexpr uint32_t LB_WIN = 8;         // look-back words read at once
 
-// SEG: the digit comes from seg[value] (a segment id per value) instead of
-// the key, which sorts stably by segment after the key passes
-template <bool SEG>
-__global__ __launch_bounds__(ST) void k_upsweep(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                 const uint32_t* __restrict__ seg, uint32_t n, int shift,
-                                                 uint32_t ntiles, uint32_t* __restrict__ hist)
+__global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__ tile_ctr, uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t h[RADIX];
+    for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += blockDim.x)
+        ghist[i] = 0u;
+    if (threadIdx.x < MAX_PASSES)
+        tile_ctr[threadIdx.x] = 0u;
+    if (threadIdx.x == 0)
+        *err = 0u;
+}
+
+// digit counts of every pass (pass p < key_passes: key byte p; then byte
+// p - key_passes of seg[value]); clears this tile's look-back words
+__global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                 const uint32_t* __restrict__ seg, uint32_t n, int key_passes,
+                                                 int seg_passes, uint32_t ntiles, uint32_t* __restrict__ ghist,
+                                                 uint32_t* __restrict__ status)
+{
+    __shared__ uint32_t h[MAX_PASSES][RADIX];
     const uint32_t tid = threadIdx.x;
-    h[tid] = 0;
+    const int passes = key_passes + seg_passes;
+    for (uint32_t i = tid; i < MAX_PASSES * RADIX; i += ST)
+        h[i / RADIX][i % RADIX] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * TILE;
 #pragma unroll 4
     for (int i = 0; i < SI; ++i) {
-        uint32_t pos = base + i * ST + tid;
-        if (pos < n)
-            atomicAdd(&h[((SEG ? seg[vals[pos]] : keys[pos]) >> shift) & 0xFFu], 1u);
+        const uint32_t pos = base + i * ST + tid;
+        if (pos >= n)
+            continue;
+        const uint32_t k = keys[pos];
+        for (int p = 0; p < key_passes; ++p)
+            atomicAdd(&h[p][(k >> (8 * p)) & 0xFFu], 1u);
+        if (seg_passes > 0) {
+            const uint32_t sg = seg[vals[pos]];
+            for (int q = 0; q < seg_passes; ++q)
+                atomicAdd(&h[key_passes + q][(sg >> (8 * q)) & 0xFFu], 1u);
+        }
     }
     __syncthreads();
-    hist[tid * ntiles + blockIdx.x] = h[tid];
+    if (tid < RADIX)
+        for (int p = 0; p < passes; ++p) {
+            if (h[p][tid] != 0u)
+                atomicAdd(&ghist[p * RADIX + tid], h[p][tid]);
+            status[((size_t)p * ntiles + blockIdx.x) * RADIX + tid] = 0u;
+        }
 }
 
-// exclusive scan of a[0..len) (len a multiple of 4) by one block of 1024
-// threads: coalesced 4096-element tiles with a running carry
-__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ a, uint32_t len)
+// one pass.  SEG: the digit comes from seg[value] instead of the key
+template <bool SEG>
+__global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                  const uint32_t* __restrict__ seg, uint32_t* __restrict__ kout,
+                                                  uint32_t* __restrict__ vout, uint32_t n, int shift,
+                                                  const uint32_t* __restrict__ ghist, uint32_t* status,
+                                                  uint32_t* tile_ctr, uint32_t* err)
 {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
+    __shared__ uint32_t wcnt[WAVES][RADIX];
+    __shared__ uint32_t toff[RADIX];
+    __shared__ uint32_t wsum[WAVES];
+    __shared__ uint32_t tile_s;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < len; base += 4096) {
-        const uint32_t idx = base + 4 * tid;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (idx < len)
-            v = *reinterpret_cast<const uint4*>(a + idx);
-        const uint32_t s = v.x + v.y + v.z + v.w;
-        uint32_t incl = s;
+    if (tid == 0)
+        tile_s = atomicAdd(tile_ctr, 1u);
+    for (uint32_t i = tid; i < WAVES * RADIX; i += ST)
+        wcnt[i / RADIX][i % RADIX] = 0u;
+    // digit base: exclusive scan of the pass's digit totals (threads < RADIX)
+    {
+        const uint32_t x = tid < RADIX ? ghist[tid] : 0u;
+        uint32_t incl = x;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(incl, o, 64);
@@ -73,105 +117,14 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ 
         if (lane == 63)
             wsum[w] = incl;
         __syncthreads();
-        if (w == 0) {
-            const uint32_t x0 = lane < 16 ? wsum[lane] : 0u;
-            uint32_t x = x0;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= (uint32_t)o)
-                    x += y;
-            }
-            if (lane < 16)
-                wsum[lane] = x - x0;
-        }
-        __syncthreads();
-        const uint32_t ex = carry + wsum[w] + incl - s;
-        if (idx < len)
-            *reinterpret_cast<uint4*>(a + idx) = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
-        if (tid == 1023)
-            carry_s = ex + s;
-        __syncthreads();
-        carry = carry_s;
-        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t q = 0; q < w; ++q)
+            before += wsum[q];
+        if (tid < RADIX)
+            toff[tid] = before + incl - x;
     }
-}
-
-// multi-block exclusive scan for long histograms: per-chunk sums, a scan of
-// the sums, then each chunk scanned with its offset (chunks of 4096)
-__device__ __forceinline__ uint32_t block_sum_1024(uint32_t x, uint32_t* wsum)
-{
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        x += __shfl_xor(x, o, 64);
-    if (lane == 0)
-        wsum[w] = x;
-    __syncthreads();
-    uint32_t t = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-        t += wsum[k];
-    return t;
-}
-
-__global__ __launch_bounds__(1024) void k_chunk_sums(const uint32_t* __restrict__ a, uint32_t len,
-                                                     uint32_t* __restrict__ sums)
-{
-    __shared__ uint32_t wsum[16];
-    const uint32_t idx = blockIdx.x * 4096 + 4 * threadIdx.x;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (idx < len)
-        v = *reinterpret_cast<const uint4*>(a + idx);
-    const uint32_t t = block_sum_1024(v.x + v.y + v.z + v.w, wsum);
-    if (threadIdx.x == 0)
-        sums[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* __restrict__ a, uint32_t len,
-                                                     const uint32_t* __restrict__ offs)
-{
-    __shared__ uint32_t wsum[16];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t idx = blockIdx.x * 4096 + 4 * tid;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (idx < len)
-        v = *reinterpret_cast<const uint4*>(a + idx);
-    const uint32_t s = v.x + v.y + v.z + v.w;
-    uint32_t incl = s;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o)
-            incl += y;
-    }
-    if (lane == 63)
-        wsum[w] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t k = 0; k < w; ++k)
-        before += wsum[k];
-    const uint32_t ex = offs[blockIdx.x] + before + incl - s;
-    if (idx < len)
-        *reinterpret_cast<uint4*>(a + idx) = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
-}
-
-template <bool SEG>
-__global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                   const uint32_t* __restrict__ seg, uint32_t* __restrict__ kout,
-                                                   uint32_t* __restrict__ vout, uint32_t n, int shift,
-                                                   uint32_t ntiles, const uint32_t* __restrict__ offs)
-{
-    __shared__ uint32_t wcnt[WAVES][RADIX];
-    __shared__ uint32_t toff[RADIX];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int k = 0; k < WAVES; ++k)
-        wcnt[k][tid] = 0;
-    toff[tid] = offs[tid * ntiles + blockIdx.x];
-    __syncthreads();
-
-    const uint32_t base = blockIdx.x * TILE + w * 64 * SI;
+    const uint32_t tile = tile_s;
+    const uint32_t base = tile * TILE + w * 64 * SI;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t k[SI], v[SI], rank[SI];
 #pragma unroll
@@ -206,12 +159,52 @@ __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ k
         rank[i] = before + r;
     }
     __syncthreads();
-    {
-        uint32_t c[WAVES];
+    if (tid < RADIX) {
+        // thread tid = digit: this tile's count, then the count of every
+        // tile before it (decoupled look-back)
+        uint32_t c[WAVES], own = 0;
 #pragma unroll
-        for (int q = 0; q < WAVES; ++q)
+        for (int q = 0; q < WAVES; ++q) {
             c[q] = wcnt[q][tid];
-        uint32_t run = toff[tid];
+            own += c[q];
+        }
+        uint32_t* word = status + (size_t)tile * RADIX + tid;
+        __hip_atomic_store(word, (tile == 0 ? LB_INC : LB_AGG) | own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // predecessors are read LB_WIN at a time (independent loads), nearest
+        // first, until one holds an inclusive count
+        uint32_t excl = 0;
+        bool done = tile == 0;
+        for (uint32_t pt = tile; !done;) {
+            uint32_t sv[LB_WIN];
+#pragma unroll
+            for (uint32_t u = 0; u < LB_WIN; ++u)
+                sv[u] = u < pt ? __hip_atomic_load(status + (size_t)(pt - 1 - u) * RADIX + tid, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : LB_INC;
+#pragma unroll
+            for (uint32_t u = 0; u < LB_WIN; ++u) {
+                if (done)
+                    continue;
+                uint32_t x = sv[u];
+                for (uint32_t spin = 0; (x & ~LB_COUNT) == 0u; ++spin) {
+                    if (spin >= LB_SPIN) { // never reached when every started tile finishes
+                        atomicOr(err, 1u);
+                        x = LB_INC;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    x = __hip_atomic_load(status + (size_t)(pt - 1 - u) * RADIX + tid, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                }
+                excl += x & LB_COUNT; // 0 for the LB_INC stand-in below tile 0
+                done = (x & LB_INC) != 0u;
+            }
+            pt = pt > LB_WIN ? pt - LB_WIN : 0u;
+            done = done || pt == 0u;
+        }
+        if (tile > 0)
+            __hip_atomic_store(word, LB_INC | (excl + own), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t run = toff[tid] + excl;
 #pragma unroll
         for (int q = 0; q < WAVES; ++q) {
             wcnt[q][tid] = run;
@@ -235,6 +228,12 @@ __global__ __launch_bounds__(ST) void k_downsweep(const uint32_t* __restrict__ k
 
 } // namespace
 
+size_t radix_sort_scratch_bytes(uint32_t n, int passes)
+{
+    const size_t ntiles = (n + TILE - 1) / TILE;
+    return sizeof(uint32_t) * ((size_t)MAX_PASSES * RADIX + 2 * MAX_PASSES + (size_t)passes * ntiles * RADIX);
+}
+
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt, const uint32_t* seg,
                             int seg_bits)
@@ -242,41 +241,34 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     *result_in_alt = false;
     if (n <= 1)
         return hipSuccess;
+    const int key_passes = (bits + 7) / 8, seg_passes = seg ? (seg_bits + 7) / 8 : 0;
+    const int passes = key_passes + seg_passes;
+    if (key_passes > 4 || seg_passes > 4 || n > LB_COUNT)
+        return hipErrorInvalidValue;
+    if (passes == 0)
+        return hipSuccess;
     const uint32_t ntiles = (n + TILE - 1) / TILE;
-    hipError_t e = hist.ensure(sizeof(uint32_t) * (size_t)RADIX * ntiles);
+    hipError_t e = hist.ensure(radix_sort_scratch_bytes(n, passes));
     if (e != hipSuccess)
         return e;
-    const uint32_t hlen = RADIX * ntiles, nchunks = (hlen + 4095) / 4096;
-    const bool chunked = nchunks > 4;
-    if (chunked) {
-        e = hist.ensure(sizeof(uint32_t) * ((size_t)hlen + ((nchunks + 3) & ~3u)));
-        if (e != hipSuccess)
-            return e;
-    }
-    uint32_t* h = hist.as<uint32_t>();
-    uint32_t* sums = h + hlen;
+    uint32_t* ghist = hist.as<uint32_t>();
+    uint32_t* tile_ctr = ghist + MAX_PASSES * RADIX;
+    uint32_t* err = tile_ctr + MAX_PASSES;
+    uint32_t* status = err + MAX_PASSES;
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr, err);
+    hipLaunchKernelGGL(k_hist_all, dim3(ntiles), dim3(ST), 0, st, keys, vals, seg, n, key_passes, seg_passes, ntiles,
+                       ghist, status);
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
-    const int key_passes = (bits + 7) / 8, seg_passes = seg ? (seg_bits + 7) / 8 : 0;
-    for (int pass = 0; pass < key_passes + seg_passes; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         const bool sp = pass >= key_passes;
         const int shift = 8 * (sp ? pass - key_passes : pass);
+        uint32_t* pstat = status + (size_t)pass * ntiles * RADIX;
         if (sp)
-            hipLaunchKernelGGL(k_upsweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, n, shift, ntiles, h);
+            hipLaunchKernelGGL(k_onesweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
+                               ghist + pass * RADIX, pstat, tile_ctr + pass, err);
         else
-            hipLaunchKernelGGL(k_upsweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, n, shift, ntiles, h);
-        if (chunked) {
-            hipLaunchKernelGGL(k_chunk_sums, dim3(nchunks), dim3(1024), 0, st, h, hlen, sums);
-            hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, sums, (nchunks + 3) & ~3u);
-            hipLaunchKernelGGL(k_chunk_scan, dim3(nchunks), dim3(1024), 0, st, h, hlen, sums);
-        } else {
-            hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, h, hlen);
-        }
-        if (sp)
-            hipLaunchKernelGGL(k_downsweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift, ntiles,
-                               h);
-        else
-            hipLaunchKernelGGL(k_downsweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
-                               ntiles, h);
+            hipLaunchKernelGGL(k_onesweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
+                               ghist + pass * RADIX, pstat, tile_ctr + pass, err);
         uint32_t* t = ki;
         ki = ko;
         ko = t;

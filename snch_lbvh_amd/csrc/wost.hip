@@ -371,8 +371,8 @@ int snch_wost_create(snch_context* ctx, const snch_scene* dirichlet, const snch_
                 {&s->ray_idx, 4 * n}, {&s->sph_r, 4 * n},  {&s->sph_u, 12 * n}, {&s->smp_idx, 4 * n},
                 {&s->smp_pdf, 4 * n}, {&s->star_r, 4 * n}, {&s->ocodes, 4 * n}, {&s->oidx, 4 * n},
                 {&s->ocodes_alt, 4 * n}, {&s->oidx_alt, 4 * n},
-                // radix-sort histograms: 256 per 4096-key tile, plus the scan's chunk sums
-                {&s->ohist, 4 * (256 * ((n + 4095) / 4096) + (256 * ((n + 4095) / 4096) + 4095) / 4096 + 8)}};
+                // radix-sort digit counts and look-back words (four 8-bit passes)
+                {&s->ohist, radix_sort_scratch_bytes(n, 4)}};
     for (auto& b : bufs) {
         if ((e = b.b->ensure(b.bytes)) != hipSuccess) {
             delete s;

@@ -86,14 +86,26 @@ def test_build_bit_exact_large(S, oracle):
 
 
 def test_build_bit_exact_many_tops(S, oracle):
-    """2,096,704 triangles: about 14,000 nodes lie above the 256-leaf refit
-    subtrees, more than the top pass keeps in LDS, so it climbs with arrival
-    counters in memory; the tree must still equal the oracle's."""
+    """2,096,704 triangles: about 3,500 refit subtrees of up to 1,024 leaves,
+    more than the top pass keeps boxes for in LDS, so it reads them from
+    memory; the tree must still equal the oracle's."""
     V, F = W.cut_torus(1448, 724)
     g = S.Scene(V, F).build_bvh().export()
     o = oracle.build_tree(V, F)
     for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
         np.testing.assert_array_equal(g[k], ov, err_msg=k)
+
+
+def test_build_bit_exact_top_climb(S, oracle):
+    """8,386,816 triangles: about 14,000 nodes lie above the refit subtrees,
+    more than the top pass keeps in LDS at all, so it climbs from the subtree
+    roots with arrival counters in memory; the tree must equal the oracle's."""
+    V, F = W.cut_torus(2896, 1448)
+    g = S.Scene(V, F).build_bvh().export()
+    o = oracle.build_tree(V, F)
+    for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
+        np.testing.assert_array_equal(g[k], ov, err_msg=k)
+    del g, o
 
 
 def test_build_deterministic(S):
