@@ -620,15 +620,17 @@ __device__ __forceinline__ void refit_write(const MeshIds& id, uint32_t p, uint2
                                             const cone3& lc, const cone3& rc, int conservative,
                                             const uint2* __restrict__ range, uint32_t leaf_size, box3& pb,
                                             cone3& pc, float4* nbox, float4* ncone, float4* __restrict__ tnode,
-                                            float4* __restrict__ tcone)
+                                            float4* __restrict__ tcone, bool own = true)
 {
     pb = box_merge(lb, rb);
     pc = DIM == 3 ? cone_merge(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0)
                   : cone_merge2d(lc, rc, box_centroid(lb), box_centroid(rb), box_centroid(pb), conservative != 0);
-    nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
-    nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
-    ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
-    ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
+    if (own) { // the node's own box and cone in the reference layout (else they live in its parent's record)
+        nbox[2 * p] = make_float4(pb.lo.x, pb.lo.y, pb.lo.z, 0.0f);
+        nbox[2 * p + 1] = make_float4(pb.hi.x, pb.hi.y, pb.hi.z, 0.0f);
+        ncone[2 * p] = make_float4(pc.axis.x, pc.axis.y, pc.axis.z, pc.half_angle);
+        ncone[2 * p + 1] = make_float4(pc.radius, 0.0f, 0.0f, 0.0f);
+    }
     const uint32_t r = p - id.off; // internal record: (off - m) + (p - nb)
     tnode[4 * r + 0] = make_float4(lb.lo.x, lb.lo.y, lb.lo.z, lb.hi.x);
     tnode[4 * r + 1] = make_float4(lb.hi.y, lb.hi.z, rb.lo.x, rb.lo.y);
@@ -762,10 +764,12 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
             const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
             const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
             const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
-            nbox[2 * node] = b0;
-            nbox[2 * node + 1] = b1;
-            ncone[2 * node] = c0;
-            ncone[2 * node + 1] = c1;
+            if (node == root) { // a cut of one leaf; other leaves live in their parents' records
+                nbox[2 * node] = b0;
+                nbox[2 * node + 1] = b1;
+                ncone[2 * node] = c0;
+                ncone[2 * node + 1] = c1;
+            }
             {
                 box3 b;
                 b.lo = mk(b0.x, b0.y, b0.z);
@@ -811,8 +815,9 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
                 cone3 lc, rc, pc;
                 nd.get(ls, lb, lc);
                 nd.get(rs, rb, rc);
-                refit_write<DIM>(id, D.nb + (first + k - D.off), ch, lb, rb, lc, rc, conservative, range, leaf_size,
-                                 pb, pc, nbox, ncone, tnode, tcone);
+                const uint32_t p = D.nb + (first + k - D.off);
+                refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
+                                 tnode, tcone, p == root);
                 nd.put(ls, pb, pc);
                 const uint32_t kp = lpar[k];
                 if (kp != NO_PARENT && atomicAdd(&lflag[kp], 1u) == 1u)

@@ -860,29 +860,51 @@ int snch_scene_export_mesh(const snch_scene* s, uint32_t mesh, uint32_t* nodes, 
         }
     }
     if (boxes || cones) {
-        std::vector<float> bx(8 * (size_t)NN), c(8 * (size_t)NN);
-        SNCH_HIP(hipMemcpy(bx.data(), s->nbox.as<float>() + 8 * (size_t)nb, sizeof(float) * bx.size(),
+        // the root's own box and cone are stored; every other node's sit in
+        // its parent's traversal record (child L or R)
+        std::vector<float> rb(8), rc(8), tn(16 * (size_t)(N - 1)), tc(16 * (size_t)(N - 1));
+        SNCH_HIP(hipMemcpy(rb.data(), s->nbox.as<float>() + 8 * (size_t)nb, sizeof(float) * 8, hipMemcpyDeviceToHost),
+                 "export");
+        SNCH_HIP(hipMemcpy(rc.data(), s->ncone.as<float>() + 8 * (size_t)nb, sizeof(float) * 8,
                            hipMemcpyDeviceToHost),
                  "export");
-        SNCH_HIP(hipMemcpy(c.data(), s->ncone.as<float>() + 8 * (size_t)nb, sizeof(float) * c.size(),
-                           hipMemcpyDeviceToHost),
-                 "export");
-        for (uint32_t i = 0; i < NN; ++i) {
+        if (N > 1) {
+            const size_t rec = 16 * (size_t)(off - mesh); // first internal record of the mesh
+            SNCH_HIP(hipMemcpy(tn.data(), s->tnode.as<float>() + rec, sizeof(float) * tn.size(),
+                               hipMemcpyDeviceToHost),
+                     "export");
+            SNCH_HIP(hipMemcpy(tc.data(), s->tcone.as<float>() + rec, sizeof(float) * tc.size(),
+                               hipMemcpyDeviceToHost),
+                     "export");
+        }
+        auto put = [&](uint32_t i, const float* lo, const float* hi, const float* axis, float ha, float r) {
             if (boxes) {
-                boxes[6 * i + 0] = bx[8 * i + 0];
-                boxes[6 * i + 1] = bx[8 * i + 1];
-                boxes[6 * i + 2] = bx[8 * i + 2];
-                boxes[6 * i + 3] = bx[8 * i + 4];
-                boxes[6 * i + 4] = bx[8 * i + 5];
-                boxes[6 * i + 5] = bx[8 * i + 6];
+                for (int k = 0; k < 3; ++k) {
+                    boxes[6 * (size_t)i + k] = lo[k];
+                    boxes[6 * (size_t)i + 3 + k] = hi[k];
+                }
             }
             if (cones) {
-                cones[5 * i + 0] = c[8 * i + 0];
-                cones[5 * i + 1] = c[8 * i + 1];
-                cones[5 * i + 2] = c[8 * i + 2];
-                cones[5 * i + 3] = c[8 * i + 3];
-                cones[5 * i + 4] = c[8 * i + 4];
+                for (int k = 0; k < 3; ++k)
+                    cones[5 * (size_t)i + k] = axis[k];
+                cones[5 * (size_t)i + 3] = ha;
+                cones[5 * (size_t)i + 4] = r;
             }
+        };
+        put(0, &rb[0], &rb[4], &rc[0], rc[3], rc[4]);
+        for (uint32_t r = 0; r + 1 < N; ++r) {
+            const float* t = &tn[16 * (size_t)r];
+            const float* c = &tc[16 * (size_t)r];
+            uint32_t ref[2];
+            std::memcpy(ref, t + 12, sizeof(ref));
+            const float llo[3] = {t[0], t[1], t[2]}, lhi[3] = {t[3], t[4], t[5]};
+            const float rlo[3] = {t[6], t[7], t[8]}, rhi[3] = {t[9], t[10], t[11]};
+            const uint32_t node[2] = {(ref[0] & LEAF_BIT) ? (N - 1) + (ref[0] & ~LEAF_BIT) : ref[0],
+                                      (ref[1] & LEAF_BIT) ? (N - 1) + (ref[1] & ~LEAF_BIT) : ref[1]};
+            if (node[0] < NN)
+                put(node[0], llo, lhi, c + 0, c[3], c[8]);
+            if (node[1] < NN)
+                put(node[1], rlo, rhi, c + 4, c[7], c[9]);
         }
     }
     return SNCH_OK;
