@@ -349,6 +349,30 @@ def workload_mlp(args):
     print(json.dumps(line))
 
 
+# Algorithmic bytes of each build phase per primitive (reference payload widths,
+# no padding): leaf generation 188 + scene box 24 + Morton 32; the stable sort
+# of 30-bit codes in four 8-bit passes, 4 + 16 per pass; Karras 28; the gather
+# of leaf boxes and cones into sorted order 88 plus the AABB + cone refit 164.
+BUILD_PHASE_BYTES = {"leaves": 244, "sort": 68, "karras": 28, "refit": 252}
+
+
+def build_phase_roofline(sc, n_prims, reps):
+    """Median GPU time of each build phase over `reps` builds timed with
+    events between the phases, against the HBM peak."""
+    ph = []
+    for _ in range(max(reps, 1) + 1):
+        sc.build_bvh(time_phases=True)
+        ph.append(sc.build_phases())
+    out = {}
+    for k, per in BUILD_PHASE_BYTES.items():
+        ms = float(np.median([p[k + "_ms"] for p in ph[1:]]))
+        gbs = per * n_prims / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"ms": ms, "bytes_per_prim": per, "achieved_GBps": gbs,
+                  "frac": gbs / HBM_PEAK_GBS if gbs else None}
+    sc.build_bvh()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -377,6 +401,7 @@ def main():
         sc.build_bvh()
         build_ms.append(sc.build_ms())
     build_ms = float(np.median(build_ms[1:]))
+    build_roofline = build_phase_roofline(sc, F.shape[0], args.build_reps)
 
     q = torch.from_numpy(q_np).to(dev)
     n = q.shape[0]
@@ -559,6 +584,7 @@ def main():
                    "triangles": int(F.shape[0]), "queries": int(n)},
         "builds_per_s": 1000.0 / build_ms,
         "build_ms": build_ms,
+        "build_roofline": build_roofline,
         "scene_create_s": t_scene,
         "learned_bound": learned,
         **extra,

@@ -44,6 +44,8 @@ typedef struct snch_scene snch_scene;
 #define SNCH_BUILD_REFERENCE_RADIUS 1u /* use the reference's cone radius (midpoints, sqrt merge);
                                           default is the enclosing radius, which never prunes a
                                           true nearest silhouette */
+#define SNCH_BUILD_TIME_PHASES 2u      /* record GPU events between the build's phases (read them
+                                          with snch_scene_build_phases); the tree is unchanged */
 
 typedef struct snch_build_opts {
     uint32_t flags;     /* SNCH_BUILD_* */
@@ -117,6 +119,12 @@ int snch_scene_info(const snch_scene *scene, uint32_t *dim, uint32_t *n_prims, u
 int snch_scene_build(snch_scene *scene, const snch_build_opts *opts, void *stream);
 /* Milliseconds of GPU time of the last build, measured with HIP events. */
 int snch_scene_build_time(const snch_scene *scene, float *ms);
+/* GPU milliseconds of the last build's phases, when it was made with
+ * SNCH_BUILD_TIME_PHASES (zeros otherwise): ms[0] leaf boxes, cones and silhouette
+ * records, scene box and Morton codes; ms[1] radix sort; ms[2] Karras emission;
+ * ms[3] refit (with the gather of the leaves into sorted order).  No counterpart
+ * in the reference, which never times its build (benchmark.cu:374-381). */
+int snch_scene_build_phases(const snch_scene *scene, float ms[4]);
 
 /* ---- queries (device pointers; asynchronous on `stream`) ---------------- */
 /* query_device(bvh, nearest(p), distance_calculator) (query.cuh:238-318).

@@ -25,6 +25,7 @@ _lib = None
 SNCH_OK = 0
 ERRORS = {1: "invalid argument", 2: "empty mesh", 3: "HIP error", 4: "not built", 5: "stack overflow"}
 BUILD_REFERENCE_RADIUS = 1
+BUILD_TIME_PHASES = 2
 
 
 class SnchError(RuntimeError):
@@ -66,7 +67,8 @@ class TuneResult(ctypes.Structure):
 EXPORTED = [
     "snch_last_error", "snch_version", "snch_context_create", "snch_context_destroy", "snch_scene_create_tri",
     "snch_scene_create_tri_batch", "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_mesh_count",
-    "snch_scene_info", "snch_scene_build", "snch_scene_build_time", "snch_batch_closest_point",
+    "snch_scene_info", "snch_scene_build", "snch_scene_build_time", "snch_scene_build_phases",
+    "snch_batch_closest_point",
     "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh",
     "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
     "snch_closest_silhouette_ex", "snch_ray", "snch_ray_ex",
@@ -95,6 +97,7 @@ def _declare(L):
     L.snch_scene_info.argtypes = [vp, vp, vp, vp]
     L.snch_scene_build.argtypes = [vp, vp, vp]
     L.snch_scene_build_time.argtypes = [vp, vp]
+    L.snch_scene_build_phases.argtypes = [vp, vp]
     L.snch_query_order.argtypes = [vp, vp, u32, vp, vp]
     L.snch_closest_point.argtypes = [vp, vp, u32, vp, vp, vp]
     L.snch_closest_silhouette.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp]
@@ -208,9 +211,11 @@ class Scene:
     def compute_silhouettes(self):
         return self
 
-    def build_bvh(self, reference_radius=False, leaf_size=1):
-        """scene<3>::build_bvh (scene.cuh:1205-1229) -> bvh::construct on the GPU."""
-        opts = BuildOpts(BUILD_REFERENCE_RADIUS if reference_radius else 0, leaf_size)
+    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False):
+        """scene<3>::build_bvh (scene.cuh:1205-1229) -> bvh::construct on the GPU.
+        time_phases: also record the GPU time of each build phase (build_phases)."""
+        flags = (BUILD_REFERENCE_RADIUS if reference_radius else 0) | (BUILD_TIME_PHASES if time_phases else 0)
+        opts = BuildOpts(flags, leaf_size)
         check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
         self.built = True
         return self
@@ -219,6 +224,14 @@ class Scene:
         ms = ctypes.c_float()
         check(lib().snch_scene_build_time(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def build_phases(self):
+        """GPU milliseconds of the last build's phases (zeros unless it was
+        built with time_phases=True): leaf generation + Morton codes, radix
+        sort, Karras emission, refit."""
+        ms = (ctypes.c_float * 4)()
+        check(lib().snch_scene_build_phases(self._h, ms))
+        return {"leaves_ms": ms[0], "sort_ms": ms[1], "karras_ms": ms[2], "refit_ms": ms[3]}
 
     def query_order(self, q):
         """Morton visiting order of the query points (int32 device tensor)."""

@@ -73,9 +73,14 @@ __device__ __forceinline__ uint32_t mesh_search(const MeshDesc* __restrict__ d, 
 }
 
 // scene-box accumulators (+inf, -inf) of every mesh, 8 ints per mesh
-__global__ void k_init_scene_box(int* bits, uint32_t M)
+// (and clears the two list counters k_morton and k_karras append to)
+__global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32_t* list_b)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        *list_a = 0u;
+        *list_b = 0u;
+    }
     if (t >= 8 * M)
         return;
     const uint32_t k = t & 7u;
@@ -352,11 +357,28 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
 }
 
 // bvh.cuh:269-277 and morton_code.cuh:85-92
+// thread m < M: a mesh of one primitive has its leaf as root, which is its
+// own refit subtree ("cut"); k_init_scene_box cleared the cut counter
+__device__ __forceinline__ void mesh_of_one_root(const MeshDesc* __restrict__ desc, uint32_t M,
+                                                 uint32_t* __restrict__ parent, uint32_t* __restrict__ cuts,
+                                                 uint32_t cut_cap, uint32_t m)
+{
+    if (m >= M || desc[m].n != 1)
+        return;
+    parent[desc[m].nb] = INVALID;
+    const uint32_t k = atomicAdd(cuts + cut_cap, 1u);
+    if (k < cut_cap)
+        cuts[k] = desc[m].nb;
+}
+
 __global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
                                                  uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
-                                                 const uint32_t* __restrict__ pmesh)
+                                                 const uint32_t* __restrict__ pmesh, const MeshDesc* __restrict__ desc,
+                                                 uint32_t M, uint32_t* __restrict__ parent, uint32_t* __restrict__ cuts,
+                                                 uint32_t cut_cap)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    mesh_of_one_root(desc, M, parent, cuts, cut_cap, i);
     if (i >= n)
         return;
     const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
@@ -375,9 +397,12 @@ __global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leaf
 // bvh.cuh:292-302 and morton_code.cuh:61-70
 __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
                                                uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
-                                               const uint32_t* __restrict__ pmesh)
+                                               const uint32_t* __restrict__ pmesh, const MeshDesc* __restrict__ desc,
+                                               uint32_t M, uint32_t* __restrict__ parent, uint32_t* __restrict__ cuts,
+                                               uint32_t cut_cap)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    mesh_of_one_root(desc, M, parent, cuts, cut_cap, i);
     if (i >= n)
         return;
     const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
@@ -412,12 +437,6 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const 
 constexpr uint32_t RT = 1024; // leaves of one refit subtree ("cut")
 constexpr uint32_t RB = 512;  // threads of one k_refit workgroup (RT / RB leaves each)
 
-__device__ __forceinline__ void append_cut(uint32_t* cuts, uint32_t cap, uint32_t node)
-{
-    const uint32_t k = atomicAdd(cuts + cap, 1u);
-    if (k < cap)
-        cuts[k] = node;
-}
 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
 // for one internal node: children, leaf range and parent links; the cuts it
@@ -542,16 +561,6 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
 
 
 
-// meshes of one primitive: the leaf is the root and its own cut
-__global__ void k_mesh_roots(const MeshDesc* __restrict__ desc, uint32_t M, uint32_t* __restrict__ parent,
-                             uint32_t* __restrict__ cuts, uint32_t cut_cap)
-{
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M || desc[m].n != 1)
-        return;
-    parent[desc[m].nb] = INVALID;
-    append_cut(cuts, cut_cap, desc[m].nb);
-}
 
 __device__ __forceinline__ box3 ldbox(const float4* nb, uint32_t n)
 {
@@ -1035,11 +1044,6 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
     }
 }
 
-__global__ void k_zero_words(uint32_t* a, uint32_t* b)
-{
-    *a = 0u;
-    *b = 0u;
-}
 
 inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
 
@@ -1056,8 +1060,12 @@ inline uint32_t blocks(uint32_t n) { return (n + BT - 1) / BT; }
 
 } // namespace
 
-hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st)
+hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st, hipEvent_t* phase)
 {
+    auto mark = [&](int k) {
+        if (phase)
+            (void)hipEventRecord(phase[k], st);
+    };
     const uint32_t N = s->nF, M = s->n_meshes, NN = 2 * N - M, NI = N - M;
     const uint32_t E = s->dim == 3 ? s->nE : s->n_internal_sil;
     const int conservative = (flags & SNCH_BUILD_REFERENCE_RADIUS) ? 0 : 1;
@@ -1094,20 +1102,25 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(s->nbox.ensure(sizeof(float4) * 2 * NN));
     SNCH_TRY(s->ncone.ensure(sizeof(float4) * 2 * NN));
 
-    hipLaunchKernelGGL(k_init_scene_box, dim3((8 * M + 255) / 256), dim3(256), 0, st, w.scene_bits.as<int>(), M);
+    mark(0);
+    hipLaunchKernelGGL(k_init_scene_box, dim3((8 * M + 255) / 256), dim3(256), 0, st, w.scene_bits.as<int>(), M,
+                       cuts + cut_cap, tops + top_cap);
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc);
         hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
-                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh);
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh, desc, M,
+                           w.parent.as<uint32_t>(), cuts, cut_cap);
     } else {
         hipLaunchKernelGGL(k_leafgen2d, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh);
         hipLaunchKernelGGL(k_morton2d, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
-                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh);
+                           w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh, desc, M,
+                           w.parent.as<uint32_t>(), cuts, cut_cap);
     }
+    mark(1);
     // stable sort by (mesh, code, index): code passes, then mesh-id passes
     int mesh_bits = 0;
     while (mesh_bits < 32 && ((M - 1) >> mesh_bits) != 0)
@@ -1117,19 +1130,18 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                               w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt, pmesh, pmesh ? mesh_bits : 0));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(1), 0, st, cuts + cut_cap, tops + top_cap);
+    mark(2);
     if (NI > 0)
         hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, desc, M,
                            w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(),
                            cuts, cut_cap, tops, top_cap);
-    hipLaunchKernelGGL(k_mesh_roots, dim3((M + 255) / 256), dim3(256), 0, st, desc, M, w.parent.as<uint32_t>(), cuts,
-                       cut_cap);
     // about 2N / RT cuts (each holds at least RT / 2 leaves but for one per
     // side of a large node, plus one per small mesh); workgroups loop over them
     const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
     bool any_big = false;
     for (uint32_t m = 0; m < M && !any_big; ++m)
         any_big = s->mesh_off[m + 1] - s->mesh_off[m] > RT;
+    mark(3);
     SNCH_DISPATCH_REFIT(s->dim, {
         // as many workgroups as are resident at once (LDS-bound), or one per cut
         static int per_cu = 0;
@@ -1150,6 +1162,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                                conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
                                s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
     });
+    mark(4);
     s->sorted_in_alt = in_alt;
 #undef SNCH_TRY
     return hipGetLastError();

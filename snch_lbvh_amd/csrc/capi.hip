@@ -454,17 +454,27 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
     SNCH_CHECK_ARG(leaf <= MAX_LEAF_SIZE, "snch_scene_build: leaf_size must be 1..8");
     SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, ph[5];
+    const bool phases = (flags & SNCH_BUILD_TIME_PHASES) != 0;
     SNCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     SNCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    for (int k = 0; phases && k < 5; ++k)
+        SNCH_HIP(hipEventCreate(&ph[k]), "hipEventCreate");
     (void)hipEventRecord(e0, st);
-    hipError_t e = build_tree(s, flags, leaf, st);
+    hipError_t e = build_tree(s, flags, leaf, st, phases ? ph : nullptr);
     (void)hipEventRecord(e1, st);
     hipError_t es = hipStreamSynchronize(st);
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    for (int k = 0; k < 4; ++k) {
+        s->last_phase_ms[k] = 0.0f;
+        if (phases && e == hipSuccess && es == hipSuccess)
+            (void)hipEventElapsedTime(&s->last_phase_ms[k], ph[k], ph[k + 1]);
+    }
+    for (int k = 0; phases && k < 5; ++k)
+        (void)hipEventDestroy(ph[k]);
     if (e != hipSuccess)
         return hip_fail(e, "snch_scene_build");
     if (es != hipSuccess)
@@ -489,6 +499,14 @@ int snch_scene_build_time(const snch_scene* s, float* ms)
 {
     SNCH_CHECK_ARG(s && ms, "snch_scene_build_time: null argument");
     *ms = s->last_build_ms;
+    return SNCH_OK;
+}
+
+int snch_scene_build_phases(const snch_scene* s, float ms[4])
+{
+    SNCH_CHECK_ARG(s && ms, "snch_scene_build_phases: null argument");
+    for (int k = 0; k < 4; ++k)
+        ms[k] = s->last_phase_ms[k];
     return SNCH_OK;
 }
 

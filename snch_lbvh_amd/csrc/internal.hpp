@@ -143,6 +143,7 @@ struct snch_scene {
     bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
     uint32_t build_flags = 0;
     float last_build_ms = 0.0f;
+    float last_phase_ms[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     snch::TreeView view(uint32_t mesh = 0) const;
 };
 
@@ -151,7 +152,10 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
 // build.hip
-hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st);
+// phase (nullable): five events recorded at the phase boundaries (start,
+// after Morton codes, after the sort, after Karras, after the refit)
+hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st,
+                      hipEvent_t* phase = nullptr);
 // sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
 // of the key, then (seg != null) on the low `seg_bits` bits of seg[value]
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,

@@ -108,6 +108,22 @@ def test_build_bit_exact_top_climb(S, oracle):
     del g, o
 
 
+def test_build_phase_timing(S):
+    """Timing the build's phases records four positive GPU times that add up
+    to at most the whole build, and leaves the tree unchanged."""
+    V, F = W.cut_torus(256, 128)
+    sc = S.Scene(V, F).build_bvh()
+    a = sc.export()
+    assert all(v == 0.0 for v in sc.build_phases().values())
+    sc.build_bvh(time_phases=True)
+    ph = sc.build_phases()
+    assert all(v > 0.0 for v in ph.values()), ph
+    assert sum(ph.values()) <= sc.build_ms() * 1.05 + 0.01
+    b = sc.export()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
 def test_build_deterministic(S):
     V, F = W.cut_torus(128, 64)
     a = S.Scene(V, F).build_bvh().export()
