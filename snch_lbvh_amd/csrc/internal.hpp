@@ -144,6 +144,11 @@ struct snch_scene {
     uint32_t build_flags = 0;
     float last_build_ms = 0.0f;
     float last_phase_ms[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // repeat builds replay the launch sequence captured on the second build
+    // with the same (flags, leaf size); every buffer it names is sized by then
+    hipGraphExec_t build_exec = nullptr;
+    uint32_t build_key = 0, eager_key = 0, eager_builds = 0;
+    ~snch_scene();
     snch::TreeView view(uint32_t mesh = 0) const;
 };
 

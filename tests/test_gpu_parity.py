@@ -124,6 +124,29 @@ def test_build_phase_timing(S):
         np.testing.assert_array_equal(a[k], b[k])
 
 
+def test_rebuild_replays_identical(S, oracle):
+    """Repeat builds of one scene replay a captured launch sequence (from the
+    second build with the same options on); every rebuild, also after a
+    change of leaf size and back, must give the oracle's tree, and queries
+    after a replayed build must match queries after the first build."""
+    V, F = W.cut_torus(192, 96)
+    o = oracle.build_tree(V, F)
+    sc = S.Scene(V, F)
+    q = _queries(V, 20000, 5)
+    ref = None
+    for leaf in (1, 1, 1, 4, 4, 4, 1, 1):
+        sc.build_bvh(leaf_size=leaf)
+        if leaf == 1:
+            g = sc.export()
+            for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
+                np.testing.assert_array_equal(g[k], ov, err_msg=k)
+        res = [t.cpu().numpy() for t in sc.closest_silhouette(q)]
+        if ref is None:
+            ref = res
+        for x, y in zip(res, ref):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_build_deterministic(S):
     V, F = W.cut_torus(128, 64)
     a = S.Scene(V, F).build_bvh().export()
