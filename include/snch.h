@@ -267,6 +267,10 @@ int snch_device_malloc(void **ptr, uint64_t bytes);
 int snch_device_free(void *ptr);
 int snch_memcpy_h2d(void *dst, const void *src, uint64_t bytes, void *stream);
 int snch_memcpy_d2h(void *dst, const void *src, uint64_t bytes, void *stream);
+/* Waits for `stream`, then reports (and clears) the device's sticky error
+ * words: SNCH_ERR_OVERFLOW if a traversal lost a stack entry, SNCH_ERR_HIP if a
+ * radix-sort pass exceeded its wait bound (neither is expected; snch_scene_build
+ * checks them too). */
 int snch_stream_synchronize(void *stream);
 
 #ifdef __cplusplus

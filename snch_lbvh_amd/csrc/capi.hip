@@ -45,8 +45,29 @@ hipError_t DeviceBuffer::ensure(size_t n)
     return e;
 }
 
-// one error word per device for traversal stack overflow (never expected:
-// the tree depth is bounded by the 64 key bits)
+// word 0: a traversal lost a stack entry; word 1: a radix-sort pass gave up
+// waiting for an earlier tile (both sticky until read here)
+int check_device_errors(const char* fn)
+{
+    uint32_t* w = traversal_error_word();
+    if (!w)
+        return SNCH_OK;
+    uint32_t h[2] = {0u, 0u};
+    if (hipMemcpy(h, w, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        return hip_fail(hipGetLastError(), fn);
+    if (h[0] == 0u && h[1] == 0u)
+        return SNCH_OK;
+    (void)hipMemset(w, 0, sizeof(h));
+    if (h[1]) {
+        set_error(std::string(fn) + ": a radix-sort pass exceeded its wait bound; results since the last check are invalid");
+        return SNCH_ERR_HIP;
+    }
+    set_error(std::string(fn) + ": a traversal exceeded its stack; results since the last check may be incomplete");
+    return SNCH_ERR_OVERFLOW;
+}
+
+// sticky error words per device (never expected: the tree depth is bounded
+// by the 64 key bits, and every tile a sort pass waits on has started)
 uint32_t* traversal_error_word()
 {
     static std::mutex mu;
@@ -239,6 +260,12 @@ int snch_context_create(int device, snch_context** out)
     SNCH_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
     SNCH_CHECK_ARG(device >= 0 && device < count, "snch_context_create: no such device");
     SNCH_HIP(hipSetDevice(device), "hipSetDevice");
+    // the device's sticky error words, allocated here so that no later call
+    // (possibly under graph capture) allocates them
+    if (!traversal_error_word()) {
+        set_error("snch_context_create: cannot allocate the device error words");
+        return SNCH_ERR_HIP;
+    }
     auto* c = new snch_context();
     c->device = device;
     hipDeviceProp_t prop;
@@ -536,6 +563,8 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
         return hip_fail(e, "snch_scene_build");
     if (es != hipSuccess)
         return hip_fail(es, "snch_scene_build: synchronize");
+    if (const int rc = check_device_errors("snch_scene_build"))
+        return rc;
     s->build_flags = flags;
     {
         std::vector<TreeView> views(s->n_meshes);
@@ -1031,7 +1060,7 @@ int snch_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream)
 int snch_stream_synchronize(void* stream)
 {
     SNCH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
-    return SNCH_OK;
+    return check_device_errors("snch_stream_synchronize");
 }
 
 } // extern "C"

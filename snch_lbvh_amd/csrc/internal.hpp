@@ -193,5 +193,7 @@ hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, u
 // batched closest point (kind 0) / closest silhouette (kind 1) over all meshes
 hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
                         uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st);
-uint32_t* traversal_error_word();
+uint32_t* traversal_error_word(); // [0] traversal overflow, [1] sort wait bound (capi.hip)
+// SNCH_OK, or the error code and message of a sticky device error (cleared)
+int check_device_errors(const char* fn);
 } // namespace snch

@@ -40,14 +40,12 @@ constexpr uint32_t LB_AGG = 1u << 30, LB_INC = 2u << 30, LB_COUNT = (1u << 30) -
 constexpr uint32_t LB_SPIN = 1u << 22; // bound on polls of one look-back word
 constexpr uint32_t LB_WIN = 8;         // look-back words read at once
 
-__global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__ tile_ctr, uint32_t* __restrict__ err)
+__global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__ tile_ctr)
 {
     for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += blockDim.x)
         ghist[i] = 0u;
     if (threadIdx.x < MAX_PASSES)
         tile_ctr[threadIdx.x] = 0u;
-    if (threadIdx.x == 0)
-        *err = 0u;
 }
 
 // digit counts of every pass (pass p < key_passes: key byte p; then byte
@@ -253,9 +251,9 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
         return e;
     uint32_t* ghist = hist.as<uint32_t>();
     uint32_t* tile_ctr = ghist + MAX_PASSES * RADIX;
-    uint32_t* err = tile_ctr + MAX_PASSES;
-    uint32_t* status = err + MAX_PASSES;
-    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr, err);
+    uint32_t* status = tile_ctr + 2 * MAX_PASSES;
+    uint32_t* err = traversal_error_word() + 1; // sticky; read by check_device_errors
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
     hipLaunchKernelGGL(k_hist_all, dim3(ntiles), dim3(ST), 0, st, keys, vals, seg, n, key_passes, seg_passes, ntiles,
                        ghist, status);
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;

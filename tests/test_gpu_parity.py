@@ -147,6 +147,17 @@ def test_rebuild_replays_identical(S, oracle):
             np.testing.assert_array_equal(x, y)
 
 
+def test_no_device_errors(S):
+    """After builds, a large query batch and a sort, the device's sticky error
+    words (traversal overflow, sort wait bound) are clear."""
+    V, F = W.cut_torus(512, 256)
+    sc = S.Scene(V, F).build_bvh()
+    q = _queries(V, 200000, 3)
+    sc.closest_silhouette(q)
+    sc.closest_point(q)
+    S.check(S.lib().snch_stream_synchronize(S._stream()))
+
+
 def test_build_deterministic(S):
     V, F = W.cut_torus(128, 64)
     a = S.Scene(V, F).build_bvh().export()
