@@ -87,112 +87,51 @@ uint32_t* traversal_error_word()
     return words[dev];
 }
 
-// Edge table of scene<3> (scene.cuh:1135-1225): edge id = first appearance
-// over (triangle i, local edge j = (v_j, v_{j+1})); slots (opp0, lo, hi,
-// opp1); each edge owned by the first triangle listing it.  Computed by
-// sorting (lo, hi, position) instead of a std::map; the owned edges of a
-// triangle are then a run of consecutive ids starting at estart[i].
-struct EdgeTable {
-    std::vector<int32_t> slots;   // 4 per edge
-    std::vector<int32_t> owned;   // 3 per triangle
-    std::vector<uint32_t> estart; // nF + 1
-    uint32_t nE = 0;
-};
-
-static void edge_table_3d(const int32_t* F, uint32_t nF, EdgeTable* s)
+// device inputs of a scene: vertices as float4, primitives with scene-global
+// vertex indices; then (2D, from the host) the silhouette slots and the first
+// owned silhouette record of each primitive, or (3D) those computed on the
+// device; then the mesh table
+static hipError_t upload_geometry(snch_scene* s, int dim, const float* Vflat, size_t nV,
+                                  const std::vector<int32_t>& Fg)
 {
-    const size_t n = 3 * (size_t)nF;
-    std::vector<uint64_t> key(n);
-    std::vector<uint32_t> order(n);
-    for (size_t p = 0; p < n; ++p) {
-        const size_t i = p / 3, j = p % 3;
-        uint32_t I = (uint32_t)F[3 * i + j], J = (uint32_t)F[3 * i + (j + 1) % 3];
-        if (I > J)
-            std::swap(I, J);
-        key[p] = ((uint64_t)I << 32) | J;
-    }
-    std::iota(order.begin(), order.end(), 0u);
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return key[a] != key[b] ? key[a] < key[b] : a < b;
-    });
-    // group -> first position
-    std::vector<uint32_t> group_of(n), first_pos;
-    first_pos.reserve(n / 2 + 1);
-    for (size_t k = 0; k < n; ++k) {
-        if (k == 0 || key[order[k]] != key[order[k - 1]])
-            first_pos.push_back(order[k]);
-        group_of[order[k]] = (uint32_t)(first_pos.size() - 1);
-    }
-    const uint32_t E = (uint32_t)first_pos.size();
-    // ids in order of first appearance: a position is the first of its group
-    // iff first_pos[group] == position; scanning positions in order numbers them
-    std::vector<uint32_t> gid(E, 0xFFFFFFFFu);
-    std::vector<int32_t> tri_edge(n);
-    uint32_t next = 0;
-    s->owned.assign(3 * (size_t)nF, -1);
-    s->estart.assign((size_t)nF + 1, 0);
-    for (size_t p = 0; p < n; ++p) {
-        const uint32_t g = group_of[p];
-        if (first_pos[g] == p) {
-            gid[g] = next++;
-            const size_t i = p / 3;
-            int k = 0;
-            while (s->owned[3 * i + k] != -1)
-                ++k;
-            s->owned[3 * i + k] = (int32_t)gid[g];
-        }
-        tri_edge[p] = (int32_t)gid[g];
-        if (p % 3 == 2)
-            s->estart[p / 3 + 1] = next;
-    }
-    s->nE = E;
-    s->slots.assign(4 * (size_t)E, -1);
-    for (uint32_t i = 0; i < nF; ++i) {
-        for (int j = 0; j < 3; ++j) {
-            int I = j - 1 < 0 ? 2 : j - 1, J = j, K = j + 1 > 2 ? 0 : j + 1, orientation = 1;
-            const int32_t e = tri_edge[3 * (size_t)i + j];
-            if (F[3 * i + J] > F[3 * i + K]) {
-                std::swap(J, K);
-                orientation = -1;
-            }
-            int32_t* sl = &s->slots[4 * (size_t)e];
-            sl[orientation == 1 ? 0 : 3] = F[3 * i + I];
-            sl[1] = F[3 * i + J];
-            sl[2] = F[3 * i + K];
-        }
-    }
-}
-
-// device inputs of a scene: vertices as float4, primitives / silhouette
-// slots with scene-global vertex indices, first owned silhouette per
-// primitive (global), the mesh table and, for batches, the mesh of each
-// primitive
-static hipError_t upload_scene(snch_scene* s, int dim, const std::vector<float>& Vflat,
-                               const std::vector<int32_t>& Fg, const std::vector<int32_t>& slots_g,
-                               const std::vector<uint32_t>& estart_g)
-{
-    const size_t nV = Vflat.size() / dim;
     std::vector<float> V4(4 * nV, 0.0f);
     for (size_t k = 0; k < nV; ++k)
         for (int c = 0; c < dim; ++c)
             V4[4 * k + c] = Vflat[dim * k + c];
+    hipError_t e;
+    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = s->dF.ensure(sizeof(int32_t) * Fg.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dF.p, Fg.data(), sizeof(int32_t) * Fg.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+// the mesh table (and, for batches, the mesh of each primitive); needs the
+// per-mesh silhouette offsets
+static hipError_t upload_tables(snch_scene* s, const std::vector<int32_t>& slots_g,
+                                const std::vector<uint32_t>& estart_g)
+{
+    hipError_t e;
+    if ((e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(slots_g.size(), 4))) != hipSuccess ||
+        (slots_g.size() && (e = hipMemcpy(s->dSlots.p, slots_g.data(), sizeof(int32_t) * slots_g.size(),
+                                          hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = s->dEstart.ensure(sizeof(uint32_t) * estart_g.size())) != hipSuccess ||
+        (e = hipMemcpy(s->dEstart.p, estart_g.data(), sizeof(uint32_t) * estart_g.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+static hipError_t upload_meshes(snch_scene* s)
+{
     const uint32_t M = s->n_meshes;
     std::vector<MeshDesc> desc(M);
     for (uint32_t m = 0; m < M; ++m)
         desc[m] = MeshDesc{s->mesh_off[m], s->mesh_off[m + 1] - s->mesh_off[m], s->mesh_eoff[m],
                            2 * s->mesh_off[m] - m};
     hipError_t e;
-    if ((e = s->dV.ensure(sizeof(float) * V4.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dV.p, V4.data(), sizeof(float) * V4.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dF.ensure(sizeof(int32_t) * Fg.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dF.p, Fg.data(), sizeof(int32_t) * Fg.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = s->dSlots.ensure(sizeof(int32_t) * std::max<size_t>(slots_g.size(), 4))) != hipSuccess ||
-        (slots_g.size() && (e = hipMemcpy(s->dSlots.p, slots_g.data(), sizeof(int32_t) * slots_g.size(),
-                                          hipMemcpyHostToDevice)) != hipSuccess) ||
-        (e = s->dEstart.ensure(sizeof(uint32_t) * estart_g.size())) != hipSuccess ||
-        (e = hipMemcpy(s->dEstart.p, estart_g.data(), sizeof(uint32_t) * estart_g.size(), hipMemcpyHostToDevice)) !=
-            hipSuccess ||
-        (e = s->dMesh.ensure(sizeof(MeshDesc) * M)) != hipSuccess ||
+    if ((e = s->dMesh.ensure(sizeof(MeshDesc) * M)) != hipSuccess ||
         (e = hipMemcpy(s->dMesh.p, desc.data(), sizeof(MeshDesc) * M, hipMemcpyHostToDevice)) != hipSuccess)
         return e;
     if (M > 1) {
@@ -347,27 +286,16 @@ int snch_scene_create_tri_batch(snch_context* ctx, uint32_t n_meshes, const floa
         s->mesh_diag2[m] = (float)d2;
     }
     s->F.resize(3 * (size_t)nF);
-    s->estart.assign(1, 0);
-    std::vector<int32_t> slots_g;
-    for (uint32_t m = 0; m < n_meshes; ++m) {
-        const int32_t* Fm = F + 3 * (size_t)f_off[m];
-        const uint32_t nf = f_off[m + 1] - f_off[m], vo = v_off[m], eo = s->mesh_eoff[m];
-        EdgeTable t;
-        edge_table_3d(Fm, nf, &t);
-        for (size_t k = 0; k < 3 * (size_t)nf; ++k)
-            s->F[3 * (size_t)f_off[m] + k] = Fm[k] + (int32_t)vo;
-        s->slots.insert(s->slots.end(), t.slots.begin(), t.slots.end());
-        s->owned.insert(s->owned.end(), t.owned.begin(), t.owned.end());
-        for (int32_t x : t.slots)
-            slots_g.push_back(x < 0 ? x : x + (int32_t)vo);
-        for (uint32_t i = 1; i <= nf; ++i)
-            s->estart.push_back(eo + t.estart[i]);
-        s->mesh_toff.push_back(s->mesh_toff[m] + t.nE);
-        s->mesh_eoff.push_back(eo + t.nE); // in 3D every edge has exactly one owner
-    }
-    s->nE = s->mesh_eoff[n_meshes];
-    std::vector<float> Vv(V, V + 3 * (size_t)nV);
-    const hipError_t e = upload_scene(s, 3, Vv, s->F, slots_g, s->estart);
+    for (uint32_t m = 0; m < n_meshes; ++m)
+        for (size_t k = 3 * (size_t)f_off[m]; k < 3 * (size_t)f_off[m + 1]; ++k)
+            s->F[k] = F[k] + (int32_t)v_off[m];
+    // edge table on the device (edges.hip): slots, first owned edge of each
+    // triangle, per-mesh edge ranges; the host copies are made on demand
+    hipError_t e = upload_geometry(s, 3, V, nV, s->F);
+    if (e == hipSuccess)
+        e = edge_table_device(s, nullptr);
+    if (e == hipSuccess)
+        e = upload_meshes(s);
     if (e != hipSuccess) {
         delete s;
         return hip_fail(e, "snch_scene_create_tri_batch: upload");
@@ -442,7 +370,11 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
         s->mesh_diag2 = {(float)((double)(hi[0] - lo[0]) * (hi[0] - lo[0]) + (double)(hi[1] - lo[1]) * (hi[1] - lo[1]))};
     }
     std::vector<float> Vv(V, V + 2 * (size_t)nV);
-    const hipError_t e = upload_scene(s, 2, Vv, s->F, internal_slots, s->estart);
+    hipError_t e = upload_geometry(s, 2, Vv.data(), nV, s->F);
+    if (e == hipSuccess)
+        e = upload_tables(s, internal_slots, s->estart);
+    if (e == hipSuccess)
+        e = upload_meshes(s);
     if (e != hipSuccess) {
         delete s;
         return hip_fail(e, "snch_scene_create_seg: upload");
@@ -1024,10 +956,33 @@ int snch_scene_mesh_count(const snch_scene* s, uint32_t* n_meshes)
 int snch_scene_silhouettes(const snch_scene* s, int32_t* slots, int32_t* owned)
 {
     SNCH_CHECK_ARG(s, "snch_scene_silhouettes: null scene");
-    if (slots && !s->slots.empty())
-        std::memcpy(slots, s->slots.data(), sizeof(int32_t) * s->slots.size());
-    if (owned)
-        std::memcpy(owned, s->owned.data(), sizeof(int32_t) * s->owned.size());
+    if (s->dim == 2) { // host tables
+        if (slots && !s->slots.empty())
+            std::memcpy(slots, s->slots.data(), sizeof(int32_t) * s->slots.size());
+        if (owned)
+            std::memcpy(owned, s->owned.data(), sizeof(int32_t) * s->owned.size());
+        return SNCH_OK;
+    }
+    // 3D: the device tables, turned back into mesh-local vertex and edge ids
+    SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    if (slots && s->nE > 0) {
+        SNCH_HIP(hipMemcpy(slots, s->dSlots.p, sizeof(int32_t) * 4 * (size_t)s->nE, hipMemcpyDeviceToHost),
+                 "snch_scene_silhouettes");
+        for (uint32_t m = 0; m < s->n_meshes; ++m)
+            for (size_t k = 4 * (size_t)s->mesh_eoff[m]; k < 4 * (size_t)s->mesh_eoff[m + 1]; ++k)
+                if (slots[k] >= 0)
+                    slots[k] -= (int32_t)s->mesh_voff[m];
+    }
+    if (owned) {
+        std::vector<uint32_t> es((size_t)s->nF + 1);
+        SNCH_HIP(hipMemcpy(es.data(), s->dEstart.p, sizeof(uint32_t) * es.size(), hipMemcpyDeviceToHost),
+                 "snch_scene_silhouettes");
+        for (uint32_t m = 0; m < s->n_meshes; ++m)
+            for (uint32_t i = s->mesh_off[m]; i < s->mesh_off[m + 1]; ++i)
+                for (uint32_t k = 0; k < 3; ++k)
+                    owned[3 * (size_t)i + k] =
+                        k < es[i + 1] - es[i] ? (int32_t)(es[i] - s->mesh_eoff[m] + k) : -1;
+    }
     return SNCH_OK;
 }
 

@@ -125,9 +125,10 @@ struct snch_scene {
     uint32_t leaf_size = 1;
     uint32_t nV = 0, nF = 0, nE = 0; // nE: silhouette table size (edges in 3D, vertices in 2D)
     uint32_t n_internal_sil = 0;     // packed silhouette records (owned silhouettes)
-    // host copies: primitives (scene-global vertex indices), the silhouette
-    // table and ownership of each mesh (mesh-local ids, concatenated), and
-    // the first owned silhouette record of each primitive (scene-global)
+    // host copies: primitives (scene-global vertex indices); for segment
+    // scenes also the silhouette table and ownership (mesh-local ids) and
+    // the first owned silhouette record of each primitive (triangle scenes
+    // keep those on the device only)
     std::vector<int32_t> F;
     std::vector<int32_t> slots;   // 4 per silhouette
     std::vector<int32_t> owned;   // 3 (2 in 2D) per primitive
@@ -155,6 +156,11 @@ struct snch_scene {
 namespace snch {
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+
+// edges.hip: the silhouette-edge table of a triangle scene from s->dF
+// (scene-global vertex indices): s->dSlots, s->dEstart, s->nE and the
+// per-mesh edge offsets.  Synchronises `st`.
+hipError_t edge_table_device(snch_scene* s, hipStream_t st);
 
 // build.hip
 // phase (nullable): five events recorded at the phase boundaries (start,
