@@ -200,7 +200,16 @@ SNCH_HD bool cone_overlap(v3 axis, float half_angle, float radius, v3 o, const b
 // vha, which are ratios of lengths.  A slack of 1e-5 (l + r) on the
 // permissive side keeps every node the reference test keeps despite either
 // test's rounding; a node kept in excess only costs a visit, never changes a
-// result.  Conservative only with the enclosing cone radius.
+// result.  Conservative only with the enclosing cone radius.  Its square
+// roots and reciprocals need not be correctly rounded (AM supplies them; on
+// the GPU the hardware's 1-ulp v_sqrt_f32 / v_rcp_f32): errors of a few ulps
+// are two orders of magnitude inside the slack.
+struct HwMath {
+    __device__ static float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+};
+
+template <class AM = HwMath>
 SNCH_HD bool cone_overlap_fast(v3 axis, float half_angle, float sin_ha, float cos_ha, float radius, v3 o,
                                const box3& b, float dist_to_box)
 {
@@ -208,25 +217,32 @@ SNCH_HD bool cone_overlap_fast(v3 axis, float half_angle, float sin_ha, float co
         return true;
     const v3 c = box_centroid(b);
     const v3 v = mk(c.x - o.x, c.y - o.y, c.z - o.z);
-    const float l2 = dot3(v, v), l = dm::sqrt_(l2);
+    const float l2 = dot3(v, v), l = AM::sqrt(l2);
     const float av = dm::fabs_(dot3(axis, v)); // l |C|
     const float tol = 1e-5f * (l + radius);
     if (l > radius) {
-        const float w = dm::sqrt_(fmx(l2 - radius * radius, 0.0f)); // l cos(vha); l sin(vha) = radius
+        const float w = AM::sqrt(fmx(l2 - radius * radius, 0.0f)); // l cos(vha); l sin(vha) = radius
         if (cos_ha * w <= sin_ha * radius + tol)
             return true; // ha + vha >= pi/2
         return av <= sin_ha * w + cos_ha * radius + tol;
     }
     if (!(l > 0.0f))
         return true;
-    const float il = 1.0f / l;
+    const float il = AM::rcp(l);
     const v3 vn = mk(v.x * il, v.y * il, v.z * il);
     const v3 e = mk(b.hi.x - c.x, b.hi.y - c.y, b.hi.z - c.z);
     const float s = l - dot3(e, cabs3(vn));
     if (s <= 0.0f)
         return true;
-    const float d = project_to_plane(vn, e); // tan(vha) = d / s
-    const float h = dm::sqrt_(d * d + s * s);
+    // project_to_plane(vn, e) with the same basis (cone.cuh:24-42, :58-66)
+    const float sign = dm::signbit_(vn.z) ? -1.0f : 1.0f;
+    const float a = -AM::rcp(sign + vn.z);
+    const float bb = vn.x * vn.y * a;
+    const v3 b1 = mk(1.0f + sign * vn.x * vn.x * a, sign * bb, -sign * vn.x);
+    const v3 b2 = mk(bb, sign + vn.y * vn.y * a, -vn.y);
+    const float r1 = dot3(e, cabs3(b1)), r2 = dot3(e, cabs3(b2));
+    const float d = AM::sqrt(r1 * r1 + r2 * r2); // tan(vha) = d / s
+    const float h = AM::sqrt(d * d + s * s);
     if (cos_ha * s <= sin_ha * d + tol)
         return true;
     return av * il * h <= sin_ha * s + cos_ha * d + tol;
