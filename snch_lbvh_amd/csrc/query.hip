@@ -421,6 +421,106 @@ __device__ __forceinline__ void sil_leaf(const TreeView& t, uint32_t j, v3 p, bo
     }
 }
 
+// Per-wave LDS of the flattened leaf tests: one round of 64 edge slots (the
+// edge, then its distance with the sign bit set when it is not a silhouette),
+// the owner lane of each slot, and each owner's query point and bound.
+struct EdgeLds {
+    uint32_t edge[64];
+    float d[64];
+    uint8_t owner[64];
+    float px[64], py[64], pz[64], bound[64];
+};
+
+// The leaf tests of the lanes with `mine` set, spread over every lane of the
+// wave that runs this (all looping lanes, wave-uniformly): each owner lists
+// the edges of its (at most two) leaves as a run of slots, and in rounds of
+// 64 slots every lane computes the distance and the silhouette test of one,
+// then each owner applies its results in its original edge order, so ties and
+// the reference's early-out at distance 0 (scene.cuh:791) come out as in
+// sil_leaf.  3D, leaves of one primitive each.
+__device__ __forceinline__ void sil_leaves_wave(const TreeView& t, EdgeLds& L, bool mine, uint32_t la, bool ha,
+                                                uint32_t lb, bool hb, v3 p, bool flip, float& best,
+                                                uint32_t& best_e, Counts& cnt)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t ea = 0, eb = 0, sa = 0, sb = 0;
+    if (mine && ha && la < t.n_leaves) {
+        sa = __float_as_uint(t.tleaf[3 * la + 1].w);
+        ea = __float_as_uint(t.tleaf[3 * la + 2].w);
+    }
+    if (mine && hb && lb < t.n_leaves) {
+        sb = __float_as_uint(t.tleaf[3 * lb + 1].w);
+        eb = __float_as_uint(t.tleaf[3 * lb + 2].w);
+    }
+    ea = min(ea, 3u);
+    eb = min(eb, 3u);
+    const uint32_t my = ea + eb; // 0..6
+    const uint64_t b0 = __ballot(my & 1u), b1 = __ballot(my & 2u), b2 = __ballot(my & 4u);
+    const uint32_t off = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) + 4u * (uint32_t)__popcll(b2 & lt);
+    const uint32_t total = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+    if (mine) {
+        L.px[lane] = p.x;
+        L.py[lane] = p.y;
+        L.pz[lane] = p.z;
+        L.bound[lane] = best;
+        cnt.leaf += (ha ? 1u : 0u) + (hb ? 1u : 0u);
+        cnt.edge += my;
+    }
+    const uint64_t helpers = __ballot(true);
+    const uint32_t hr = (uint32_t)__popcll(helpers & lt), nh = (uint32_t)__popcll(helpers);
+    bool stop = false; // best reached 0: the owner's remaining edges are skipped
+    for (uint32_t base = 0; base < total; base += 64) {
+        // this round's slots [base, base + 64) of every owner
+        for (uint32_t k = 0; mine && k < my; ++k) {
+            const uint32_t sl = off + k;
+            if (sl >= base && sl < base + 64) {
+                L.edge[sl - base] = k < ea ? sa + k : sb + (k - ea);
+                L.owner[sl - base] = (uint8_t)lane;
+            }
+        }
+        // every lane of the wave (LDS operations of one wave complete in order)
+        const uint32_t n_round = min(64u, total - base);
+        for (uint32_t r = hr; r < n_round; r += nh) {
+            const uint32_t e = L.edge[r], o = L.owner[r];
+            const v3 q = mk(L.px[o], L.py[o], L.pz[o]);
+            const float b = L.bound[o], mr2 = b * b;
+            const float4 A = t.tedge[4 * e], B = t.tedge[4 * e + 1];
+            const uint32_t fl = __float_as_uint(B.w);
+            const v3 pa = mk(A.x, A.y, A.z), pb = mk(B.x, B.y, B.z);
+            v3 cp;
+            const float d = closest_point_segment(pa, pb, q, cp);
+            bool sil = (fl & 3u) != 3u; // a boundary edge is always a silhouette
+            if (!sil && d * d <= mr2) {
+                const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
+                sil = is_silhouette_edge(pa, pb, mk(N0.x, N0.y, N0.z), mk(N1.x, N1.y, N1.z),
+                                         mk(q.x - cp.x, q.y - cp.y, q.z - cp.z), d, flip);
+            }
+            L.d[r] = sil ? d : -d; // d >= 0: the sign bit carries "not a silhouette"
+        }
+        for (uint32_t k = 0; mine && !stop && k < my; ++k) {
+            const uint32_t sl = off + k;
+            if (sl < base || sl >= base + 64)
+                continue;
+            const float mr2 = best * best;
+            if (!(0.0f < mr2)) {
+                stop = true; // min_radius_squared (0) >= max_radius_squared
+                break;
+            }
+            const float sd = L.d[sl - base];
+            const float d = dm::fabs_(sd);
+            if (d * d > mr2 || dm::signbit_(sd))
+                continue;
+            const uint32_t e = k < ea ? sa + k : sb + (k - ea);
+            const uint32_t id = __float_as_uint(t.tedge[4 * e].w);
+            if (d < best || (d == best && id < best_e)) {
+                best = d;
+                best_e = id;
+            }
+        }
+    }
+}
+
 template <int DIM>
 __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p, const box3& b, float md)
 {
@@ -439,9 +539,10 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 // than the stack keeps, the resumed search restarts from the root with the
 // bound found so far.  With index tie-breaking the answer equals the
 // uncapped search.
-template <int BS, int SSZ, bool STATS, int DIM, bool HINT>
+template <int BS, int SSZ, bool STATS, int DIM, bool HINT, bool PARK>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
-                                          float& best, uint32_t& best_e, Counts& cnt, bool& overflow)
+                                          EdgeLds* elds, float& best, uint32_t& best_e, Counts& cnt,
+                                          bool& overflow)
 {
     best = F_INF;
     best_e = INVALID;
@@ -510,16 +611,23 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
             decide = !leaves;
         }
         const uint64_t parked = __ballot(leaves), looping = __ballot(true);
-        if (leaves &&
-            (!HINT || 4u * (uint32_t)__popcll(parked) >= (uint32_t)__popcll(looping) || parked == looping)) {
-            if (hL && lLeaf)
-                for (uint32_t k = 0; k < lc; ++k)
-                    sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
-            if (hR && rLeaf)
-                for (uint32_t k = 0; k < rc; ++k)
-                    sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
+        if (parked &&
+            (!PARK || 4u * (uint32_t)__popcll(parked) >= (uint32_t)__popcll(looping) || parked == looping)) {
+            // wave-uniform: the parked lanes' leaves are tested now
+            const bool la = hL && lLeaf, ra = hR && rLeaf;
+            const bool runs = __ballot(leaves && ((la && lc > 1) || (ra && rc > 1))) != 0;
+            if (DIM == 3 && elds && !runs) {
+                sil_leaves_wave(t, *elds, leaves, lf, la, rf, ra, p, flip, best, best_e, cnt);
+            } else if (leaves) {
+                if (la)
+                    for (uint32_t k = 0; k < lc; ++k)
+                        sil_leaf<DIM>(t, lf + k, p, flip, best, best_e, cnt);
+                if (ra)
+                    for (uint32_t k = 0; k < rc; ++k)
+                        sil_leaf<DIM>(t, rf + k, p, flip, best, best_e, cnt);
+            }
+            decide = decide || leaves;
             leaves = false;
-            decide = true;
         }
         if (!decide)
             continue;
@@ -596,6 +704,7 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
                                                            uint32_t* __restrict__ err, unsigned long long* stats)
 {
     __shared__ uint32_t stk[SSZ * BS];
+    __shared__ EdgeLds elds[DIM == 3 ? BS / 64 : 1];
     const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -613,7 +722,9 @@ __global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const flo
     uint32_t best_e;
     bool ovf = false;
     const v3 p = load_point<DIM>(q, qi);
-    sil_query<BS, SSZ, STATS, DIM, MODE != 2>(t, p, flip != 0, cap, stk + threadIdx.x, best, best_e, cnt, ovf);
+    sil_query<BS, SSZ, STATS, DIM, MODE != 2, MODE != 2>(t, p, flip != 0, cap, stk + threadIdx.x,
+                                                         DIM == 3 ? &elds[threadIdx.x / 64] : nullptr, best,
+                                                         best_e, cnt, ovf);
     if (ovf && err)
         atomicOr(err, 1u);
     if (out_e)
@@ -782,6 +893,7 @@ __global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __r
                                               uint32_t* __restrict__ err)
 {
     __shared__ uint32_t stk[SSZ * BS];
+    __shared__ EdgeLds elds[BS / 64];
     const uint32_t lane = threadIdx.x & 63;
     bool ovf = false;
     Counts cnt;
@@ -816,7 +928,8 @@ __global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __r
                 cp_query<BS, SSZ, false, 3>(tv, p, stk + threadIdx.x, best, bi, cnt, ovf);
                 best = dm::sqrt_(best);
             } else {
-                sil_query<BS, SSZ, false, 3, false>(tv, p, flip != 0, F_INF, stk + threadIdx.x, best, bi, cnt, ovf);
+                sil_query<BS, SSZ, false, 3, false, true>(tv, p, flip != 0, F_INF, stk + threadIdx.x,
+                                                          &elds[threadIdx.x / 64], best, bi, cnt, ovf);
             }
             if (out_idx)
                 out_idx[qi] = bi;
