@@ -4,9 +4,13 @@
 //   k_init_scene_box  scene-box accumulators to (+inf, -inf)
 //   k_leafgen         one thread per triangle: leaf AABB (vertex extent
 //                     +- FLT_EPSILON), leaf normal cone from the owned
-//                     silhouette edges, the packed silhouette-edge records,
-//                     and a wave64 + LDS reduction of the scene AABB finished
-//                     with one ordered-int atomic per block and component
+//                     silhouette edges (the edges' normals computed one
+//                     edge per lane across the wave), the packed
+//                     silhouette-edge records,
+//                     and a wave64 + LDS reduction of the scene AABB into
+//                     one partial box per workgroup
+//   k_box_final       the partial boxes into the scene AABB (ordered-int
+//                     atomics, one per workgroup and component)
 //   k_morton          30-bit Morton code of each leaf-box centroid
 //   radix sort        stable (code, object index) sort, sort.hip
 //   k_karras          Karras (2012) radix-tree emission.  Keys are compared
@@ -78,7 +82,8 @@ __global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
-        *list_a = 0u;
+        list_a[0] = 0u;
+        list_a[1] = 0u; // k_refit's next cut
         *list_b = 0u;
     }
     if (t >= 8 * M)
@@ -87,12 +92,15 @@ __global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32
     bits[t] = k < 3 ? f2ord(F_INF) : f2ord(-F_INF);
 }
 
-// block reduction of a scene box into the mesh accumulators: wave butterfly,
-// then one atomic per component per block when the whole block holds one
-// mesh (always for a single mesh), else per wave or per lane
+// block reduction of a scene box: wave butterfly, then, when the whole block
+// holds one mesh (always for a single mesh), the block's box in its partial
+// record part[8 b, 8 b + 8) (box slots 0-5, then the mesh) for k_box_final;
+// else atomics per wave or per lane and a record marked INVALID.  Thousands
+// of workgroups updating the same six words one after the other would
+// serialise on them.
 template <int NV>
 __device__ __forceinline__ void reduce_scene_box(float (&v)[NV], uint32_t mesh, bool valid, int* scene_bits,
-                                                 const int* slot_of)
+                                                 const int* slot_of, float* __restrict__ part)
 {
     __shared__ float red[BT / 64][NV];
     __shared__ uint32_t redm[BT / 64];
@@ -124,27 +132,116 @@ __device__ __forceinline__ void reduce_scene_box(float (&v)[NV], uint32_t mesh, 
         redu[w] = uni ? 1 : 0;
     }
     __syncthreads();
-    if (threadIdx.x < NV) {
+    if (threadIdx.x < 8) {
         const int k = threadIdx.x;
+        float* rec = part + 8 * blockIdx.x;
         bool same = true;
         for (int q = 0; q < BT / 64; ++q)
             same = same && redu[q] && redm[q] == redm[0];
-        if (same) {
-            float acc = red[0][k];
-            for (int q = 1; q < BT / 64; ++q)
-                acc = k < NV / 2 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
-            if (k < NV / 2)
-                atomicMin(scene_bits + 8 * redm[0] + slot_of[k], f2ord(acc));
+        if (k == 7)
+            rec[6] = __uint_as_float(same ? redm[0] : INVALID);
+        if (NV == 4 && k == 6) { // slots of z in 2D
+            rec[2] = F_INF;
+            rec[5] = -F_INF;
+        }
+        if (k < NV) {
+            if (same) {
+                float acc = red[0][k];
+                for (int q = 1; q < BT / 64; ++q)
+                    acc = k < NV / 2 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
+                rec[slot_of[k]] = acc;
+            } else {
+                for (int q = 0; q < BT / 64; ++q) {
+                    if (!redu[q])
+                        continue;
+                    if (k < NV / 2)
+                        atomicMin(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+                    else
+                        atomicMax(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+                }
+            }
+        }
+    }
+}
+
+// the partial boxes of reduce_scene_box into the mesh accumulators: one
+// record per thread, a wave butterfly when the wave's records hold one mesh,
+// then one atomic per component per workgroup when all its waves do
+constexpr int BF = 1024;
+constexpr uint32_t WAVE_EMPTY = 0xFFFFFFFEu; // no record of a mesh in the wave
+__global__ __launch_bounds__(BF) void k_box_final(const float* __restrict__ part, uint32_t n_part,
+                                                  int* __restrict__ scene_bits)
+{
+    __shared__ float red[BF / 64][6];
+    __shared__ uint32_t redm[BF / 64]; // the wave's mesh, WAVE_EMPTY, or INVALID (mixed: already applied)
+    const uint32_t i = blockIdx.x * BF + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t mesh = INVALID;
+    float v[6] = {F_INF, F_INF, F_INF, -F_INF, -F_INF, -F_INF};
+    if (i < n_part) {
+        mesh = __float_as_uint(part[8 * i + 6]);
+        if (mesh != INVALID) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                v[k] = part[8 * i + k];
+        }
+    }
+    const uint64_t live = __ballot(mesh != INVALID);
+    const uint32_t m0 = live ? __shfl(mesh, __ffsll((long long)live) - 1, 64) : WAVE_EMPTY;
+    const bool uni = live == 0ull || __ballot(mesh != INVALID && mesh != m0) == 0ull;
+    if (uni) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                v[k] = k < 3 ? fmn(v[k], __shfl_xor(v[k], o, 64)) : fmx(v[k], __shfl_xor(v[k], o, 64));
+        }
+    } else if (mesh != INVALID) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k < 3)
+                atomicMin(scene_bits + 8 * mesh + k, f2ord(v[k]));
             else
-                atomicMax(scene_bits + 8 * redm[0] + slot_of[k], f2ord(acc));
-        } else {
-            for (int q = 0; q < BT / 64; ++q) {
-                if (!redu[q])
+                atomicMax(scene_bits + 8 * mesh + k, f2ord(v[k]));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            red[w][k] = v[k];
+        redm[w] = uni ? m0 : INVALID;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t mb = WAVE_EMPTY;
+        bool same = true;
+        for (int q = 0; q < BF / 64; ++q) {
+            const uint32_t mq = redm[q];
+            if (mq == WAVE_EMPTY)
+                continue;
+            if (mb == WAVE_EMPTY)
+                mb = mq;
+            same = same && mq == mb && mq != INVALID;
+        }
+        if (same && mb != WAVE_EMPTY) {
+            float acc = k < 3 ? F_INF : -F_INF;
+            for (int q = 0; q < BF / 64; ++q)
+                if (redm[q] != WAVE_EMPTY)
+                    acc = k < 3 ? fmn(acc, red[q][k]) : fmx(acc, red[q][k]);
+            if (k < 3)
+                atomicMin(scene_bits + 8 * mb + k, f2ord(acc));
+            else
+                atomicMax(scene_bits + 8 * mb + k, f2ord(acc));
+        } else if (!same) {
+            for (int q = 0; q < BF / 64; ++q) {
+                const uint32_t mq = redm[q];
+                if (mq == WAVE_EMPTY || mq == INVALID)
                     continue;
-                if (k < NV / 2)
-                    atomicMin(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+                if (k < 3)
+                    atomicMin(scene_bits + 8 * mq + k, f2ord(red[q][k]));
                 else
-                    atomicMax(scene_bits + 8 * redm[q] + slot_of[k], f2ord(red[q][k]));
+                    atomicMax(scene_bits + 8 * mq + k, f2ord(red[q][k]));
             }
         }
     }
@@ -166,37 +263,86 @@ __device__ __forceinline__ v3 edge_face_normal_raw(const float4* V, int4 s, int 
     return cross3(mk(pb.x - pa.x, pb.y - pa.y, pb.z - pa.z), mk(pc.x - pa.x, pc.y - pa.y, pc.z - pa.z));
 }
 
+// Per-wave LDS of k_leafgen's flattened edge work: one round of 64 edge
+// slots (edge id, owner lane) with the owner's box centroid and mesh edge
+// offset, and per slot the results the owner folds into its cone: unit edge
+// normal, unit face normals, the two radius terms and the face bits.
+struct LeafEdgeLds {
+    uint32_t edge[64];
+    uint32_t owner[64];
+    float cx[64], cy[64], cz[64];
+    uint32_t eoff[64];
+    float n[3][64], n0[3][64], n1[3][64];
+    float ra[64], rb[64];
+    uint32_t faces[64];
+};
+
+// Leaf box and leaf cone of each triangle (scene.cuh:870-961) and the packed
+// silhouette-edge records.  A triangle owns 0..3 edges; rather than looping
+// over them per lane (three divergent iterations per wave), the wave lists
+// every owned edge as a run of slots and, in rounds of 64 slots, each lane
+// computes the normals and radius terms of one edge and writes its record.
+// Each owner then folds its edges in their original order, so the cone's
+// sums and maxima are formed exactly as in the per-triangle loop.
 __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                                 const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
                                                 uint32_t nF, int conservative, float4* __restrict__ leafbox,
                                                 float4* __restrict__ leafcone, float4* __restrict__ tedge,
                                                 int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh,
-                                                const MeshDesc* __restrict__ desc)
+                                                const MeshDesc* __restrict__ desc, float* __restrict__ part)
 {
+    __shared__ LeafEdgeLds lds[BT / 64];
+    LeafEdgeLds& L = lds[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     const uint32_t mesh = mesh_of_prim(pmesh, i < nF ? i : nF - 1);
     const uint32_t eoff = desc[mesh].eoff;
     box3 b;
     b.lo = mk(F_INF, F_INF, F_INF);
     b.hi = mk(-F_INF, -F_INF, -F_INF);
+    uint32_t e0 = 0, my = 0;
+    v3 c = mk(0.0f, 0.0f, 0.0f);
     if (i < nF) {
         const v3 p0 = ld3(V, F[3 * i]), p1 = ld3(V, F[3 * i + 1]), p2 = ld3(V, F[3 * i + 2]);
         b = box_of_point(p0);
         box_expand(b, p1);
         box_expand(b, p2);
-        const v3 c = box_centroid(b);
-        // leaf cone (scene.cuh:887-961)
-        cone3 r;
-        r.axis = mk(0.0f, 0.0f, 0.0f);
-        r.half_angle = F_PI;
-        r.radius = 0.0f;
-        bool any = false, all_two = true;
-        const uint32_t e0 = estart[i], e1 = estart[i + 1];
-        v3 nn[3][2];
+        c = box_centroid(b);
+        e0 = estart[i];
+        const uint32_t e1 = estart[i + 1];
+        my = e1 > e0 ? min(e1 - e0, 3u) : 0u;
+    }
+    L.cx[lane] = c.x;
+    L.cy[lane] = c.y;
+    L.cz[lane] = c.z;
+    L.eoff[lane] = eoff;
+    const uint64_t b0 = __ballot(my & 1u), b1 = __ballot(my & 2u);
+    const uint32_t off = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt);
+    const uint32_t total = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1);
+    // leaf cone accumulators (scene.cuh:887-961)
+    cone3 r;
+    r.axis = mk(0.0f, 0.0f, 0.0f);
+    r.half_angle = F_PI;
+    r.radius = 0.0f;
+    bool all_two = true;
+    v3 nn[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        nn[k][0] = nn[k][1] = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t base = 0; base < total; base += 64) {
+#pragma unroll
         for (uint32_t k = 0; k < 3; ++k) {
-            if (e0 + k >= e1)
-                break;
-            const uint32_t e = e0 + k;
+            const uint32_t sl = off + k;
+            if (k < my && sl >= base && sl < base + 64) {
+                L.edge[sl - base] = e0 + k;
+                L.owner[sl - base] = lane;
+            }
+        }
+        // one edge per lane (LDS operations of one wave complete in order)
+        if (lane < min(64u, total - base)) {
+            const uint32_t e = L.edge[lane], o = L.owner[lane];
+            const v3 co = mk(L.cx[o], L.cy[o], L.cz[o]);
             const int4 s = slots[e];
             const bool has0 = s.w != -1, has1 = s.x != -1;
             v3 raw0 = mk(0.0f, 0.0f, 0.0f), raw1 = mk(0.0f, 0.0f, 0.0f);
@@ -211,28 +357,55 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
             }
             n = normalize3(n);
             const v3 pa = ld3(V, s.y), pb = ld3(V, s.z);
-            r.axis.x += n.x;
-            r.axis.y += n.y;
-            r.axis.z += n.z;
+            float ra, rb = 0.0f;
             if (conservative) {
-                r.radius = smax(r.radius, len3(mk(pa.x - c.x, pa.y - c.y, pa.z - c.z)));
-                r.radius = smax(r.radius, len3(mk(pb.x - c.x, pb.y - c.y, pb.z - c.z)));
+                ra = len3(mk(pa.x - co.x, pa.y - co.y, pa.z - co.z));
+                rb = len3(mk(pb.x - co.x, pb.y - co.y, pb.z - co.z));
             } else {
                 const v3 m = mk((pa.x + pb.x) / 2.0f, (pa.y + pb.y) / 2.0f, (pa.z + pb.z) / 2.0f);
-                r.radius = smax(r.radius, len3(mk(m.x - c.x, m.y - c.y, m.z - c.z)));
+                ra = len3(mk(m.x - co.x, m.y - co.y, m.z - co.z));
             }
-            all_two = all_two && has0 && has1;
-            any = true;
-            nn[k][0] = has0 ? normalize3(raw0) : mk(0.0f, 0.0f, 0.0f);
-            nn[k][1] = has1 ? normalize3(raw1) : mk(0.0f, 0.0f, 0.0f);
+            const v3 u0 = has0 ? normalize3(raw0) : mk(0.0f, 0.0f, 0.0f);
+            const v3 u1 = has1 ? normalize3(raw1) : mk(0.0f, 0.0f, 0.0f);
+            const uint32_t faces = (has0 ? 1u : 0u) | (has1 ? 2u : 0u);
             // packed silhouette-edge record (each edge has exactly one owner)
-            tedge[4 * e + 0] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(e - eoff));
-            tedge[4 * e + 1] =
-                make_float4(pb.x, pb.y, pb.z, __uint_as_float((has0 ? 1u : 0u) | (has1 ? 2u : 0u)));
-            tedge[4 * e + 2] = make_float4(nn[k][0].x, nn[k][0].y, nn[k][0].z, 0.0f);
-            tedge[4 * e + 3] = make_float4(nn[k][1].x, nn[k][1].y, nn[k][1].z, 0.0f);
+            tedge[4 * e + 0] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(e - L.eoff[o]));
+            tedge[4 * e + 1] = make_float4(pb.x, pb.y, pb.z, __uint_as_float(faces));
+            tedge[4 * e + 2] = make_float4(u0.x, u0.y, u0.z, 0.0f);
+            tedge[4 * e + 3] = make_float4(u1.x, u1.y, u1.z, 0.0f);
+            L.n[0][lane] = n.x;
+            L.n[1][lane] = n.y;
+            L.n[2][lane] = n.z;
+            L.n0[0][lane] = u0.x;
+            L.n0[1][lane] = u0.y;
+            L.n0[2][lane] = u0.z;
+            L.n1[0][lane] = u1.x;
+            L.n1[1][lane] = u1.y;
+            L.n1[2][lane] = u1.z;
+            L.ra[lane] = ra;
+            L.rb[lane] = rb;
+            L.faces[lane] = faces;
         }
-        if (!any) {
+        // each owner folds its edges of this round, in edge order
+#pragma unroll
+        for (uint32_t k = 0; k < 3; ++k) {
+            const uint32_t sl = off + k;
+            if (k < my && sl >= base && sl < base + 64) {
+                const uint32_t q = sl - base;
+                r.axis.x += L.n[0][q];
+                r.axis.y += L.n[1][q];
+                r.axis.z += L.n[2][q];
+                r.radius = smax(r.radius, L.ra[q]);
+                if (conservative)
+                    r.radius = smax(r.radius, L.rb[q]);
+                all_two = all_two && L.faces[q] == 3u;
+                nn[k][0] = mk(L.n0[0][q], L.n0[1][q], L.n0[2][q]);
+                nn[k][1] = mk(L.n1[0][q], L.n1[1][q], L.n1[2][q]);
+            }
+        }
+    }
+    if (i < nF) {
+        if (my == 0) {
             r.half_angle = -F_PI;
         } else if (!all_two) {
             r.half_angle = F_PI;
@@ -243,8 +416,9 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
                 r.axis.y /= an;
                 r.axis.z /= an;
                 r.half_angle = 0.0f;
+#pragma unroll
                 for (uint32_t k = 0; k < 3; ++k) {
-                    if (e0 + k >= e1)
+                    if (k >= my)
                         break;
                     for (int jj = 0; jj < 2; ++jj) {
                         const v3 nv = nn[k][jj];
@@ -262,7 +436,7 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
     // scene AABB of the primitive's mesh
     float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
     const int slot_of[6] = {0, 1, 2, 3, 4, 5};
-    reduce_scene_box<6>(v, mesh, i < nF, scene_bits, slot_of);
+    reduce_scene_box<6>(v, mesh, i < nF, scene_bits, slot_of, part);
 }
 
 // scene<2>: leaf box (scene.cuh:420-433) and leaf cone from the owned
@@ -272,7 +446,8 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
                                                   const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
                                                   uint32_t nS, float4* __restrict__ leafbox,
                                                   float4* __restrict__ leafcone, float4* __restrict__ tedge,
-                                                  int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh)
+                                                  int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh,
+                                                  float* __restrict__ part)
 {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
     const uint32_t mesh = mesh_of_prim(pmesh, i < nS ? i : nS - 1);
@@ -353,7 +528,7 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
     }
     float v[4] = {b.lo.x, b.lo.y, b.hi.x, b.hi.y};
     const int slot_of[4] = {0, 1, 3, 4};
-    reduce_scene_box<4>(v, mesh, i < nS, scene_bits, slot_of);
+    reduce_scene_box<4>(v, mesh, i < nS, scene_bits, slot_of, part);
 }
 
 // bvh.cuh:269-277 and morton_code.cuh:85-92
@@ -423,15 +598,31 @@ __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbo
     idx[i] = i;
 }
 
-// delta of the virtual 64-bit key (code << 32 | local index); codes and idx
-// point at the mesh's first sorted position, idx holds global primitive ids
-__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ idx,
-                                      uint32_t off, uint32_t a, uint32_t b)
+// The sorted codes around one k_karras block in LDS: positions [lo, lo + KWL)
+// (the block's own nodes and KW on either side).  Most searches of a node
+// stay within a few hundred positions of it; the rest read memory.
+constexpr int KW = 512;
+constexpr int KWL = 256 + 2 * KW;
+struct CodeWindow {
+    const uint32_t* lds;
+    int lo;
+    __device__ __forceinline__ uint32_t at(const uint32_t* __restrict__ codes_all, uint32_t pos) const
+    {
+        const uint32_t u = (uint32_t)((int)pos - lo);
+        return u < (uint32_t)KWL ? lds[u] : codes_all[pos];
+    }
+};
+
+// delta of the virtual 64-bit key (code << 32 | local index) of the mesh's
+// sorted positions a and b; codes_all and idx_all are scene-wide, idx holds
+// global primitive ids
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes_all, const uint32_t* __restrict__ idx_all,
+                                      const CodeWindow& w, uint32_t off, uint32_t a, uint32_t b)
 {
-    const uint32_t ca = codes[a], cb = codes[b];
+    const uint32_t ca = w.at(codes_all, off + a), cb = w.at(codes_all, off + b);
     if (ca != cb)
         return __clz(ca ^ cb);
-    return 32 + __clz((idx[a] - off) ^ (idx[b] - off));
+    return 32 + __clz((idx_all[off + a] - off) ^ (idx_all[off + b] - off));
 }
 
 constexpr uint32_t RT = 1024; // leaves of one refit subtree ("cut")
@@ -443,7 +634,7 @@ constexpr uint32_t RB = 512;  // threads of one k_refit workgroup (RT / RB leave
 // roots (maximal subtrees of at most RT leaves below a node of more) and
 // whether it is a top (a node of more than RT leaves)
 __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_all,
-                                            const uint32_t* __restrict__ idx_all, uint32_t t,
+                                            const uint32_t* __restrict__ idx_all, const CodeWindow& w, uint32_t t,
                                             const MeshDesc* __restrict__ desc, uint32_t M,
                                             uint32_t* __restrict__ parent, uint2* __restrict__ child,
                                             uint2* __restrict__ range, uint32_t& n_cut, uint32_t* cut,
@@ -454,30 +645,28 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
     const uint32_t i = t - (D.off - m), N = D.n, off = D.off, nb = D.nb;
     if (N < 2 || i >= N - 1)
         return; // not reached for consistent offsets
-    const uint32_t* codes = codes_all + off;
-    const uint32_t* idx = idx_all + off;
     uint32_t first, last;
     if (i == 0) {
         first = 0;
         last = N - 1;
         parent[nb] = INVALID;
     } else {
-        const int L = kdelta(codes, idx, off, i, i - 1);
-        const int R = kdelta(codes, idx, off, i, i + 1);
+        const int L = kdelta(codes_all, idx_all, w, off, i, i - 1);
+        const int R = kdelta(codes_all, idx_all, w, off, i, i + 1);
         const int d = (R > L) ? 1 : -1;
         const int dmin = L < R ? L : R;
         int lmax = 2;
         int itmp = (int)i + d * lmax;
-        int delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
+        int delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes_all, idx_all, w, off, i, (uint32_t)itmp) : -1;
         while (delta > dmin) {
             lmax <<= 1;
             itmp = (int)i + d * lmax;
-            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes_all, idx_all, w, off, i, (uint32_t)itmp) : -1;
         }
         int l = 0;
         for (int s = lmax >> 1; s > 0; s >>= 1) {
             itmp = (int)i + (l + s) * d;
-            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes, idx, off, i, (uint32_t)itmp) : -1;
+            delta = (0 <= itmp && itmp < (int)N) ? kdelta(codes_all, idx_all, w, off, i, (uint32_t)itmp) : -1;
             if (delta > dmin)
                 l += s;
         }
@@ -488,14 +677,14 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
     // find_split; first != last always holds for internal nodes and the
     // virtual 64-bit keys are unique, so the equal-key branch of the
     // reference (bvh.cuh:175-178) cannot trigger
-    const int dnode = kdelta(codes, idx, off, first, last);
+    const int dnode = kdelta(codes_all, idx_all, w, off, first, last);
     int split = (int)first;
     int stride = (int)(last - first);
     do {
         stride = (stride + 1) >> 1;
         const int mid = split + stride;
         if (mid < (int)last) {
-            if (kdelta(codes, idx, off, first, (uint32_t)mid) > dnode)
+            if (kdelta(codes_all, idx_all, w, off, first, (uint32_t)mid) > dnode)
                 split = mid;
         }
     } while (stride > 1);
@@ -523,20 +712,30 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
 // ("cuts") and the nodes above them ("tops"); flags[top] = its list slot.
 __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes_all,
                                                const uint32_t* __restrict__ idx_all, uint32_t n_internal,
-                                               const MeshDesc* __restrict__ desc, uint32_t M,
+                                               uint32_t n_all, const MeshDesc* __restrict__ desc, uint32_t M,
                                                uint32_t* __restrict__ parent, uint2* __restrict__ child,
                                                uint2* __restrict__ range, uint32_t* __restrict__ flags,
                                                uint32_t* __restrict__ cuts, uint32_t cut_cap,
                                                uint32_t* __restrict__ tops, uint32_t top_cap)
 {
+    static_assert(BT == 256, "the code window spans BT nodes");
     __shared__ uint32_t s_n[2], s_base[2]; // cuts and tops of this block: counts, then list offsets
+    __shared__ uint32_t win[KWL];
     if (threadIdx.x < 2)
         s_n[threadIdx.x] = 0u;
+    // internal-record slot t of mesh m is the node at sorted position t + m
+    const uint32_t t0 = blockIdx.x * BT;
+    const int lo = (int)(t0 + mesh_search<false>(desc, M, t0)) - KW;
+    for (int u = threadIdx.x; u < KWL; u += BT) {
+        const int pos = lo + u;
+        win[u] = pos >= 0 && pos < (int)n_all ? codes_all[pos] : 0u; // positions outside [0, n_all) are never read
+    }
     __syncthreads();
-    const uint32_t t = blockIdx.x * BT + threadIdx.x;
+    const CodeWindow w{win, lo};
+    const uint32_t t = t0 + threadIdx.x;
     uint32_t n_cut = 0, cut[2] = {INVALID, INVALID}, self = INVALID; // self: set for a top
     if (t < n_internal)
-        karras_node(codes_all, idx_all, t, desc, M, parent, child, range, n_cut, cut, self);
+        karras_node(codes_all, idx_all, w, t, desc, M, parent, child, range, n_cut, cut, self);
     // one list append per block and list (the counters are shared by all blocks)
     const uint32_t cut_k = n_cut ? atomicAdd(&s_n[0], n_cut) : 0u;
     const uint32_t top_k = self != INVALID ? atomicAdd(&s_n[1], 1u) : 0u;
@@ -647,12 +846,21 @@ __device__ __forceinline__ void refit_write(const MeshIds& id, uint32_t p, uint2
     tnode[4 * r + 3] = make_float4(__uint_as_float(id.ref(ch.x)), __uint_as_float(id.ref(ch.y)),
                                    __uint_as_float(run_of(id, ch.x, range, leaf_size)),
                                    __uint_as_float(run_of(id, ch.y, range, leaf_size)));
-    float ls, lcs, rs, rcs;
-    dm::sincosf_(lc.half_angle, &ls, &lcs);
-    dm::sincosf_(rc.half_angle, &rs, &rcs);
     tcone[4 * r + 0] = make_float4(lc.axis.x, lc.axis.y, lc.axis.z, lc.half_angle);
     tcone[4 * r + 1] = make_float4(rc.axis.x, rc.axis.y, rc.axis.z, rc.half_angle);
     tcone[4 * r + 2] = make_float4(lc.radius, rc.radius, 0.0f, 0.0f);
+    // tcone[4 r + 3] (sin, cos of the half angles) is filled by tcone_trig
+    // once the merges are done: off the chain of dependent merges
+}
+
+// sin and cos of both children's half angles of internal record r, read
+// back from the record's first two entries
+__device__ __forceinline__ void tcone_trig(float4* __restrict__ tcone, uint32_t r)
+{
+    const float hl = tcone[4 * r + 0].w, hr = tcone[4 * r + 1].w;
+    float ls, lcs, rs, rcs;
+    dm::sincosf_(hl, &ls, &lcs);
+    dm::sincosf_(hr, &rs, &rcs);
     tcone[4 * r + 3] = make_float4(ls, lcs, rs, rcs);
 }
 
@@ -702,6 +910,19 @@ __device__ __forceinline__ uint16_t local_parent(const MeshIds& id, uint32_t p, 
     return id.internal(p) && id.pos(p) - first <= last - first ? (uint16_t)(id.pos(p) - first) : NO_PARENT;
 }
 
+// A cut's internal node at local position k (Karras position first + k), in
+// one LDS word: left and right child as local positions (bit 10: a leaf) and
+// the node's slot (its first leaf - first).
+__device__ __forceinline__ uint32_t cut_child_local(const MeshIds& id, uint32_t c, uint32_t first)
+{
+    return id.internal(c) ? id.pos(c) - first : (1u << 10) | (id.pos(c) - first);
+}
+__device__ __forceinline__ uint32_t cut_child_global(const MeshIds& id, uint32_t lc, uint32_t first)
+{
+    const uint32_t j = first + (lc & 0x3FFu) - id.off; // mesh-local position
+    return lc & (1u << 10) ? id.nb + (id.n - 1) + j : id.nb + j;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
@@ -711,21 +932,32 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
                                               const MeshDesc* __restrict__ desc, uint32_t M, uint32_t N_all,
                                               int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
                                               float4* __restrict__ tnode, float4* __restrict__ tcone,
-                                              float4* __restrict__ tleaf, const uint32_t* __restrict__ cuts,
+                                              float4* __restrict__ tleaf, uint32_t* __restrict__ cuts,
                                               uint32_t cut_cap)
 {
+    static_assert(RT == 1024, "10-bit local positions");
     constexpr uint32_t LPT = RT / RB; // leaves per thread
     __shared__ NodeLds nd;
-    __shared__ uint2 lch[RT];         // children of the internal node at Karras position first + k
-    __shared__ uint16_t lslot[RT];    // its slot (range.x - first)
-    __shared__ uint16_t lpar[RT];     // its parent (Karras position - first), or NO_PARENT for the cut root
-    __shared__ uint16_t lpl[RT];      // parent of leaf first + u, or NO_PARENT
-    __shared__ uint32_t lflag[RT];    // arrivals at internal node k
+    __shared__ uint32_t lnode[RT];    // children and slot of the internal node at local position k
+    __shared__ uint16_t lpar[RT];     // its parent (local position), or NO_PARENT for the cut root
+    __shared__ uint32_t larr[RT / 32]; // first arrival at internal node k (bit k)
     __shared__ uint16_t queue[RT];
     __shared__ RoundQueue rq;
+    __shared__ uint32_t s_cut;
     const uint32_t t = threadIdx.x;
     const uint32_t count = min(cuts[cut_cap], cut_cap);
-    for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
+    if (t < 3)
+        rq.n[t] = 0u;
+    if (t < RT / 32)
+        larr[t] = 0u;
+    for (;;) {
+        // workgroups take cuts in turn (cut sizes vary)
+        if (t == 0)
+            s_cut = atomicAdd(cuts + cut_cap + 1, 1u);
+        __syncthreads(); // also orders the cleared counters and arrival bits
+        const uint32_t c = s_cut;
+        if (c >= count)
+            break;
         const uint32_t root = cuts[c];
         const uint32_t m = mesh_search<true>(desc, M, root);
         const MeshDesc D = desc[m];
@@ -739,37 +971,40 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
             first = last = id.pos(root);
         }
         const bool ok = last - first < RT && last < N_all; // holds for every cut of a Karras tree
-        if (t < 3)
-            rq.n[t] = 0u;
-        // the cut's topology, one coalesced load per array (an internal node of
-        // the cut sits at a Karras position in [first, last]; the position not
-        // used by the cut may name a node outside it, which is never read back)
+        // the cut's topology, one coalesced load per array: the internal
+        // nodes of the cut are the Karras positions in [first, last] whose
+        // range lies inside it.  A node with two leaf children is ready at
+        // once; one with one leaf child has had its first arrival.
 #pragma unroll
         for (uint32_t h = 0; h < LPT; ++h) {
             const uint32_t u = t + h * RB, j = first + u;
-            lflag[u] = 0u;
-            if (ok && j <= last) {
-                const uint32_t leaf = D.nb + (D.n - 1) + (j - D.off);
-                lpl[u] = leaf == root ? NO_PARENT : local_parent(id, parent[leaf], first, last);
-                if (j - D.off < D.n - 1) {
-                    const uint32_t ni = D.nb + (j - D.off);
-                    lch[u] = child[ni];
-                    lslot[u] = (uint16_t)min(range[ni].x - first, RT - 1);
+            if (ok && j <= last && j - D.off < D.n - 1) {
+                const uint32_t ni = D.nb + (j - D.off);
+                const uint2 rg = range[ni];
+                if (rg.x >= first && rg.y <= last) {
+                    const uint2 ch = child[ni];
+                    const uint32_t lc = cut_child_local(id, ch.x, first), rc = cut_child_local(id, ch.y, first);
+                    lnode[u] = lc | (rc << 11) | ((rg.x - first) << 22);
                     lpar[u] = ni == root ? NO_PARENT : local_parent(id, parent[ni], first, last);
+                    const uint32_t leaves = (lc >> 10) + (rc >> 10);
+                    if (leaves == 2u)
+                        rq.push(queue, 0u, 0u, u);
+                    else if (leaves == 1u)
+                        atomicOr(&larr[u >> 5], 1u << (u & 31u));
                 }
             }
         }
-        bool leaf_ok[LPT];
+        bool bad = !ok;
 #pragma unroll
         for (uint32_t h = 0; h < LPT; ++h) {
             const uint32_t u = t + h * RB, j = first + u;
-            leaf_ok[h] = false;
             if (!(ok && j <= last))
                 continue;
             const uint32_t orig = sidx[j];
-            if (orig - D.off >= D.n)
-                continue; // not a permutation of the mesh's primitives: the node stays unwritten
-            leaf_ok[h] = true;
+            if (orig - D.off >= D.n) {
+                bad = true; // not a permutation of the mesh's primitives: the cut stays unwritten
+                continue;
+            }
             const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
             const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
             const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
@@ -803,33 +1038,27 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
                 tleaf[3 * j + 2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(ec));
             }
         }
-        __syncthreads(); // topology, leaf slots and cleared counters
-        // round 0: the leaves arrive at their parents
-#pragma unroll
-        for (uint32_t h = 0; h < LPT; ++h) {
-            const uint32_t u = t + h * RB;
-            if (leaf_ok[h] && lpl[u] != NO_PARENT && atomicAdd(&lflag[lpl[u]], 1u) == 1u)
-                rq.push(queue, 0u, 0u, lpl[u]);
-        }
-        __syncthreads();
-        uint32_t qs = 0, qe = rq.n[0];
+        // topology, leaf slots and the ready nodes
+        uint32_t qs = 0, qe = __syncthreads_or(bad ? 1 : 0) ? 0u : rq.n[0];
         for (uint32_t round = 1; qs < qe && round < RT; ++round) {
             if (t == 0)
                 rq.n[(round + 1) % 3] = 0u;
             for (uint32_t i = qs + t; i < qe; i += RB) {
                 const uint32_t k = queue[i];
-                const uint2 ch = lch[k];
-                const uint32_t ls = lslot[k], rs = min(id.pos(ch.x) + 1u - first, RT - 1);
+                const uint32_t wd = lnode[k];
+                const uint32_t lc = wd & 0x7FFu, rc = (wd >> 11) & 0x7FFu;
+                const uint32_t ls = wd >> 22, rs = rc & 0x3FFu; // the right child's first leaf is its position
+                const uint2 ch = make_uint2(cut_child_global(id, lc, first), cut_child_global(id, rc, first));
                 box3 lb, rb, pb;
-                cone3 lc, rc, pc;
-                nd.get(ls, lb, lc);
-                nd.get(rs, rb, rc);
+                cone3 lc3, rc3, pc;
+                nd.get(ls, lb, lc3);
+                nd.get(rs, rb, rc3);
                 const uint32_t p = D.nb + (first + k - D.off);
-                refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
+                refit_write<DIM>(id, p, ch, lb, rb, lc3, rc3, conservative, range, leaf_size, pb, pc, nbox, ncone,
                                  tnode, tcone, p == root);
                 nd.put(ls, pb, pc);
                 const uint32_t kp = lpar[k];
-                if (kp != NO_PARENT && atomicAdd(&lflag[kp], 1u) == 1u)
+                if (kp != NO_PARENT && (atomicOr(&larr[kp >> 5], 1u << (kp & 31u)) >> (kp & 31u)) & 1u)
                     rq.push(queue, qe, round, kp);
             }
             __syncthreads();
@@ -837,14 +1066,24 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
             qs = qe;
             qe += added;
         }
+        // every merged node is in queue[0, qe); the barrier above orders its
+        // record's writes before these reads
+        for (uint32_t i = t; i < qe; i += RB)
+            tcone_trig(tcone, D.nb + (first + queue[i] - D.off) - id.off);
+        // clear for the next cut (after the barrier that ends this one's rounds)
+        if (t < 3)
+            rq.n[t] = 0u;
+        if (t < RT / 32)
+            larr[t] = 0u;
         __syncthreads(); // the next cut reuses the LDS
     }
 }
 
 // k_refit_top keeps, when the scene has at most TOP_BOX cuts and tops, the
 // box and cone of every live node in LDS (slot: the cut's index; a top takes
-// over its left child's slot) with each top's parent slot and arrival count,
-// children and children's slots and a queue entry, 18 B per top.  Up to
+// over its left child's slot) with each top's node, mesh, parent slot and
+// arrival count, children and children's slots and a queue entry, 26 B per
+// top, so a round reads nothing from memory but the mesh's offsets.  Up to
 // TOP_LDS tops, the topology alone (14 B per top) sits in LDS and the boxes
 // are read from memory.  Beyond that it climbs from the cut roots with
 // arrival counters in memory.
@@ -852,13 +1091,13 @@ constexpr uint32_t TOP_BOX = 2048;
 constexpr uint32_t TOP_LDS = 11264;
 constexpr uint32_t TOP_ROUNDS = 128; // a Karras tree over 64-bit keys is at most 64 levels deep
 constexpr uint32_t TOP_SMEM = TOP_LDS * 14;
-static_assert(TOP_BOX * (44 + 18) <= TOP_SMEM, "LDS layouts of k_refit_top");
+static_assert(TOP_BOX * (44 + 26) <= TOP_SMEM, "LDS layouts of k_refit_top");
 
 // topology of the tops: tword[k] = parent slot << 16 (0xFFFF: a mesh root),
-// tch[k] = children
+// tch[k] = children, tnode_id[k] (optional) = the node
 __device__ __forceinline__ void load_tops(const uint32_t* __restrict__ parent, const uint2* __restrict__ child,
                                           const uint32_t* flags, const uint32_t* __restrict__ tops, uint32_t n_top,
-                                          uint32_t n_nodes, uint32_t* tword, uint2* tch)
+                                          uint32_t n_nodes, uint32_t* tword, uint2* tch, uint32_t* tnode_id = nullptr)
 {
     for (uint32_t base = threadIdx.x; base < n_top; base += 4 * 1024) {
         uint32_t p[4], pp[4];
@@ -880,6 +1119,8 @@ __device__ __forceinline__ void load_tops(const uint32_t* __restrict__ parent, c
                 const uint32_t ps = pp[u] < n_nodes ? flags[pp[u]] : 0xFFFFu;
                 tword[k] = min(ps, 0xFFFFu) << 16;
                 tch[k] = ch[u];
+                if (tnode_id)
+                    tnode_id[k] = p[u];
             }
         }
     }
@@ -902,12 +1143,23 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
     const uint32_t n_cut = min(cuts[cut_cap], cut_cap);
     if (t < 3)
         rq.n[t] = 0u;
+    // once every top is merged: the sin and cos of its children's half angles
+    auto trig_tops = [&]() {
+        __syncthreads(); // the records' writes precede their reads
+        for (uint32_t k = t; k < n_top; k += 1024) {
+            const uint32_t p = tops[k];
+            if (p < n_nodes)
+                tcone_trig(tcone, p - desc[mesh_search<true>(desc, M, p)].off);
+        }
+    };
     if (n_top <= TOP_BOX && n_cut <= TOP_BOX) {
         float(*bx)[TOP_BOX] = reinterpret_cast<float(*)[TOP_BOX]>(smem); // 11 rows: box, cone
         uint2* tch = reinterpret_cast<uint2*>(smem + 44 * TOP_BOX);
         uint32_t* tword = reinterpret_cast<uint32_t*>(smem + 52 * TOP_BOX);
         uint32_t* tslot = reinterpret_cast<uint32_t*>(smem + 56 * TOP_BOX); // children's slots, left | right << 16
         uint16_t* queue = reinterpret_cast<uint16_t*>(smem + 60 * TOP_BOX);
+        uint32_t* tp = reinterpret_cast<uint32_t*>(smem + 62 * TOP_BOX); // node of each top
+        uint32_t* tm = reinterpret_cast<uint32_t*>(smem + 66 * TOP_BOX); // its mesh
         auto put = [&](uint32_t sl, const box3& b, const cone3& c) {
             bx[0][sl] = b.lo.x; bx[1][sl] = b.lo.y; bx[2][sl] = b.lo.z;
             bx[3][sl] = b.hi.x; bx[4][sl] = b.hi.y; bx[5][sl] = b.hi.z;
@@ -921,9 +1173,11 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
             c.half_angle = bx[9][sl];
             c.radius = bx[10][sl];
         };
-        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch);
-        for (uint32_t k = t; k < n_top; k += 1024)
+        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch, tp);
+        for (uint32_t k = t; k < n_top; k += 1024) {
             tslot[k] = 0u;
+            tm[k] = mesh_search<true>(desc, M, tp[k]); // each thread reads back its own tp[k]
+        }
         __syncthreads();
         // the cut roots (written by k_refit) take their slots and arrive
         for (uint32_t c = t; c < n_cut; c += 1024) {
@@ -946,10 +1200,10 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
                 rq.n[(round + 1) % 3] = 0u;
             for (uint32_t i = qs + t; i < qe; i += 1024) {
                 const uint32_t k = queue[i];
-                const uint32_t p = tops[k];
+                const uint32_t p = tp[k];
                 const uint2 ch = tch[k];
                 const uint32_t ls = min(tslot[k] & 0xFFFFu, TOP_BOX - 1), rs = min(tslot[k] >> 16, TOP_BOX - 1);
-                const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
+                const MeshDesc D = desc[min(tm[k], M - 1)];
                 const MeshIds id{D.off, D.n, D.nb};
                 box3 lb, rb, pb;
                 cone3 lc, rc, pc;
@@ -970,6 +1224,7 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
             qs = qe;
             qe += added;
         }
+        trig_tops();
         return;
     }
     if (n_top <= TOP_LDS) {
@@ -1015,6 +1270,7 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
             qs = qe;
             qe += added;
         }
+        trig_tops();
         return;
     }
     // more tops than LDS holds: climb from the cut roots with arrival
@@ -1042,6 +1298,7 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
             p = parent[p];
         }
     }
+    trig_tops();
 }
 
 
@@ -1081,6 +1338,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(w.leafbox.ensure(sizeof(float4) * 2 * N));
     SNCH_TRY(w.leafcone.ensure(sizeof(float4) * 2 * N));
     SNCH_TRY(w.scene_bits.ensure(sizeof(int) * 8 * M));
+    SNCH_TRY(w.boxpart.ensure(sizeof(float) * 8 * blocks(N)));
     SNCH_TRY(w.codes.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.idx.ensure(sizeof(uint32_t) * N));
     SNCH_TRY(w.codes_alt.ensure(sizeof(uint32_t) * N));
@@ -1090,7 +1348,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(w.range.ensure(sizeof(uint2) * NN));
     SNCH_TRY(w.flags.ensure(sizeof(uint32_t) * NN));
     const uint32_t cut_cap = N; // cuts are disjoint subtrees of at least one leaf
-    SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 1)));
+    SNCH_TRY(w.top.ensure(sizeof(uint32_t) * (cut_cap + 2))); // cuts, their count, k_refit's next cut
     uint32_t* cuts = w.top.as<uint32_t>();
     const uint32_t top_cap = NI > 0 ? NI : 1u; // tops are internal nodes
     SNCH_TRY(w.tops.ensure(sizeof(uint32_t) * (top_cap + 1)));
@@ -1108,14 +1366,20 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc);
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc,
+                           w.boxpart.as<float>());
+        hipLaunchKernelGGL(k_box_final, dim3((blocks(N) + BF - 1) / BF), dim3(BF), 0, st, w.boxpart.as<float>(),
+                           blocks(N), w.scene_bits.as<int>());
         hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
                            w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh, desc, M,
                            w.parent.as<uint32_t>(), cuts, cut_cap);
     } else {
         hipLaunchKernelGGL(k_leafgen2d, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh);
+                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh,
+                           w.boxpart.as<float>());
+        hipLaunchKernelGGL(k_box_final, dim3((blocks(N) + BF - 1) / BF), dim3(BF), 0, st, w.boxpart.as<float>(),
+                           blocks(N), w.scene_bits.as<int>());
         hipLaunchKernelGGL(k_morton2d, dim3(blocks(N)), dim3(BT), 0, st, w.leafbox.as<float4>(),
                            w.scene_bits.as<int>(), N, w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), pmesh, desc, M,
                            w.parent.as<uint32_t>(), cuts, cut_cap);
@@ -1132,7 +1396,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
     mark(2);
     if (NI > 0)
-        hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, desc, M,
+        hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, N, desc, M,
                            w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(),
                            cuts, cut_cap, tops, top_cap);
     // about 2N / RT cuts (each holds at least RT / 2 leaves but for one per

@@ -102,8 +102,8 @@ struct BatchConfig {
 };
 
 struct BuildScratch {
-    DeviceBuffer leafbox, leafcone, scene_bits, codes, idx, codes_alt, idx_alt, hist, parent, child, range, flags,
-        top, tops, status;
+    DeviceBuffer leafbox, leafcone, scene_bits, boxpart, codes, idx, codes_alt, idx_alt, hist, parent, child, range,
+        flags, top, tops, status;
 };
 
 } // namespace snch
