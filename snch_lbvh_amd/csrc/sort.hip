@@ -17,6 +17,9 @@
 //                 (inclusive), flag and count in one 32-bit word.  The digit
 //                 base (an exclusive scan of the pass's totals) plus that
 //                 prefix plus the in-tile rank is each pair's destination.
+//                 The pairs go through LDS in sorted tile order first, so
+//                 each run of equal digits leaves as consecutive stores
+//                 rather than one scattered store per lane.
 // Ranks follow tile order (wave, row, lane), so equal keys keep their input
 // order: the sort is stable, which the reference relies on (objects with
 // equal Morton codes stay in index order).  Tiles are numbered by an atomic
@@ -97,6 +100,9 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
     __shared__ uint32_t toff[RADIX];
     __shared__ uint32_t wsum[WAVES];
     __shared__ uint32_t tile_s;
+    __shared__ uint32_t gdelta[RADIX];
+    __shared__ uint32_t sk[TILE], svl[TILE];
+    __shared__ uint8_t sd[TILE];
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0)
         tile_s = atomicAdd(tile_ctr, 1u);
@@ -157,17 +163,34 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
         rank[i] = before + r;
     }
     __syncthreads();
+    // thread tid = digit: this tile's count, published at once, and its
+    // start within the tile (a scan over the digits)
+    uint32_t c[WAVES], own = 0, lincl = 0;
     if (tid < RADIX) {
-        // thread tid = digit: this tile's count, then the count of every
-        // tile before it (decoupled look-back)
-        uint32_t c[WAVES], own = 0;
 #pragma unroll
         for (int q = 0; q < WAVES; ++q) {
             c[q] = wcnt[q][tid];
             own += c[q];
         }
+        __hip_atomic_store(status + (size_t)tile * RADIX + tid, (tile == 0 ? LB_INC : LB_AGG) | own,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lincl = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(lincl, o, 64);
+            if (lane >= (uint32_t)o)
+                lincl += y;
+        }
+        if (lane == 63)
+            wsum[w] = lincl;
+    }
+    __syncthreads();
+    if (tid < RADIX) {
+        uint32_t lstart = lincl - own;
+        for (uint32_t q = 0; q < w; ++q)
+            lstart += wsum[q];
         uint32_t* word = status + (size_t)tile * RADIX + tid;
-        __hip_atomic_store(word, (tile == 0 ? LB_INC : LB_AGG) | own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the count of every tile before this one (decoupled look-back):
         // predecessors are read LB_WIN at a time (independent loads), nearest
         // first, until one holds an inclusive count
         uint32_t excl = 0;
@@ -202,23 +225,38 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
         }
         if (tile > 0)
             __hip_atomic_store(word, LB_INC | (excl + own), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t run = toff[tid] + excl;
+        // tile position of wave q's first key of this digit; a key's
+        // destination is its tile position + gdelta[digit]
+        uint32_t run = lstart;
 #pragma unroll
         for (int q = 0; q < WAVES; ++q) {
             wcnt[q][tid] = run;
             run += c[q];
         }
+        gdelta[tid] = toff[tid] + excl - lstart;
     }
     __syncthreads();
+    // keys in tile order through LDS, then out in runs of equal digits
 #pragma unroll
     for (int i = 0; i < SI; ++i) {
-        bool valid = base + i * 64 + lane < n;
-        if (valid) {
-            uint32_t d = (dsrc[i] >> shift) & 0xFFu;
-            uint32_t dst = wcnt[w][d] + rank[i];
+        if (base + i * 64 + lane < n) {
+            const uint32_t d = (dsrc[i] >> shift) & 0xFFu;
+            const uint32_t lp = min(wcnt[w][d] + rank[i], (uint32_t)TILE - 1u); // < TILE when ranks agree
+            sk[lp] = k[i];
+            svl[lp] = v[i];
+            sd[lp] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_n = min((uint32_t)TILE, n - tile * TILE);
+#pragma unroll
+    for (int i = 0; i < SI; ++i) {
+        const uint32_t j = i * ST + tid;
+        if (j < tile_n) {
+            const uint32_t dst = j + gdelta[sd[j]];
             if (dst < n) { // always true when histogram and ranks agree
-                kout[dst] = k[i];
-                vout[dst] = v[i];
+                kout[dst] = sk[j];
+                vout[dst] = svl[j];
             }
         }
     }
