@@ -247,22 +247,6 @@ __global__ __launch_bounds__(BF) void k_box_final(const float* __restrict__ part
     }
 }
 
-// normal of face f of a silhouette edge (scene.cuh:747-772); slots (opp0, lo, hi, opp1)
-__device__ __forceinline__ v3 edge_face_normal_raw(const float4* V, int4 s, int f)
-{
-    v3 pa, pb, pc;
-    if (f == 0) {
-        pa = ld3(V, s.y);
-        pb = ld3(V, s.z);
-        pc = ld3(V, s.w);
-    } else {
-        pa = ld3(V, s.z);
-        pb = ld3(V, s.y);
-        pc = ld3(V, s.x);
-    }
-    return cross3(mk(pb.x - pa.x, pb.y - pa.y, pb.z - pa.z), mk(pc.x - pa.x, pc.y - pa.y, pc.z - pa.z));
-}
-
 // Per-wave LDS of k_leafgen's flattened edge work: one round of 64 edge
 // slots (edge id, owner lane) with the owner's box centroid and mesh edge
 // offset, and per slot the results the owner folds into its cone: unit edge
@@ -345,18 +329,23 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
             const v3 co = mk(L.cx[o], L.cy[o], L.cz[o]);
             const int4 s = slots[e];
             const bool has0 = s.w != -1, has1 = s.x != -1;
+            // the edge's four vertices, each read once; slots are (opp0, lo, hi,
+            // opp1), face 0 is (pa, pb, opp0) and face 1 is (pb, pa, opp1), and a
+            // face normal is the cross product of its first two edges from its
+            // first vertex (scene.cuh:747-772)
+            const v3 pa = ld3(V, s.y), pb = ld3(V, s.z);
+            const v3 o0 = has0 ? ld3(V, s.w) : pa, o1 = has1 ? ld3(V, s.x) : pb;
             v3 raw0 = mk(0.0f, 0.0f, 0.0f), raw1 = mk(0.0f, 0.0f, 0.0f);
             v3 n = mk(0.0f, 0.0f, 0.0f);
             if (has0) {
-                raw0 = edge_face_normal_raw(V, s, 0);
+                raw0 = cross3(mk(pb.x - pa.x, pb.y - pa.y, pb.z - pa.z), mk(o0.x - pa.x, o0.y - pa.y, o0.z - pa.z));
                 n = mk(n.x + raw0.x, n.y + raw0.y, n.z + raw0.z);
             }
             if (has1) {
-                raw1 = edge_face_normal_raw(V, s, 1);
+                raw1 = cross3(mk(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z), mk(o1.x - pb.x, o1.y - pb.y, o1.z - pb.z));
                 n = mk(n.x + raw1.x, n.y + raw1.y, n.z + raw1.z);
             }
             n = normalize3(n);
-            const v3 pa = ld3(V, s.y), pb = ld3(V, s.z);
             float ra, rb = 0.0f;
             if (conservative) {
                 ra = len3(mk(pa.x - co.x, pa.y - co.y, pa.z - co.z));
