@@ -951,6 +951,7 @@ int orc_cone_overlap(const float *cone, const float *o, const float *box, float 
 void orc_math(int fn, const float *x, const float *y, float *out, int32_t n)
 {
     int32_t i;
+#pragma omp parallel for schedule(static)
     for (i = 0; i < n; ++i) {
         switch (fn) {
         case 0: out[i] = odm_acosf(x[i]); break;

@@ -404,17 +404,19 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
                 r.axis.x /= an;
                 r.axis.y /= an;
                 r.axis.z /= an;
-                r.half_angle = 0.0f;
+                // the largest angle to a face normal: dm::acosf_ never
+                // increases over [-1, 1] (checked for every float), so the
+                // maximum of the angles is the angle of the smallest cosine,
+                // bit for bit, for one arccos instead of up to six
+                float cmin = 1.0f;
 #pragma unroll
                 for (uint32_t k = 0; k < 3; ++k) {
                     if (k >= my)
                         break;
-                    for (int jj = 0; jj < 2; ++jj) {
-                        const v3 nv = nn[k][jj];
-                        const float ang = dm::acosf_(clamp11(dot3(r.axis, nv)));
-                        r.half_angle = smax(r.half_angle, ang);
-                    }
+                    for (int jj = 0; jj < 2; ++jj)
+                        cmin = smin(cmin, clamp11(dot3(r.axis, nn[k][jj])));
                 }
+                r.half_angle = smax(0.0f, dm::acosf_(cmin));
             }
         }
         leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, b.lo.z, 0.0f);
@@ -501,13 +503,16 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
             if (an > F_EPS) {
                 ax /= an;
                 ay /= an;
-                ha = 0.0f;
+                // one arccos of the smallest cosine (dm::acosf_ never
+                // increases over [-1, 1]: the same bits as the largest angle)
+                float cmin = 1.0f;
                 for (uint32_t k = 0; k < 2; ++k) {
                     if (e0 + k >= e1)
                         break;
                     for (int jj = 0; jj < 2; ++jj)
-                        ha = smax(ha, dm::acosf_(clamp11(dot2(ax, ay, nn[k][jj][0], nn[k][jj][1]))));
+                        cmin = smin(cmin, clamp11(dot2(ax, ay, nn[k][jj][0], nn[k][jj][1])));
                 }
+                ha = smax(0.0f, dm::acosf_(cmin));
             }
         }
         leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, 0.0f, 0.0f);

@@ -19,6 +19,34 @@ def test_detmath_unary_error_bound(oracle, fn, lo, hi):
     assert np.all((err <= 3 * ulp) | (err <= 2e-7 * 0.1)), (fn, float(np.max(err / ulp)))
 
 
+def test_detmath_acos_never_increases(oracle):
+    """Every float in [-1, 1], in increasing order: acos never increases.  The
+    leaf-cone kernels rely on it to take one arccos of the smallest cosine
+    instead of the largest of several arccosines (the same bits)."""
+    step = 1 << 25
+
+    def chunks():  # bit patterns in increasing value order
+        s = 0xBF800000  # -1 .. -0: patterns going down to 0x80000000
+        while s >= 0x80000000:
+            e = max(s - step, 0x80000000 - 1)
+            yield np.ascontiguousarray(np.arange(e + 1, s + 1, dtype=np.uint32)[::-1])
+            s = e
+        s = 0  # +0 .. 1
+        while s <= 0x3F800000:
+            e = min(s + step, 0x3F800000 + 1)
+            yield np.arange(s, e, dtype=np.uint32)
+            s = e
+
+    prev, n = np.float32(np.inf), 0
+    for u in chunks():
+        y = oracle.math(0, u.view(np.float32))
+        assert y[0] <= prev, hex(int(u[0]))
+        up = y[1:] > y[:-1]
+        assert not up.any(), hex(int(u[1:][up][0]))
+        prev, n = y[-1], n + u.size
+    assert n == 2 * (0x3F800000 + 1)  # -0 and +0 both
+
+
 def test_detmath_atan2_error_bound(oracle):
     rng = np.random.default_rng(0)
     y = rng.normal(size=300000).astype(np.float32)
