@@ -776,13 +776,14 @@ __device__ __forceinline__ cone3 ldcone(const float4* nc, uint32_t n)
 // Bottom-up refit in two launches, with no device-wide fence.
 //
 // k_karras lists the maximal subtrees of at most RT leaves ("cuts"); they are
-// disjoint and cover every leaf of every mesh.  k_refit gives each cut to one
-// workgroup, which keeps the topology of the cut and the box and cone of its
-// live nodes in LDS.  It merges level by level: the leaves arrive at their
-// parents (an LDS arrival counter per internal node), the second arrival
-// queues the parent, and each round merges the queued nodes with consecutive
-// threads, so the merge math runs on full wavefronts; a barrier separates the
-// rounds.  k_refit_top does the same for the nodes above the cuts ("tops",
+// disjoint and cover every leaf of every mesh.  k_refit's workgroups take the
+// cuts in turn (an atomic counter); a workgroup keeps the topology of its cut
+// and the box and cone of its live nodes in LDS.  It merges level by level:
+// nodes with two leaf children are ready at once, one with one leaf child
+// starts with one arrival (an LDS arrival bit per internal node), the second
+// arrival queues the parent, and each round merges the queued nodes with
+// consecutive threads, so the merge math runs on full wavefronts; a barrier
+// separates the rounds.  k_refit_top does the same for the nodes above the cuts ("tops",
 // about 2N/RT of them, listed by k_karras) in one workgroup, with the boxes
 // in LDS when they fit and in memory otherwise (and, for very large scenes,
 // a climb from the cut roots with arrival counters in memory).  All of it
