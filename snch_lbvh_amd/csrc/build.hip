@@ -1,7 +1,8 @@
 // LBVH construction for triangle scenes on gfx950 (replaces bvh::construct,
 // bvh.cuh:380-613, and the per-object getters of scene.cuh:870-961).
 //
-//   k_init_scene_box  scene-box accumulators to (+inf, -inf)
+//   k_init_scene_box  scene-box accumulators to (+inf, -inf), list and sort
+//                     counters to zero
 //   k_leafgen         one thread per triangle: leaf AABB (vertex extent
 //                     +- FLT_EPSILON), leaf normal cone from the owned
 //                     silhouette edges (the edges' normals computed one
@@ -77,10 +78,15 @@ __device__ __forceinline__ uint32_t mesh_search(const MeshDesc* __restrict__ d, 
 }
 
 // scene-box accumulators (+inf, -inf) of every mesh, 8 ints per mesh
-// (and clears the two list counters k_morton and k_karras append to)
-__global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32_t* list_b)
+// (and clears the two list counters k_morton and k_karras append to, and
+// the radix sort's digit totals and tile counters)
+__global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32_t* list_b, uint32_t* sort_words,
+                                 uint32_t n_sort_words)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) // the radix sort's digit totals and tile counters
+        for (uint32_t k = threadIdx.x; k < n_sort_words; k += blockDim.x)
+            sort_words[k] = 0u;
     if (t == 0) {
         list_a[0] = 0u;
         list_a[1] = 0u; // k_refit's next cut
@@ -1355,9 +1361,15 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(s->nbox.ensure(sizeof(float4) * 2 * NN));
     SNCH_TRY(s->ncone.ensure(sizeof(float4) * 2 * NN));
 
+    // stable sort by (mesh, code, index): code passes, then mesh-id passes;
+    // k_init_scene_box clears the sort's counters
+    int mesh_bits = 0;
+    while (mesh_bits < 32 && ((M - 1) >> mesh_bits) != 0)
+        ++mesh_bits;
+    SNCH_TRY(w.hist.ensure(radix_sort_scratch_bytes(N, 4 + (pmesh ? (mesh_bits + 7) / 8 : 0))));
     mark(0);
     hipLaunchKernelGGL(k_init_scene_box, dim3((8 * M + 255) / 256), dim3(256), 0, st, w.scene_bits.as<int>(), M,
-                       cuts + cut_cap, tops + top_cap);
+                       cuts + cut_cap, tops + top_cap, w.hist.as<uint32_t>(), (uint32_t)radix_sort_clear_words());
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
@@ -1380,13 +1392,10 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                            w.parent.as<uint32_t>(), cuts, cut_cap);
     }
     mark(1);
-    // stable sort by (mesh, code, index): code passes, then mesh-id passes
-    int mesh_bits = 0;
-    while (mesh_bits < 32 && ((M - 1) >> mesh_bits) != 0)
-        ++mesh_bits;
     bool in_alt = false;
     SNCH_TRY(radix_sort_pairs(w.codes.as<uint32_t>(), w.idx.as<uint32_t>(), w.codes_alt.as<uint32_t>(),
-                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt, pmesh, pmesh ? mesh_bits : 0));
+                              w.idx_alt.as<uint32_t>(), N, 30, w.hist, st, &in_alt, pmesh, pmesh ? mesh_bits : 0,
+                              true));
     const uint32_t* codes = in_alt ? w.codes_alt.as<uint32_t>() : w.codes.as<uint32_t>();
     const uint32_t* sidx = in_alt ? w.idx_alt.as<uint32_t>() : w.idx.as<uint32_t>();
     mark(2);

@@ -168,12 +168,16 @@ hipError_t edge_table_device(snch_scene* s, hipStream_t st);
 hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStream_t st,
                       hipEvent_t* phase = nullptr);
 // sort.hip: stable LSD radix sort of (key, value) pairs on the low `bits` bits
-// of the key, then (seg != null) on the low `seg_bits` bits of seg[value]
+// of the key, then (seg != null) on the low `seg_bits` bits of seg[value].
+// pre_cleared: the caller has sized `hist` (radix_sort_scratch_bytes) and
+// zeroed its first radix_sort_clear_words() words on `st` already
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt,
-                            const uint32_t* seg = nullptr, int seg_bits = 0);
+                            const uint32_t* seg = nullptr, int seg_bits = 0, bool pre_cleared = false);
 // bytes of `hist` a sort of n pairs over `passes` 8-bit digits uses
 size_t radix_sort_scratch_bytes(uint32_t n, int passes);
+// leading words of `hist` a sort expects to start at zero
+size_t radix_sort_clear_words();
 // query.hip (dim = 2 or 3).  perm (nullable): slot -> query index, the
 // order in which queries are visited (query_order computes a Morton order)
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,

@@ -270,9 +270,11 @@ size_t radix_sort_scratch_bytes(uint32_t n, int passes)
     return sizeof(uint32_t) * ((size_t)MAX_PASSES * RADIX + 2 * MAX_PASSES + (size_t)passes * ntiles * RADIX);
 }
 
+size_t radix_sort_clear_words() { return (size_t)MAX_PASSES * RADIX + 2 * MAX_PASSES; }
+
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt, const uint32_t* seg,
-                            int seg_bits)
+                            int seg_bits, bool pre_cleared)
 {
     *result_in_alt = false;
     if (n <= 1)
@@ -291,7 +293,8 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     uint32_t* tile_ctr = ghist + MAX_PASSES * RADIX;
     uint32_t* status = tile_ctr + 2 * MAX_PASSES;
     uint32_t* err = traversal_error_word() + 1; // sticky; read by check_device_errors
-    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
+    if (!pre_cleared)
+        hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
     hipLaunchKernelGGL(k_hist_all, dim3(ntiles), dim3(ST), 0, st, keys, vals, seg, n, key_passes, seg_passes, ntiles,
                        ghist, status);
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
