@@ -376,8 +376,8 @@ def build_phase_roofline(sc, n_prims, reps):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(queries shard without communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path "
+              "(query batches would shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "silhouette":
         {"small": workload_small, "batched": workload_batched, "wost": workload_wost,
