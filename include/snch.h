@@ -262,6 +262,21 @@ int snch_wost_destroy(snch_wost *wost);
 int snch_wost_solve(snch_wost *wost, const float *x0, uint32_t n, uint32_t max_steps, float eps, uint64_t seed,
                     int use_graph, float *out, uint32_t *live_after, void *stream);
 
+/* ---- the closest-silhouette query without device allocation -------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, results and visiting order as
+ * snch_closest_silhouette (Morton order from SNCH_ORDER_MIN_QUERIES queries
+ * up), but the ordering scratch lives in a static device array of each GPU
+ * (about 71 MB, sized for SNCH_NOALLOC_MAX_QUERIES queries, held until the
+ * process ends) instead of per-scene buffers grown with hipMalloc; n above
+ * SNCH_NOALLOC_MAX_QUERIES is rejected with SNCH_ERR_INVALID.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define SNCH_NOALLOC_MAX_QUERIES (1u << 22)
+int snch_closest_silhouette_noalloc(const snch_scene *scene, const float *q, uint32_t n, int flip,
+                                    const float *seed_radius, uint32_t *out_sil, float *out_dist, void *stream);
+
 /* ---- device memory helpers for callers without their own allocator ------- */
 int snch_device_malloc(void **ptr, uint64_t bytes);
 int snch_device_free(void *ptr);

@@ -71,7 +71,7 @@ EXPORTED = [
     "snch_batch_closest_point",
     "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh",
     "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
-    "snch_closest_silhouette_ex", "snch_ray", "snch_ray_ex",
+    "snch_closest_silhouette_ex", "snch_closest_silhouette_noalloc", "snch_ray", "snch_ray_ex",
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
     "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
@@ -102,6 +102,7 @@ def _declare(L):
     L.snch_closest_point.argtypes = [vp, vp, u32, vp, vp, vp]
     L.snch_closest_silhouette.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp]
     L.snch_closest_silhouette_ex.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp, vp]
+    L.snch_closest_silhouette_noalloc.argtypes = [vp, vp, u32, c_int, vp, vp, vp, vp]
     L.snch_closest_point_ex.argtypes = [vp, vp, u32, vp, vp, vp, vp]
     L.snch_ray_ex.argtypes = [vp, vp, vp, vp, u32, c_int, vp, vp, vp, vp, vp, vp]
     L.snch_ray.argtypes = [vp, vp, vp, vp, u32, c_int, vp, vp, vp, vp, vp]

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsnch.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ["capi.hip", "build.hip", "sort.hip", "edges.hip", "query.hip", "mlp.hip", "wost.hip"]
+SOURCES = ["capi.hip", "build.hip", "sort.hip", "edges.hip", "query.hip", "mlp.hip", "wost.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SNCH_OFFLOAD_ARCH", "gfx950")

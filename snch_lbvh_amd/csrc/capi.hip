@@ -207,6 +207,14 @@ int snch_context_create(int device, snch_context** out)
     }
     auto* c = new snch_context();
     c->device = device;
+    c->err_word = traversal_error_word();
+    {
+        const hipError_t e = noalloc_init(device, &c->noalloc_scratch);
+        if (e != hipSuccess) {
+            delete c;
+            return hip_fail(e, "snch_context_create: static scratch");
+        }
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess)
         c->n_cu = prop.multiProcessorCount;

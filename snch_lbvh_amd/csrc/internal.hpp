@@ -111,6 +111,8 @@ struct BuildScratch {
 struct snch_context {
     int device = 0;
     int n_cu = 0;
+    uint32_t* err_word = nullptr;        // the device's sticky error words (traversal_error_word)
+    uint32_t* noalloc_scratch = nullptr; // device address of the static scratch array (noalloc.hip)
 };
 
 struct snch_scene {
@@ -174,6 +176,12 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
                             int bits, DeviceBuffer& hist, hipStream_t st, bool* result_in_alt,
                             const uint32_t* seg = nullptr, int seg_bits = 0, bool pre_cleared = false);
+// The same on caller-provided memory: `hist` holds radix_sort_scratch_bytes()
+// bytes; `err` is the sticky word the sort's bounded waits report to.  Launches
+// kernels only (no allocation, no memset).
+hipError_t radix_sort_pairs_in(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                               int bits, uint32_t* hist, hipStream_t st, bool* result_in_alt, uint32_t* err,
+                               const uint32_t* seg = nullptr, int seg_bits = 0, bool pre_cleared = false);
 // bytes of `hist` a sort of n pairs over `passes` 8-bit digits uses
 size_t radix_sort_scratch_bytes(uint32_t n, int passes);
 // leading words of `hist` a sort expects to start at zero
@@ -182,6 +190,11 @@ size_t radix_sort_clear_words();
 // order in which queries are visited (query_order computes a Morton order)
 hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, QueryScratch& w, hipStream_t st,
                        const uint32_t** perm);
+// query_order on caller memory (query_order_scratch_bytes(n) bytes at
+// `words`); err: the sort's sticky wait word.  Launches kernels only.
+size_t query_order_scratch_bytes(uint32_t n);
+hipError_t query_order_in(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* words, hipStream_t st,
+                          const uint32_t** perm, uint32_t* err);
 hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* idx, float* dist,
                                 uint32_t block, uint32_t depth, const uint32_t* perm, uint64_t* stats,
                                 hipStream_t st);
@@ -189,9 +202,12 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
 // it are not searched; the returned distance is then larger than the
 // radius, possibly +inf).  Otherwise a hint: queries not settled within it
 // continue without it; results equal the unseeded search.
+// err (nullable): the sticky overflow word to report to (default: the current
+// device's, traversal_error_word())
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
                                      const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
-                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st);
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st,
+                                     uint32_t* err = nullptr);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);
@@ -204,6 +220,9 @@ hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, u
 hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
                         uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st);
 uint32_t* traversal_error_word(); // [0] traversal overflow, [1] sort wait bound (capi.hip)
+// noalloc.hip: the device address of the static scratch array on `device`
+// (current), resolved once per device
+hipError_t noalloc_init(int device, uint32_t** out);
 // SNCH_OK, or the error code and message of a sticky device error (cleared)
 int check_device_errors(const char* fn);
 } // namespace snch

@@ -1147,6 +1147,29 @@ hipError_t query_order(const TreeView& t, int dim, const float* q, uint32_t n, Q
     return hipGetLastError();
 }
 
+size_t query_order_scratch_bytes(uint32_t n)
+{
+    return sizeof(uint32_t) * 4 * (size_t)n + radix_sort_scratch_bytes(n, 4);
+}
+
+hipError_t query_order_in(const TreeView& t, int dim, const float* q, uint32_t n, uint32_t* words, hipStream_t st,
+                          const uint32_t** perm, uint32_t* err)
+{
+    *perm = nullptr;
+    if (n < 2)
+        return hipSuccess;
+    uint32_t *codes = words, *idx = words + n, *codes_alt = words + 2 * (size_t)n, *idx_alt = words + 3 * (size_t)n;
+    SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_query_codes<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, t.nbox,
+                                              q, n, codes, idx));
+    bool in_alt = false;
+    const hipError_t e = radix_sort_pairs_in(codes, idx, codes_alt, idx_alt, n, 30, words + 4 * (size_t)n, st,
+                                             &in_alt, err);
+    if (e != hipSuccess)
+        return e;
+    *perm = in_alt ? idx_alt : idx;
+    return hipGetLastError();
+}
+
 template <int BS, int DIM, int SSZ>
 static void launch_cp_cfg(const TreeView& t, const float* q, uint32_t n, uint32_t* idx, float* dist,
                           const uint32_t* perm, unsigned long long* stats, hipStream_t st)
@@ -1189,10 +1212,9 @@ hipError_t launch_closest_point(const TreeView& t, int dim, const float* q, uint
 template <int BS, int DIM, int SSZ>
 static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
                            uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
-                           hipStream_t st)
+                           uint32_t* err, hipStream_t st)
 {
     const dim3 g(nblocks(n, BS)), b(BS);
-    uint32_t* err = traversal_error_word();
 #define SNCH_SIL(S_, M_, ...) \
     hipLaunchKernelGGL((k_closest_silhouette<BS, S_, DIM, M_, SSZ>), g, b, 0, st, __VA_ARGS__)
     if (!seed) {
@@ -1217,31 +1239,34 @@ static void launch_sil_cfg(const TreeView& t, const float* q, uint32_t n, int fl
 template <int BS, int DIM>
 static void launch_sil_bs(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed, bool capped,
                           uint32_t* sil, float* dist, uint32_t depth, const uint32_t* perm,
-                          unsigned long long* stats, hipStream_t st)
+                          unsigned long long* stats, uint32_t* err, hipStream_t st)
 {
     if (DIM == 2 || depth == 16) // scene<2> uses the default depth only
-        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 16>(t, q, n, flip, seed, capped, sil, dist, perm, stats, err, st);
     else if (depth == 8)
-        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 8>(t, q, n, flip, seed, capped, sil, dist, perm, stats, err, st);
     else if (depth == 24)
-        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 24>(t, q, n, flip, seed, capped, sil, dist, perm, stats, err, st);
     else
-        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, sil, dist, perm, stats, st);
+        launch_sil_cfg<BS, DIM, 32>(t, q, n, flip, seed, capped, sil, dist, perm, stats, err, st);
 }
 
 hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
                                      const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
-                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st)
+                                     uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st,
+                                     uint32_t* err)
 {
     if (n == 0)
         return hipSuccess;
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
+    if (!err)
+        err = traversal_error_word();
     SNCH_DISPATCH_DIM(dim, {
         switch (block) {
-        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
-        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, st); break;
+        case 128: launch_sil_bs<128, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
+        case 256: launch_sil_bs<256, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
+        case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
+        default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
         }
     });
     return hipGetLastError();

@@ -280,19 +280,34 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     if (n <= 1)
         return hipSuccess;
     const int key_passes = (bits + 7) / 8, seg_passes = seg ? (seg_bits + 7) / 8 : 0;
-    const int passes = key_passes + seg_passes;
     if (key_passes > 4 || seg_passes > 4 || n > LB_COUNT)
+        return hipErrorInvalidValue;
+    if (key_passes + seg_passes == 0)
+        return hipSuccess;
+    hipError_t e = hist.ensure(radix_sort_scratch_bytes(n, key_passes + seg_passes));
+    if (e != hipSuccess)
+        return e;
+    return radix_sort_pairs_in(keys, vals, keys_alt, vals_alt, n, bits, hist.as<uint32_t>(), st, result_in_alt,
+                               traversal_error_word() + 1, seg, seg_bits, pre_cleared);
+}
+
+hipError_t radix_sort_pairs_in(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                               int bits, uint32_t* hist, hipStream_t st, bool* result_in_alt, uint32_t* err,
+                               const uint32_t* seg, int seg_bits, bool pre_cleared)
+{
+    *result_in_alt = false;
+    if (n <= 1)
+        return hipSuccess;
+    const int key_passes = (bits + 7) / 8, seg_passes = seg ? (seg_bits + 7) / 8 : 0;
+    const int passes = key_passes + seg_passes;
+    if (key_passes > 4 || seg_passes > 4 || n > LB_COUNT || !hist || !err)
         return hipErrorInvalidValue;
     if (passes == 0)
         return hipSuccess;
     const uint32_t ntiles = (n + TILE - 1) / TILE;
-    hipError_t e = hist.ensure(radix_sort_scratch_bytes(n, passes));
-    if (e != hipSuccess)
-        return e;
-    uint32_t* ghist = hist.as<uint32_t>();
+    uint32_t* ghist = hist;
     uint32_t* tile_ctr = ghist + MAX_PASSES * RADIX;
     uint32_t* status = tile_ctr + 2 * MAX_PASSES;
-    uint32_t* err = traversal_error_word() + 1; // sticky; read by check_device_errors
     if (!pre_cleared)
         hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
     hipLaunchKernelGGL(k_hist_all, dim3(ntiles), dim3(ST), 0, st, keys, vals, seg, n, key_passes, seg_passes, ntiles,
