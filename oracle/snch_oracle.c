@@ -1005,7 +1005,13 @@ static float tri_dist(const orc_tree *t, uint32_t obj, f3 q)
 }
 
 /* query.cuh:238-318; counts[0] += child box tests, counts[1] += leaf evaluations */
-static void cp_one(const orc_tree *t, f3 q, uint32_t *idx_out, float *dist_out, uint64_t *counts)
+/* tie_min = 0: among exactly tied triangles the last visited wins, as in
+ * query.cuh:286 (d <= best); tie_min = 1: the smallest index wins, the
+ * library's rule (used by the walk-on-stars restatement, whose walks then
+ * match the library's bit for bit).  Every tied candidate is visited either
+ * way: nodes at distance == best are not pruned. */
+static void cp_one(const orc_tree *t, f3 q, uint32_t *idx_out, float *dist_out, uint64_t *counts, int tie_min)
+
 {
     uint32_t sn[128];
     float sd[128];
@@ -1046,7 +1052,7 @@ static void cp_one(const orc_tree *t, f3 q, uint32_t *idx_out, float *dist_out, 
                 float d = tri_dist(t, o, q);
                 if (counts) counts[1] += 1;
                 d *= d;
-                if (d <= best) { best = d; nearest = o; }
+                if (tie_min ? (d < best || (d == best && o < nearest)) : d <= best) { best = d; nearest = o; }
             } else {
                 sn[sp] = L; sd[sp] = Lmd; ++sp;
             }
@@ -1057,7 +1063,7 @@ static void cp_one(const orc_tree *t, f3 q, uint32_t *idx_out, float *dist_out, 
                 float d = tri_dist(t, o, q);
                 if (counts) counts[1] += 1;
                 d *= d;
-                if (d <= best) { best = d; nearest = o; }
+                if (tie_min ? (d < best || (d == best && o < nearest)) : d <= best) { best = d; nearest = o; }
             } else {
                 sn[sp] = R; sd[sp] = Rmd; ++sp;
             }
@@ -1153,8 +1159,11 @@ static void sil_one(const orc_tree *t, f3 q, int flip, uint32_t *edge_out, float
 }
 
 /* query.cuh:79-169 (closest hit; any_hit = TestOnly) */
+/* tie_min: as for cp_one; the reference keeps the first hit found among
+ * equal t (query.cuh:115, th < min_dist) */
 static void ray_one(const orc_tree *t, f3 o, f3 d, float max_dist, int any_hit, uint32_t *idx_out, float *t_out,
-                    float *uv_out, uint8_t *hit_out, uint64_t *counts)
+                    float *uv_out, uint8_t *hit_out, uint64_t *counts, int tie_min)
+
 {
     uint32_t sn[128];
     float sd[128];
@@ -1178,7 +1187,7 @@ static void ray_one(const orc_tree *t, f3 o, f3 d, float max_dist, int any_hit, 
             float th, uh, vh;
             if (counts) counts[1] += 1;
             if (ray_triangle(o, d, vget(t->V, f[0]), vget(t->V, f[1]), vget(t->V, f[2]), &th, &uh, &vh) &&
-                th < max_dist && th < min_dist) {
+                th < max_dist && (th < min_dist || (tie_min && th == min_dist && obj < nearest))) {
                 min_dist = th;
                 found = 1;
                 if (any_hit) break;
@@ -1356,7 +1365,7 @@ void orc_wost(const orc_tree *D, const orc_tree *N, const float *x0, int64_t n, 
             float dD, dS = INFINITY, R, u1, u2, z, s, sp, cp, rt, uv[2];
             uint8_t hit = 0;
             f3 d, o;
-            cp_one(D, p, &cidx, &dD, NULL);
+            cp_one(D, p, &cidx, &dD, NULL, 1);
             if (N) sil_one(N, p, 0, &sidx, &dS, NULL);
             if (dD < eps) {
                 value = wost_g(D, cidx, p);
@@ -1375,7 +1384,7 @@ void orc_wost(const orc_tree *D, const orc_tree *N, const float *x0, int64_t n, 
                 if (dn < 0.0f) d = mk(d.x - 2.0f * dn * nrm.x, d.y - 2.0f * dn * nrm.y, d.z - 2.0f * dn * nrm.z);
                 o = mk(p.x + 1e-5f * nrm.x, p.y + 1e-5f * nrm.y, p.z + 1e-5f * nrm.z);
             }
-            if (N) ray_one(N, o, d, R, 0, &ridx, &rt, uv, &hit, NULL);
+            if (N) ray_one(N, o, d, R, 0, &ridx, &rt, uv, &hit, NULL, 1);
             if (N && hit && rt < R) {
                 f3 a = tri_vert(N, ridx, 0), b = tri_vert(N, ridx, 1), c = tri_vert(N, ridx, 2);
                 f3 nn = normalize3(cross3(sub(b, a), sub(c, a)));
@@ -1416,7 +1425,7 @@ void orc_closest_point(const orc_tree *t, const float *q, int64_t n, uint32_t *i
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : c0, c1)
     for (i = 0; i < n; ++i) {
         uint64_t c[3] = {0, 0, 0};
-        cp_one(t, mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]), idx + i, dist + i, counts ? c : NULL);
+        cp_one(t, mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]), idx + i, dist + i, counts ? c : NULL, 0);
         c0 += c[0];
         c1 += c[1];
     }
@@ -1450,7 +1459,7 @@ void orc_ray(const orc_tree *t, const float *o, const float *d, const float *tma
     for (i = 0; i < n; ++i) {
         uint64_t c[3] = {0, 0, 0};
         ray_one(t, mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]), mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]), tmax[i],
-                any_hit, idx + i, tout + i, uv + 2 * i, hit + i, counts ? c : NULL);
+                any_hit, idx + i, tout + i, uv + 2 * i, hit + i, counts ? c : NULL, 0);
         c0 += c[0];
         c1 += c[1];
     }

@@ -169,6 +169,25 @@ def _stats_ptr(stats):
     return stats.data_ptr()
 
 
+def _need_rows(t, n, name):
+    """Companion arrays must hold one row per query: the kernels read row i of
+    every input for query i."""
+    if t is not None and t.shape[0] != n:
+        raise ValueError(f"{name} has {t.shape[0]} rows for {n} queries")
+
+
+def _check_qoff(q_off, n_meshes, n):
+    """q_off: n_meshes + 1 non-decreasing offsets from 0 to the number of
+    queries (the library reads n_meshes + 1 host entries and writes outputs
+    for queries [0, q_off[-1]))."""
+    if q_off.ndim != 1 or q_off.shape[0] != n_meshes + 1:
+        raise ValueError(f"q_off must hold n_meshes + 1 = {n_meshes + 1} offsets, got {q_off.shape}")
+    if q_off[0] != 0 or np.any(np.diff(q_off.astype(np.int64)) < 0):
+        raise ValueError("q_off must start at 0 and must not decrease")
+    if int(q_off[-1]) != n:
+        raise ValueError(f"q_off[-1] = {int(q_off[-1])} but there are {n} queries")
+
+
 def _order_ptr(order, n):
     if order is None:
         return None
@@ -264,6 +283,7 @@ class Scene:
         q = _dev_f32(q, self.dim, self.device)
         n = q.shape[0]
         seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
+        _need_rows(seed, n, "seed_radius")
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
         opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0,
@@ -279,6 +299,8 @@ class Scene:
         d = _dev_f32(d, self.dim, self.device)
         n = o.shape[0]
         tm = _dev_f32(tmax, 0, self.device).reshape(-1) if tmax is not None else None
+        _need_rows(d, n, "d")
+        _need_rows(tm, n, "tmax")
         hit = torch.empty(n, dtype=torch.uint8, device=o.device)
         t = torch.empty(n, dtype=torch.float32, device=o.device)
         uv = torch.empty((n, 2) if self.dim == 3 else (n,), dtype=torch.float32, device=o.device)
@@ -295,6 +317,8 @@ class Scene:
         n = c.shape[0]
         r = _dev_f32(r, 0, self.device).reshape(-1)
         u = _dev_f32(u, 3, self.device)
+        _need_rows(r, n, "r")
+        _need_rows(u, n, "u")
         idx = torch.empty(n, dtype=torch.int32, device=c.device)
         pdf = torch.empty(n, dtype=torch.float32, device=c.device)
         pt = torch.empty((n, self.dim), dtype=torch.float32, device=c.device) if with_point else None
@@ -307,6 +331,7 @@ class Scene:
 
         idx = torch.as_tensor(idx, dtype=torch.int32, device=f"cuda:{self.device}").contiguous()
         uv = _dev_f32(uv, 2, self.device)
+        _need_rows(uv, idx.shape[0], "uv")
         pt = torch.empty((idx.shape[0], self.dim), dtype=torch.float32, device=idx.device)
         check(lib().snch_sample_on_object(self._h, _ptr(idx), _ptr(uv), idx.shape[0], _ptr(pt), _stream()))
         return pt
@@ -402,6 +427,7 @@ class BatchScene:
         q = _dev_f32(q, 3, self.device)
         q_off = np.ascontiguousarray(q_off, np.uint32)
         n = q.shape[0]
+        _check_qoff(q_off, self.n_meshes, n)
         idx = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
         opts = self._opts(tune, coherent)
@@ -415,6 +441,7 @@ class BatchScene:
         q = _dev_f32(q, 3, self.device)
         q_off = np.ascontiguousarray(q_off, np.uint32)
         n = q.shape[0]
+        _check_qoff(q_off, self.n_meshes, n)
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
         opts = self._opts(tune, coherent)
@@ -427,6 +454,7 @@ class BatchScene:
         rebuilds with the best leaf size and returns the choice as a dict."""
         q = _dev_f32(q, 3, self.device)
         q_off = np.ascontiguousarray(q_off, np.uint32)
+        _check_qoff(q_off, self.n_meshes, q.shape[0])
         r = TuneResult()
         check(lib().snch_batch_autotune(self._h, 1 if kind == "silhouette" else 0, _ptr(q), q_off.ctypes.data,
                                         sample_per_mesh, int(bool(flip)), ctypes.byref(r), _stream()))

@@ -146,6 +146,37 @@ static hipError_t upload_meshes(snch_scene* s)
     return hipSuccess;
 }
 
+// Per-handle scratch guard.  The query-ordering and batch scratch of a scene
+// handle is overwritten by every call; launches are asynchronous, so a call on
+// a stream other than the previous call's first makes its stream wait for the
+// previous call's launches (an event recorded after them).  Calls on one
+// stream are ordered by the stream itself.  Skipped while the stream is being
+// captured into a graph: a captured call uses one stream per handle.
+static hipError_t scratch_begin(const snch_scene* s, hipStream_t st)
+{
+    if (!s->scratch_ev || !s->scratch_used || s->scratch_last == st)
+        return hipSuccess;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipSuccess;
+    return hipStreamWaitEvent(st, s->scratch_ev, 0);
+}
+
+static hipError_t scratch_end(const snch_scene* s, hipStream_t st)
+{
+    if (!s->scratch_ev)
+        return hipSuccess;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipSuccess;
+    const hipError_t e = hipEventRecord(s->scratch_ev, st);
+    if (e == hipSuccess) {
+        s->scratch_last = st;
+        s->scratch_used = true;
+    }
+    return e;
+}
+
 } // namespace snch
 
 using namespace snch;
@@ -272,6 +303,10 @@ int snch_scene_create_tri_batch(snch_context* ctx, uint32_t n_meshes, const floa
     SNCH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto* s = new snch_scene();
     s->ctx = ctx;
+    if (hipEventCreateWithFlags(&s->scratch_ev, hipEventDisableTiming) != hipSuccess) {
+        delete s;
+        return hip_fail(hipGetLastError(), "snch_scene_create_tri_batch: event");
+    }
     s->dim = 3;
     s->n_meshes = n_meshes;
     s->nV = nV;
@@ -330,6 +365,10 @@ int snch_scene_create_seg(snch_context* ctx, const float* V, uint32_t nV, const 
     SNCH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto* s = new snch_scene();
     s->ctx = ctx;
+    if (hipEventCreateWithFlags(&s->scratch_ev, hipEventDisableTiming) != hipSuccess) {
+        delete s;
+        return hip_fail(hipGetLastError(), "snch_scene_create_seg: event");
+    }
     s->dim = 2;
     s->nV = nV;
     s->nF = nS;
@@ -396,6 +435,8 @@ snch_scene::~snch_scene()
 {
     if (build_exec)
         (void)hipGraphExecDestroy(build_exec);
+    if (scratch_ev)
+        (void)hipEventDestroy(scratch_ev);
 }
 
 namespace {
@@ -517,6 +558,7 @@ int snch_scene_build(snch_scene* s, const snch_build_opts* opts, void* stream)
     s->built = true;
     s->build_flags = flags;
     s->leaf_size = leaf;
+    ++s->build_gen;
     s->last_build_ms = ms;
     return SNCH_OK;
 }
@@ -567,12 +609,14 @@ int snch_query_order(const snch_scene* s, const float* q, uint32_t n, uint32_t* 
         return SNCH_OK;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = nullptr;
+    SNCH_HIP(scratch_begin(s, st), "snch_query_order: stream wait");
     SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_query_order");
     if (perm)
         SNCH_HIP(hipMemcpyAsync(out_perm, perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st),
                  "snch_query_order: copy");
     else
         SNCH_HIP(hipMemsetAsync(out_perm, 0, sizeof(uint32_t), st), "snch_query_order: copy");
+    SNCH_HIP(scratch_end(s, st), "snch_query_order: event");
     return SNCH_OK;
 }
 
@@ -582,17 +626,22 @@ int snch_closest_point_ex(const snch_scene* s, const float* q, uint32_t n, uint3
     SNCH_NEED_SINGLE(s, "snch_closest_point");
     SNCH_CHECK_ARG(n == 0 || (q && out_idx && out_dist), "snch_closest_point: null pointer");
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    const uint32_t* perm = opts ? opts->order : nullptr;
-    if (!perm && want_order(opts, n))
-        SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_closest_point: ordering");
     const uint32_t block = opts && opts->block_size ? opts->block_size : 64u;
     const uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 16u;
     SNCH_CHECK_ARG(block == 64 || block == 128 || block == 256 || block == 512,
                    "snch_closest_point: block_size must be 64, 128, 256 or 512");
     SNCH_CHECK_ARG(depth == 8 || depth == 16, "snch_closest_point: stack_depth must be 8 or 16");
+    const uint32_t* perm = opts ? opts->order : nullptr;
+    const bool own_order = !perm && want_order(opts, n);
+    if (own_order) {
+        SNCH_HIP(scratch_begin(s, st), "snch_closest_point: stream wait");
+        SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm), "snch_closest_point: ordering");
+    }
     SNCH_HIP(launch_closest_point(s->view(), (int)s->dim, q, n, out_idx, out_dist, block, depth, perm,
                                   opts ? opts->stats : nullptr, st),
              "snch_closest_point");
+    if (own_order)
+        SNCH_HIP(scratch_end(s, st), "snch_closest_point: event");
     return SNCH_OK;
 }
 
@@ -615,13 +664,18 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
                    "snch_closest_silhouette: stack_depth must be 8, 16, 24 or 32");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = opts ? opts->order : nullptr;
-    if (!perm && want_order(opts, n))
+    const bool own_order = !perm && want_order(opts, n);
+    if (own_order) {
+        SNCH_HIP(scratch_begin(s, st), "snch_closest_silhouette: stream wait");
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
+    }
     const bool capped = opts && opts->seed_is_cap;
     SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, out_sil, out_dist, block,
                                        depth, perm, opts ? opts->stats : nullptr, st),
              "snch_closest_silhouette");
+    if (own_order)
+        SNCH_HIP(scratch_end(s, st), "snch_closest_silhouette: event");
     return SNCH_OK;
 }
 
@@ -715,8 +769,10 @@ int snch_batch_closest_point(const snch_scene* s, const float* q, const uint32_t
     BatchConfig cfg;
     if ((rc = batch_config(opts, n, &cfg, "snch_batch_closest_point")) != SNCH_OK)
         return rc;
-    SNCH_HIP(launch_batch(s, 0, q, q_off, 0, out_idx, out_dist, cfg, static_cast<hipStream_t>(stream)),
-             "snch_batch_closest_point");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    SNCH_HIP(scratch_begin(s, st), "snch_batch_closest_point: stream wait");
+    SNCH_HIP(launch_batch(s, 0, q, q_off, 0, out_idx, out_dist, cfg, st), "snch_batch_closest_point");
+    SNCH_HIP(scratch_end(s, st), "snch_batch_closest_point: event");
     return SNCH_OK;
 }
 
@@ -733,8 +789,10 @@ int snch_batch_closest_silhouette(const snch_scene* s, const float* q, const uin
     BatchConfig cfg;
     if ((rc = batch_config(opts, n, &cfg, "snch_batch_closest_silhouette")) != SNCH_OK)
         return rc;
-    SNCH_HIP(launch_batch(s, 1, q, q_off, flip, out_sil, out_dist, cfg, static_cast<hipStream_t>(stream)),
-             "snch_batch_closest_silhouette");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    SNCH_HIP(scratch_begin(s, st), "snch_batch_closest_silhouette: stream wait");
+    SNCH_HIP(launch_batch(s, 1, q, q_off, flip, out_sil, out_dist, cfg, st), "snch_batch_closest_silhouette");
+    SNCH_HIP(scratch_end(s, st), "snch_batch_closest_silhouette: event");
     return SNCH_OK;
 }
 
@@ -748,6 +806,7 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
         return rc;
     SNCH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    SNCH_HIP(scratch_begin(s, st), "snch_batch_autotune: stream wait");
     const uint32_t M = s->n_meshes;
     // the tuning set: every query (sample_per_mesh = 0) or the first
     // sample_per_mesh queries of each mesh, gathered into one batch
@@ -852,6 +911,7 @@ int snch_batch_autotune(snch_scene* s, uint32_t kind, const float* q, const uint
     if (err != hipSuccess)
         return hip_fail(err, "snch_batch_autotune");
     s->leaf_size = cur.leaf_size;
+    ++s->build_gen;
     cur.ms = best_ms;
     *out = cur;
     return SNCH_OK;

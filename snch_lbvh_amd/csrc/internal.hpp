@@ -145,12 +145,19 @@ struct snch_scene {
     bool built = false;
     bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
     uint32_t build_flags = 0;
+    uint64_t build_gen = 0; // bumped by every build: graphs that hold this scene's TreeView re-capture
     float last_build_ms = 0.0f;
     float last_phase_ms[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     // repeat builds replay the launch sequence captured on the second build
     // with the same (flags, leaf size); every buffer it names is sized by then
     hipGraphExec_t build_exec = nullptr;
     uint32_t build_key = 0, eager_key = 0, eager_builds = 0;
+    // the per-handle query scratch (qscratch, batch) is reused by every call:
+    // a call on another stream than the last user waits for this event,
+    // recorded after the last user's launches
+    hipEvent_t scratch_ev = nullptr;
+    mutable hipStream_t scratch_last = nullptr;
+    mutable bool scratch_used = false;
     ~snch_scene();
     snch::TreeView view(uint32_t mesh = 0) const;
 };

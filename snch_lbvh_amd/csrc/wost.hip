@@ -56,7 +56,8 @@ struct Walks {
     float* sph_u;
     int32_t* smp_idx;
     float* smp_pdf;
-    float* star_r;    // R per compact slot (0 when the walk terminated)
+    float* star_r;    // R per compact slot (-inf when the walk terminated; R = 0 is a live walk
+                      // sitting on a silhouette, which stays put until max_steps)
 };
 
 // counter-based random numbers: splitmix64 of (seed, walk, step, k)
@@ -185,7 +186,7 @@ __global__ void k_star(Walks w, int cur, uint32_t cap, const float4* __restrict_
         const uint32_t t = w.cp_idx[i];
         const v3 c = closest_point_on_tri(tri_vertex(VD, FD, t, 0), tri_vertex(VD, FD, t, 1), tri_vertex(VD, FD, t, 2), p);
         w.value[wk] = c.x * c.y; // g(x) = x y
-        w.star_r[i] = 0.0f;
+        w.star_r[i] = -F_INF; // also an empty ray range: the ray query finds nothing
         return;
     }
     const float R = dD < dS ? dD : dS;
@@ -227,7 +228,7 @@ __global__ void k_update(Walks w, int cur, uint32_t cap, const float4* __restric
     if (i >= n)
         return;
     const float R = w.star_r[i];
-    if (!(R > 0.0f))
+    if (!(R >= 0.0f))
         return; // terminated in k_star
     const uint32_t wk = w.live[cur][i];
     const v3 o = mk(w.ray_o[3 * i], w.ray_o[3 * i + 1], w.ray_o[3 * i + 2]);
@@ -279,6 +280,9 @@ struct snch_wost {
     uint32_t graph_steps = 0;
     uint64_t graph_seed = 0;
     float graph_eps = 0.0f;
+    // build generations of the two scenes when the graph was captured: the
+    // graph holds their tree views (pointers, cone-test flags) by value
+    uint64_t graph_gen_d = 0, graph_gen_n = 0;
     ~snch_wost()
     {
         if (exec)
@@ -425,7 +429,9 @@ int snch_wost_solve(snch_wost* s, const float* x0, uint32_t n, uint32_t max_step
     hipError_t e;
     hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, st, s->w, x0, n);
     if (use_graph) {
-        if (!s->exec || s->graph_steps != max_steps || s->graph_eps != eps || s->graph_seed != seed) {
+        const uint64_t gen_d = s->dir->build_gen, gen_n = s->neu ? s->neu->build_gen : 0;
+        if (!s->exec || s->graph_steps != max_steps || s->graph_eps != eps || s->graph_seed != seed ||
+            s->graph_gen_d != gen_d || s->graph_gen_n != gen_n) {
             if (s->exec) {
                 (void)hipGraphExecDestroy(s->exec);
                 (void)hipGraphDestroy(s->graph);
@@ -451,6 +457,8 @@ int snch_wost_solve(snch_wost* s, const float* x0, uint32_t n, uint32_t max_step
             s->graph_steps = max_steps;
             s->graph_eps = eps;
             s->graph_seed = seed;
+            s->graph_gen_d = gen_d;
+            s->graph_gen_n = gen_n;
         }
         if ((e = hipGraphLaunch(s->exec, st)) != hipSuccess)
             return hip_fail(e, "snch_wost_solve: graph launch");

@@ -262,3 +262,41 @@ def batched_meshes(n_meshes=64, f_lo=1e3, f_hi=5e5, q_lo=2 ** 12, q_hi=2 ** 20, 
     q_off = np.zeros(n_meshes + 1, np.uint32)
     q_off[1:] = np.cumsum([len(x) for x in qs])
     return meshes, np.concatenate(qs).astype(np.float32), q_off
+
+
+def slab_box(n=16, h=0.25):
+    """The closed box [-1, 1]^2 x [-h, h] with outward-facing triangles, n x n
+    quads per face (two triangles each), split into its four side faces
+    (x = +-1, y = +-1) and its two caps (z = +-h).  Returns (V, F_sides,
+    F_caps); the two parts share the vertex array.  For Laplace with
+    Dirichlet data g = x y on the sides and zero Neumann flux on the caps the
+    exact solution is u = x y (its z-derivative vanishes)."""
+    verts, index = [], {}
+
+    def vid(p):
+        key = tuple(np.round(np.asarray(p, np.float64), 12))
+        if key not in index:
+            index[key] = len(verts)
+            verts.append(key)
+        return index[key]
+
+    def face(origin, eu, ev):
+        tris = []
+        g = np.linspace(0.0, 1.0, n + 1)
+        for a in range(n):
+            for b in range(n):
+                p00 = origin + g[a] * eu + g[b] * ev
+                p10 = origin + g[a + 1] * eu + g[b] * ev
+                p01 = origin + g[a] * eu + g[b + 1] * ev
+                p11 = origin + g[a + 1] * eu + g[b + 1] * ev
+                i00, i10, i01, i11 = vid(p00), vid(p10), vid(p01), vid(p11)
+                tris += [(i00, i10, i11), (i00, i11, i01)]
+        return tris
+
+    X, Y, Z = np.eye(3)
+    # each face spanned so that eu x ev points outward
+    sides = (face(np.array([1.0, -1.0, -h]), 2 * Y, 2 * h * Z) + face(np.array([-1.0, 1.0, -h]), -2 * Y, 2 * h * Z) +
+             face(np.array([1.0, 1.0, -h]), -2 * X, 2 * h * Z) + face(np.array([-1.0, -1.0, -h]), 2 * X, 2 * h * Z))
+    caps = face(np.array([-1.0, -1.0, h]), 2 * X, 2 * Y) + face(np.array([-1.0, 1.0, -h]), 2 * X, -2 * Y)
+    V = np.array(verts, np.float32)
+    return V, np.array(sides, np.int32), np.array(caps, np.int32)

@@ -136,3 +136,12 @@ def test_batch_errors(S):
         b.closest_point(q, q_off, tune={"queries_per_wave": 100})
     with pytest.raises(S.SnchError):
         S.BatchScene([meshes[0], (np.zeros((3, 3), np.float32), np.zeros((0, 3), np.int32))])
+    # q_off: n_meshes + 1 non-decreasing offsets ending at the query count
+    with pytest.raises(ValueError):
+        b.closest_point(q, q_off[:-1])
+    with pytest.raises(ValueError):
+        b.closest_silhouette(q, np.array([0, 15, 5], np.uint32))
+    with pytest.raises(ValueError):
+        b.closest_silhouette(q, np.array([0, 10, 30], np.uint32))
+    with pytest.raises(ValueError):
+        b.autotune(q, np.array([1, 10, 20], np.uint32))

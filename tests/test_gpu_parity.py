@@ -190,12 +190,16 @@ def test_closest_point_parity(S, oracle, name):
 
 @pytest.mark.parametrize("name", ["ico5", "torus", "soup", "one", "two", "three"])
 @pytest.mark.parametrize("flip", [False, True])
-def test_closest_silhouette_parity(S, oracle, name, flip):
+@pytest.mark.parametrize("reference_radius", [False, True])
+def test_closest_silhouette_parity(S, oracle, name, flip, reference_radius):
+    """reference_radius: the reference's midpoint cone radius, with which the
+    traversal uses the reference's acos-based cone test instead of the
+    trig-free one; GPU and oracle must still agree."""
     V, F = MESHES[name]()
     q = _queries(V, 20000, 3)
-    sc = S.Scene(V, F).build_bvh()
+    sc = S.Scene(V, F).build_bvh(reference_radius=reference_radius)
     ge, gd = (t.cpu().numpy() for t in sc.closest_silhouette(q, flip=flip))
-    o = oracle.build_tree(V, F)
+    o = oracle.build_tree(V, F, conservative_radius=not reference_radius)
     oe, od = oracle.closest_silhouette(o, q, flip=flip)
     f = lambda e, qq: oracle.edge_sil_dist(V, o.slots, e, qq, flip=flip)  # noqa: E731
     _check_ties(oracle, V, F, q, ge.view(np.uint32), oe, gd, od, f)
@@ -332,3 +336,49 @@ def test_error_paths(S):
     idx, d = sc.closest_point(np.zeros((0, 3), np.float32))
     assert idx.numel() == 0
     assert L.snch_closest_point(sc._h, None, 5, None, None, None) == 1
+
+
+def test_companion_row_counts(S):
+    """Every per-query input must hold one row per query; a short array is
+    rejected before any launch."""
+    V, F = MESHES["ico5"]()
+    sc = S.Scene(V, F).build_bvh()
+    q = _queries(V, 100, 2)
+    with pytest.raises(ValueError):
+        sc.closest_silhouette(q, seed_radius=np.ones(99, np.float32))
+    with pytest.raises(ValueError):
+        sc.ray_intersect(q, W.unit_directions(99, 5))
+    with pytest.raises(ValueError):
+        sc.ray_intersect(q, W.unit_directions(100, 5), tmax=np.ones(50, np.float32))
+    with pytest.raises(ValueError):
+        sc.sample_in_sphere(q, np.ones(99, np.float32), np.zeros((100, 3), np.float32))
+    with pytest.raises(ValueError):
+        sc.sample_in_sphere(q, np.ones(100, np.float32), np.zeros((10, 3), np.float32))
+    with pytest.raises(ValueError):
+        sc.sample_on_object(np.zeros(5, np.int32), np.zeros((4, 2), np.float32))
+
+
+def test_two_streams_share_scratch_safely(S):
+    """Ordered queries (which use the handle's ordering scratch) issued on two
+    streams back to back give the results of serial calls."""
+    import torch
+
+    V, F = W.cut_torus(256, 128)
+    sc = S.Scene(V, F).build_bvh()
+    qa = torch.from_numpy(_queries(V, 300000, 41)).cuda()
+    qb = torch.from_numpy(_queries(V, 250000, 42)).cuda()
+    ra = [t.clone() for t in sc.closest_silhouette(qa)]
+    rb = [t.clone() for t in sc.closest_silhouette(qb)]
+    ca = [t.clone() for t in sc.closest_point(qa)]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            ga = sc.closest_silhouette(qa)
+        with torch.cuda.stream(s2):
+            gb = sc.closest_silhouette(qb)
+        with torch.cuda.stream(s1):
+            gc = sc.closest_point(qa)
+        torch.cuda.synchronize()
+        for got, ref in zip(ga + gb + gc, ra + rb + ca):
+            assert torch.equal(got, ref)
