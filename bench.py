@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--nu", type=int, default=1024)
     ap.add_argument("--nv", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 13)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 15)
     ap.add_argument("--build-reps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--input-order", action="store_true", help="also time the kernel over unordered queries")
@@ -60,32 +60,102 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(V, F, q, sample, silhouette_ms_gpu):
-    """The CPU oracle (plain-C restatement, OpenMP) on a bounded sample of the
-    same queries, on this host's cores."""
+def host_cpu():
+    """This host's CPU model, logical CPU count and the CPUs this process may
+    run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "cpus_allowed": allowed}
+
+
+def omp_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the GPU host
+    sets it to this process's CPU share), else every CPU this process may use."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or host_cpu()["cpus_allowed"]
+
+
+def _median_rate(fn, n, reps):
+    fn(min(n, 256))  # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(n)
+        times.append(time.perf_counter() - t0)
+    return n / float(np.median(times)), times
+
+
+def cpu_baseline(V, F, q, sample, sample_1t=4096, reps=5):
+    """The CPU oracle (plain-C restatement of the reference algorithm, OpenMP)
+    on bounded samples of the same queries, on this host's cores: the median
+    of `reps` runs after a warm-up, with all allowed threads and with one.
+    Also returns the oracle's answers for the sample, for the agreement check."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = omp_threads()
     t0 = time.perf_counter()
     tree = oracle.build_tree(V, F)
     t_build = time.perf_counter() - t0
     qs = q[:sample]
-    oracle.closest_silhouette(tree, qs[:256], nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    oracle.closest_silhouette(tree, qs, nthreads=threads)
-    dt = time.perf_counter() - t0
+    rate, times = _median_rate(lambda m: oracle.closest_silhouette(tree, qs[:m], nthreads=threads), sample, reps)
+    rate1, _ = _median_rate(lambda m: oracle.closest_silhouette(tree, qs[:m], nthreads=1), sample_1t, 3)
+    oe, od = oracle.closest_silhouette(tree, qs, nthreads=threads)
+    info = host_cpu()
     return {
-        "value": sample / dt,
+        "value": rate,
         "unit": "queries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample} of the {len(q)} closest-silhouette queries (same seeds), oracle/snch_oracle.c "
-                  f"with OpenMP on {threads} threads; CPU build of the same tree took {t_build:.2f} s",
+        "sample": f"{sample} of the {len(q)} closest-silhouette queries (same seeds), median of {reps} runs after a "
+                  f"warm-up; oracle/snch_oracle.c with OpenMP on {threads} threads",
+        "one_thread_value": rate1,
+        "one_thread_sample": f"{sample_1t} queries, median of 3 runs",
+        "runs_s": times,
         "build_s": t_build,
-    }
+        **info,
+    }, tree, oe, od
+
+
+def agreement(tree, V, q_sample, oe, od, ge, gd):
+    """GPU answers against the oracle's on the same queries: distances bit for
+    bit; silhouette ids equal, or different ids at exactly the same FP32
+    distance (the library keeps the smallest id, the reference the last
+    visited)."""
+    import oracle
+
+    ge = ge.view(np.uint32)
+    dist_equal = bool(np.array_equal(gd, od))
+    diff = ge != oe
+    ties_ok = True
+    if diff.any():
+        a = oracle.edge_sil_dist(V, tree.slots, ge[diff], q_sample[diff])
+        b = oracle.edge_sil_dist(V, tree.slots, oe[diff], q_sample[diff])
+        ties_ok = bool(np.array_equal(a, b))
+    return {"queries": int(len(od)), "distances_bit_exact": dist_equal, "ids_equal": int((~diff).sum()),
+            "ids_tied": int(diff.sum()), "all_agree": dist_equal and ties_ok}
 
 
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_silhouette_torus1M.md")
+COUNTS_FILE = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
+
+
+def reference_counts():
+    """Per-query visit counts of the reference traversal order on this
+    workload (oracle/traversal_counts.py), or None."""
+    if not os.path.exists(COUNTS_FILE):
+        return None
+    with open(COUNTS_FILE) as f:
+        return json.load(f)
 
 
 def traffic_from_profile():
@@ -142,10 +212,6 @@ def base_line(metric, value, unit, args, ms_per_step, config, data, dtype="f32",
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
             "dtype": dtype, "data": data, "config": config}
-
-
-def omp_threads():
-    return int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
 
 
 def workload_small(args):
@@ -480,8 +546,22 @@ def main():
     box, leaf, edge, node, cone = (v / n for v in stats.tolist())
     per_q = S_IN + S_OUT + node * S_NODE + cone * S_CONE + leaf * S_LEAF + edge * S_EDGE
     achieved = per_q * n / (kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+    # the same rate with the reference traversal's visit counts (measured once
+    # by the oracle on this workload) at the reference's payload widths: box
+    # tests x (24 + 8 + 20) B, leaves x 40 B, edges x (32 + 48) B
+    ref_counts = reference_counts()
+    ref_bytes = None
+    if ref_counts:
+        c = ref_counts["silhouette"]
+        ref_bytes = 13 + 8 + c["box_tests"] * (24 + 8 + 20) + c["leaf_evals"] * 40 + c["edge_tests"] * (32 + 48)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(), "bytes_per_query": per_q,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_query": per_q,
+                "definition": "algorithmic bytes = this kernel's own per-query visits (counters of the same launch) "
+                              "x record sizes as read; the reference traversal's visits would give "
+                              "bytes_per_query_reference_order (it tests ~4x more boxes)",
+                "bytes_per_query_reference_order": ref_bytes,
+                "traffic_frac": (traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                 "kernel": "k_closest_silhouette<64, false, 3, 0, 16>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
                 "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms, "shape_sweep_ms": sweep,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
@@ -562,9 +642,28 @@ def main():
     extra["ray_qps"] = n / timed(lambda: sc.ray_intersect(q, d))
     extra["sample_in_sphere_qps"] = n / timed(lambda: sc.sample_in_sphere(q, r, u))
 
+    # end to end in the reference's protocol (benchmark.cu:393-416): device
+    # allocation, host-to-device copy of the queries, the query, copy of the
+    # results back; mean of 32 after one warm-up.  Not `value`, which has the
+    # inputs resident in HBM.
+    def e2e():
+        qq = torch.from_numpy(q_np).to(dev)
+        e2, d2 = sc.closest_silhouette(qq)
+        return e2.cpu(), d2.cpu()
+
+    e2e_wall, _ = run_timed(e2e, 32, 1)
+    extra["end_to_end_qps"] = n * 32 / e2e_wall
+    extra["end_to_end_protocol"] = "alloc + H2D of the queries + query + D2H of both outputs, mean of 32 after 1 warm-up"
+
     cpu = None
+    agree = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(V, F, q_np, args.cpu_sample, kernel_ms)
+        cpu, otree, oe, od = cpu_baseline(V, F, q_np, args.cpu_sample)
+        m = len(od)
+        agree = agreement(otree, V, q_np[:m], oe, od, out_e[:m].cpu().numpy(), out_d[:m].cpu().numpy())
+        if not agree["all_agree"]:
+            print(json.dumps({"error": "GPU results differ from the oracle", "agreement": agree}), file=sys.stderr)
+            sys.exit(1)
 
     line = {
         "metric": "closest-silhouette queries/sec @1M tris (plain vs learned-bound); LBVH builds/sec",
@@ -590,6 +689,7 @@ def main():
         **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "agreement_with_oracle": agree,
     }
     print(json.dumps(line))
 

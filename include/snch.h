@@ -154,6 +154,22 @@ int snch_sample_in_sphere(const snch_scene *scene, const float *c, const float *
 /* sample_on_object(bvh, idx, sample_on_object, u, v) (sample.cuh:7-21; scene.cuh:1231-1240) */
 int snch_sample_on_object(const snch_scene *scene, const int32_t *idx, const float *uv, uint32_t n, float *out_pt,
                           void *stream);
+/* query_device(bvh, overlaps(box), out, max) (query.cuh:176-231; predicator.cuh:115-136):
+ * for each query box [lo[i], hi[i]] (n x dim floats each), the primitives whose
+ * leaf boxes intersect it (aabb.cuh:54-82), in the reference's visit order.
+ * The first max_per_query ids land in out_idx[i * max_per_query ...];
+ * out_count[i] is the number found (it may exceed max_per_query, as the
+ * reference's return value does).  out_idx may be NULL when max_per_query = 0. */
+int snch_overlap(const snch_scene *scene, const float *lo, const float *hi, uint32_t n, uint32_t max_per_query,
+                 uint32_t *out_count, uint32_t *out_idx, void *stream);
+/* query_device(bvh, line_intersect(line(o, d)), element_test, out, max)
+ * (query.cuh:12-77; predicator.cuh:8-40; line-box test aabb.cuh:330-364): the
+ * primitives the infinite line o + t d crosses, t of any sign, with the
+ * scene's ray test (scene.cuh:1005-1052 / :545-576) as the element test minus
+ * its t >= 0 condition; ids and line parameters t in the reference's visit
+ * order, as for snch_overlap.  out_t may be NULL. */
+int snch_line_intersect(const snch_scene *scene, const float *o, const float *d, uint32_t n, uint32_t max_per_query,
+                        uint32_t *out_count, uint32_t *out_idx, float *out_t, void *stream);
 /* Same queries with explicit traversal options. */
 /* Morton visiting order of n query points (dim floats each) in the scene's
  * root box: out_perm (DEVICE uint32[n]) receives a permutation of [0, n).

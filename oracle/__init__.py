@@ -438,3 +438,42 @@ def bf_closest_silhouette2d(V, slots, q, flip=False):
                                               ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.orc_bf_closest_silhouette2d(_p(V), _p(slots), slots.shape[0], _p(q), n, int(bool(flip)), _p(sil), _p(dist))
     return sil, dist
+
+
+# ---------------------------------------------------------------- generic queries
+def _generic(tree, q6, kind, max_out):
+    dim = 2 if isinstance(tree, Tree2D) else 3
+    q6 = _c(q6, F32).reshape(-1, 6)
+    n = q6.shape[0]
+    counts = np.empty(n, U32)
+    idx = np.empty((n, max(max_out, 1)), U32)
+    t = np.empty((n, max(max_out, 1)), F32) if kind == 1 else None
+    L = lib()
+    L.orc_generic_query.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    L.orc_generic_query(tree.ref, dim, kind, _p(q6), n, max(max_out, 1), _p(counts), _p(idx), _p(t), 0)
+    return counts, idx[:, :max_out], (t[:, :max_out] if t is not None else None)
+
+
+def _pad6(a, b, dim):
+    a = np.asarray(a, np.float32).reshape(-1, dim)
+    b = np.asarray(b, np.float32).reshape(-1, dim)
+    z = np.zeros((a.shape[0], 3 - dim), np.float32)
+    return np.concatenate([a, z, b, z], 1)
+
+
+def overlap(tree, lo, hi, max_out):
+    """query_device(bvh, overlaps(box), out, max_out) (query.cuh:176-231) per
+    query box [lo, hi]: (total found, first max_out ids in the reference's
+    visit order)."""
+    dim = 2 if isinstance(tree, Tree2D) else 3
+    counts, idx, _ = _generic(tree, _pad6(lo, hi, dim), 0, max_out)
+    return counts, idx
+
+
+def line_intersect(tree, o, d, max_out):
+    """query_device(bvh, line_intersect(line(o, d)), test, out, max_out)
+    (query.cuh:12-77), the element test being the scene's ray test without
+    t >= 0: (total found, first max_out ids, their line parameters t)."""
+    dim = 2 if isinstance(tree, Tree2D) else 3
+    return _generic(tree, _pad6(o, d, dim), 1, max_out)

@@ -2456,3 +2456,153 @@ void orc_bf_closest_silhouette2d(const float *V, const int32_t *slots, int32_t n
         dist[qi] = best;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Generic overlap and line queries (query.cuh:12-77, :176-231): every  */
+/* primitive whose leaf box intersects the query box (aabb.cuh:54-82),   */
+/* or whose leaf box the infinite line crosses (aabb.cuh:330-364) and    */
+/* which passes the element test, in the reference's stack order: pop a  */
+/* node, test its left child then its right child, emit a leaf at once,  */
+/* push an internal child.  At most max_out ids per query are written;   */
+/* the count returned is the total found.  dim 2 uses the x, y slabs.    */
+/* The line's element test is scene.cuh:1005-1052 (3D) / :545-576 (2D)   */
+/* without the t >= 0 condition of a ray (a line extends both ways).     */
+/* N = 1 (the reference reads nodes[0xFFFFFFFF]): the single leaf's box  */
+/* is tested directly.                                                   */
+/* ------------------------------------------------------------------ */
+static int box_box(const float *a, const float *b, int dim) /* aabb.cuh:54-82 */
+{
+    int k;
+    for (k = 0; k < dim; ++k)
+        if (a[3 + k] < b[k] || b[3 + k] < a[k]) return 0;
+    return 1;
+}
+
+static int line_box(const float *b, const float *o, const float *inv, int dim) /* aabb.cuh:330-364 */
+{
+    float t1 = (b[0] - o[0]) * inv[0], t2 = (b[3] - o[0]) * inv[0];
+    float tmin = fminf(t1, t2), tmax = fmaxf(t1, t2);
+    int k;
+    for (k = 1; k < dim; ++k) {
+        t1 = (b[k] - o[k]) * inv[k];
+        t2 = (b[3 + k] - o[k]) * inv[k];
+        tmin = fmaxf(tmin, fminf(t1, t2));
+        tmax = fminf(tmax, fmaxf(t1, t2));
+    }
+    return tmax >= tmin;
+}
+
+/* the element tests of the line query; 1 and *t_out on a hit */
+static int line_triangle(f3 o, f3 dir, f3 v0, f3 v1, f3 v2, float *t_out)
+{
+    f3 e1 = mk(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z);
+    f3 e2 = mk(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z);
+    f3 h = mk(dir.y * e2.z - dir.z * e2.y, dir.z * e2.x - dir.x * e2.z, dir.x * e2.y - dir.y * e2.x);
+    float det = e1.x * h.x + e1.y * h.y + e1.z * h.z, inv, u, v;
+    f3 s, q;
+    if (fabsf(det) < F_EPS) return 0;
+    inv = 1.0f / det;
+    s = mk(o.x - v0.x, o.y - v0.y, o.z - v0.z);
+    u = (s.x * h.x + s.y * h.y + s.z * h.z) * inv;
+    if (u < 0.0f || u > 1.0f) return 0;
+    q = mk(s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x);
+    v = (dir.x * q.x + dir.y * q.y + dir.z * q.z) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0;
+    *t_out = (e2.x * q.x + e2.y * q.y + e2.z * q.z) * inv;
+    return 1;
+}
+
+static int line_segment(f2 o, f2 dir, f2 p0, f2 p1, float *t_out)
+{
+    f2 sd = mk2(p1.x - p0.x, p1.y - p0.y);
+    float D = dir.x * (-sd.y) + dir.y * sd.x, inv, t, s;
+    if (fabsf(D) < F_EPS) return 0;
+    inv = 1.0f / D;
+    t = ((p0.x - o.x) * (-sd.y) - (p0.y - o.y) * (-sd.x)) * inv;
+    s = (dir.x * (p0.y - o.y) - dir.y * (p0.x - o.x)) * inv;
+    if (s >= -1e-3f && s <= 1.0f + 1e-3f) {
+        *t_out = t;
+        return 1;
+    }
+    return 0;
+}
+
+typedef struct { /* the common prefix of orc_tree and orc_tree2d */
+    uint32_t n_obj;
+    const uint32_t *nodes;
+    const float *boxes;
+    const float *cones;
+    const float *V;
+    const int32_t *F;
+} orc_tree_any;
+
+/* kind 0: overlap with the box q (6 floats: lo xyz, hi xyz); kind 1: the
+ * line through q (origin xyz, direction xyz) */
+static uint32_t generic_one(const orc_tree_any *t, int dim, int kind, const float *q, uint32_t max_out,
+                            uint32_t *out_idx, float *out_t)
+{
+    uint32_t stack[128], found = 0;
+    int sp = 0, c;
+    float inv[3] = {0.0f, 0.0f, 0.0f};
+    if (kind == 1) { /* line<T, dim> constructor: dir_inv = 1 / dir (aabb.cuh:288-294) */
+        inv[0] = 1.0f / q[3];
+        inv[1] = 1.0f / q[4];
+        if (dim == 3) inv[2] = 1.0f / q[5];
+    }
+#define GEN_HIT_BOX(b) (kind == 0 ? box_box((b), q, dim) : line_box((b), q, inv, dim))
+#define GEN_EMIT(obj)                                                                            \
+    do {                                                                                         \
+        float tt_ = 0.0f;                                                                        \
+        int ok_ = 1;                                                                             \
+        if (kind == 1) {                                                                         \
+            if (dim == 3) {                                                                      \
+                const int32_t *f_ = t->F + 3 * (size_t)(obj);                                    \
+                ok_ = line_triangle(mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), vget(t->V, f_[0]), \
+                                    vget(t->V, f_[1]), vget(t->V, f_[2]), &tt_);                 \
+            } else {                                                                             \
+                const int32_t *s_ = t->F + 2 * (size_t)(obj);                                    \
+                ok_ = line_segment(mk2(q[0], q[1]), mk2(q[3], q[4]), vget2(t->V, s_[0]),         \
+                                   vget2(t->V, s_[1]), &tt_);                                    \
+            }                                                                                    \
+        }                                                                                        \
+        if (ok_) {                                                                               \
+            if (found < max_out) {                                                               \
+                out_idx[found] = (obj);                                                          \
+                if (out_t) out_t[found] = tt_;                                                   \
+            }                                                                                    \
+            ++found;                                                                             \
+        }                                                                                        \
+    } while (0)
+    if (t->n_obj == 1) {
+        if (GEN_HIT_BOX(t->boxes)) GEN_EMIT(T_OBJ(t, 0));
+        return found;
+    }
+    stack[sp++] = 0;
+    do {
+        uint32_t node = stack[--sp];
+        uint32_t ch[2];
+        ch[0] = T_LEFT(t, node);
+        ch[1] = T_RIGHT(t, node);
+        for (c = 0; c < 2; ++c) {
+            if (GEN_HIT_BOX(t->boxes + 6 * (size_t)ch[c])) {
+                uint32_t obj = T_OBJ(t, ch[c]);
+                if (obj != INVALID) GEN_EMIT(obj);
+                else stack[sp++] = ch[c];
+            }
+        }
+    } while (sp > 0);
+#undef GEN_HIT_BOX
+#undef GEN_EMIT
+    return found;
+}
+
+void orc_generic_query(const void *tree, int dim, int kind, const float *q, int64_t n, uint32_t max_out,
+                       uint32_t *counts, uint32_t *out_idx, float *out_t, int nthreads)
+{
+    int64_t i;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i = 0; i < n; ++i)
+        counts[i] = generic_one((const orc_tree_any *)tree, dim, kind, q + 6 * i, max_out,
+                                out_idx + (size_t)i * max_out, out_t ? out_t + (size_t)i * max_out : NULL);
+}

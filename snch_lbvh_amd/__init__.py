@@ -72,7 +72,7 @@ EXPORTED = [
     "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh",
     "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
     "snch_closest_silhouette_ex", "snch_closest_silhouette_noalloc", "snch_ray", "snch_ray_ex",
-    "snch_sample_in_sphere", "snch_sample_on_object", "snch_scene_export", "snch_scene_silhouettes",
+    "snch_sample_in_sphere", "snch_sample_on_object", "snch_overlap", "snch_line_intersect", "snch_scene_export", "snch_scene_silhouettes",
     "snch_device_malloc", "snch_device_free", "snch_memcpy_h2d", "snch_memcpy_d2h", "snch_stream_synchronize",
     "snch_mlp_num_params", "snch_mlp_create", "snch_mlp_destroy", "snch_mlp_params", "snch_mlp_train",
     "snch_mlp_set_loss",
@@ -108,6 +108,8 @@ def _declare(L):
     L.snch_ray.argtypes = [vp, vp, vp, vp, u32, c_int, vp, vp, vp, vp, vp]
     L.snch_sample_in_sphere.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp]
     L.snch_sample_on_object.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.snch_overlap.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp]
+    L.snch_line_intersect.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp, vp]
     L.snch_scene_export.argtypes = [vp, vp, vp, vp, vp, vp]
     L.snch_scene_silhouettes.argtypes = [vp, vp, vp]
     L.snch_mlp_create.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(vp)]
@@ -335,6 +337,40 @@ class Scene:
         pt = torch.empty((idx.shape[0], self.dim), dtype=torch.float32, device=idx.device)
         check(lib().snch_sample_on_object(self._h, _ptr(idx), _ptr(uv), idx.shape[0], _ptr(pt), _stream()))
         return pt
+
+    def overlap(self, lo, hi, max_per_query=64):
+        """query_device(bvh, overlaps(box)) (query.cuh:176-231) for boxes
+        [lo[i], hi[i]]: (count per query, ids: the first max_per_query in the
+        reference's visit order, -1 beyond the count)."""
+        import torch
+
+        lo = _dev_f32(lo, self.dim, self.device)
+        hi = _dev_f32(hi, self.dim, self.device)
+        n = lo.shape[0]
+        _need_rows(hi, n, "hi")
+        cnt = torch.empty(n, dtype=torch.int32, device=lo.device)
+        idx = torch.full((n, max_per_query), -1, dtype=torch.int32, device=lo.device)
+        check(lib().snch_overlap(self._h, _ptr(lo), _ptr(hi), n, max_per_query, _ptr(cnt),
+                                 _ptr(idx) if max_per_query else None, _stream()))
+        return cnt, idx
+
+    def line_intersect(self, o, d, max_per_query=16):
+        """query_device(bvh, line_intersect(line(o, d))) (query.cuh:12-77):
+        (count per query, ids, line parameters t), ids in the reference's
+        visit order, -1 / nan beyond the count."""
+        import torch
+
+        o = _dev_f32(o, self.dim, self.device)
+        d = _dev_f32(d, self.dim, self.device)
+        n = o.shape[0]
+        _need_rows(d, n, "d")
+        cnt = torch.empty(n, dtype=torch.int32, device=o.device)
+        idx = torch.full((n, max_per_query), -1, dtype=torch.int32, device=o.device)
+        t = torch.full((n, max_per_query), float("nan"), dtype=torch.float32, device=o.device)
+        check(lib().snch_line_intersect(self._h, _ptr(o), _ptr(d), n, max_per_query, _ptr(cnt),
+                                        _ptr(idx) if max_per_query else None, _ptr(t) if max_per_query else None,
+                                        _stream()))
+        return cnt, idx, t
 
     def export(self):
         """The tree in the reference layout (host numpy arrays)."""

@@ -714,6 +714,30 @@ int snch_sample_in_sphere(const snch_scene* s, const float* c, const float* r, c
     return SNCH_OK;
 }
 
+int snch_overlap(const snch_scene* s, const float* lo, const float* hi, uint32_t n, uint32_t max_per_query,
+                 uint32_t* out_count, uint32_t* out_idx, void* stream)
+{
+    SNCH_NEED_SINGLE(s, "snch_overlap");
+    SNCH_CHECK_ARG(n == 0 || (lo && hi && out_count && (max_per_query == 0 || out_idx)),
+                   "snch_overlap: null pointer");
+    SNCH_HIP(launch_generic(s->view(), (int)s->dim, 0, lo, hi, n, max_per_query, out_count, out_idx, nullptr,
+                            static_cast<hipStream_t>(stream)),
+             "snch_overlap");
+    return SNCH_OK;
+}
+
+int snch_line_intersect(const snch_scene* s, const float* o, const float* d, uint32_t n, uint32_t max_per_query,
+                        uint32_t* out_count, uint32_t* out_idx, float* out_t, void* stream)
+{
+    SNCH_NEED_SINGLE(s, "snch_line_intersect");
+    SNCH_CHECK_ARG(n == 0 || (o && d && out_count && (max_per_query == 0 || out_idx)),
+                   "snch_line_intersect: null pointer");
+    SNCH_HIP(launch_generic(s->view(), (int)s->dim, 1, o, d, n, max_per_query, out_count, out_idx, out_t,
+                            static_cast<hipStream_t>(stream)),
+             "snch_line_intersect");
+    return SNCH_OK;
+}
+
 int snch_sample_on_object(const snch_scene* s, const int32_t* idx, const float* uv, uint32_t n, float* out_pt,
                           void* stream)
 {

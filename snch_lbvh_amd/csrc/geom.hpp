@@ -409,7 +409,9 @@ SNCH_HD bool is_silhouette_edge(v3 pa, v3 pb, v3 n0, v3 n1, v3 view, float d, bo
 
 // Moller-Trumbore (scene.cuh:1005-1052); the reference's device reciprocal
 // __frcp_rn is the correctly rounded 1/det, which IEEE division gives.
-SNCH_HD bool ray_triangle(v3 o, v3 dir, v3 v0, v3 v1, v3 v2, float& t_out, float& u_out, float& v_out)
+// ray = false drops the ray's t >= 0 condition: the infinite line's test.
+SNCH_HD bool ray_triangle(v3 o, v3 dir, v3 v0, v3 v1, v3 v2, float& t_out, float& u_out, float& v_out,
+                          bool ray = true)
 {
     v3 e1 = mk(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z);
     v3 e2 = mk(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z);
@@ -427,13 +429,17 @@ SNCH_HD bool ray_triangle(v3 o, v3 dir, v3 v0, v3 v1, v3 v2, float& t_out, float
     if (v < 0.0f || u + v > 1.0f)
         return false;
     float t = (e2.x * q.x + e2.y * q.y + e2.z * q.z) * inv;
-    if (t >= 0.0f) {
+    if (!ray || t >= 0.0f) {
         t_out = t;
         u_out = u;
         v_out = v;
         return true;
     }
     return false;
+}
+SNCH_HD bool line_triangle(v3 o, v3 dir, v3 v0, v3 v1, v3 v2, float& t_out, float& u_out, float& v_out)
+{
+    return ray_triangle(o, dir, v0, v1, v2, t_out, u_out, v_out, false);
 }
 
 // triangle-sphere test (scene.cuh:1054-1117)
@@ -650,8 +656,8 @@ SNCH_HD bool is_silhouette_vertex(v3 n0, v3 n1, float vx, float vy, float d, boo
     return dot0 * dot1 < 0.0f;
 }
 
-// ray-segment test (scene.cuh:545-576)
-SNCH_HD bool ray_segment(v3 o, v3 dir, v3 p0, v3 p1, float& t_out, float& s_out)
+// ray-segment test (scene.cuh:545-576); ray = false: the infinite line's test
+SNCH_HD bool ray_segment(v3 o, v3 dir, v3 p0, v3 p1, float& t_out, float& s_out, bool ray = true)
 {
     const float sx = p1.x - p0.x, sy = p1.y - p0.y;
     const float D = dir.x * (-sy) + dir.y * sx;
@@ -660,12 +666,16 @@ SNCH_HD bool ray_segment(v3 o, v3 dir, v3 p0, v3 p1, float& t_out, float& s_out)
     const float inv = 1.0f / D;
     const float t = ((p0.x - o.x) * (-sy) - (p0.y - o.y) * (-sx)) * inv;
     const float s = (dir.x * (p0.y - o.y) - dir.y * (p0.x - o.x)) * inv;
-    if (s >= -1e-3f && s <= 1.0f + 1e-3f && t >= 0.0f) {
+    if (s >= -1e-3f && s <= 1.0f + 1e-3f && (!ray || t >= 0.0f)) {
         t_out = t;
         s_out = s;
         return true;
     }
     return false;
+}
+SNCH_HD bool line_segment(v3 o, v3 dir, v3 p0, v3 p1, float& t_out, float& s_out)
+{
+    return ray_segment(o, dir, p0, p1, t_out, s_out, false);
 }
 
 // segment-sphere test (scene.cuh:581-602)

@@ -223,6 +223,10 @@ hipError_t launch_sample(const TreeView& t, int dim, const float4* V, const int3
                          hipStream_t st);
 hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, uint32_t nF, const int32_t* idx,
                                    const float* uv, uint32_t n, float* pt, hipStream_t st);
+// overlap (kind 0: boxes [qa, qb]) and line (kind 1: through qa along qb)
+// queries: count per query, the first max_out ids (and line parameters) per row
+hipError_t launch_generic(const TreeView& t, int dim, int kind, const float* qa, const float* qb, uint32_t n,
+                          uint32_t max_out, uint32_t* count, uint32_t* idx, float* tout, hipStream_t st);
 // batched closest point (kind 0) / closest silhouette (kind 1) over all meshes
 hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
                         uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st);

@@ -1094,6 +1094,150 @@ __global__ __launch_bounds__(256) void k_sample_on_object(const float4* __restri
     }
 }
 
+// ------------------------------------------------------------ overlap / line
+// query_device(bvh, overlaps(box), out, max) (query.cuh:176-231) and
+// query_device(bvh, line_intersect(line), element_test, out, max)
+// (query.cuh:12-77): every primitive whose leaf box meets the query box
+// (aabb.cuh:54-82) or the infinite line (slab test, aabb.cuh:330-364; the
+// element test is the scene's ray test without t >= 0), emitted in the
+// reference's order: pop a node, test its left then its right child, emit a
+// leaf at once, push an internal child.  The first max_out ids (and line
+// parameters) land in the query's row; count = all found.  The node ids are
+// the reference's, so the order is the reference's bit for bit.
+template <int BS, int SSZ>
+struct IdStack {
+    uint32_t* lds; // this lane's column: entry s at lds[s * BS]
+    int sp;
+    uint32_t ov[OVF];
+    bool overflow;
+    __device__ __forceinline__ void init(uint32_t* base)
+    {
+        lds = base;
+        sp = 0;
+        overflow = false;
+    }
+    __device__ __forceinline__ void push(uint32_t n)
+    {
+        const int s = sp % SSZ;
+        if (sp >= SSZ) {
+            if (sp - SSZ >= OVF) {
+                overflow = true;
+                return;
+            }
+            ov[sp - SSZ] = lds[s * BS];
+        }
+        lds[s * BS] = n;
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        --sp;
+        const int s = sp % SSZ;
+        const uint32_t n = lds[s * BS];
+        if (sp >= SSZ)
+            lds[s * BS] = ov[sp - SSZ];
+        return n;
+    }
+};
+
+template <int DIM>
+__device__ __forceinline__ bool overlap_box(const box3& b, const box3& q)
+{
+    if (b.hi.x < q.lo.x || q.hi.x < b.lo.x || b.hi.y < q.lo.y || q.hi.y < b.lo.y)
+        return false;
+    return DIM == 2 || !(b.hi.z < q.lo.z || q.hi.z < b.lo.z);
+}
+
+template <int DIM>
+__device__ __forceinline__ bool line_box(const box3& b, v3 o, v3 inv)
+{
+    float t1 = (b.lo.x - o.x) * inv.x, t2 = (b.hi.x - o.x) * inv.x;
+    float tmin = fmn(t1, t2), tmax = fmx(t1, t2);
+    t1 = (b.lo.y - o.y) * inv.y;
+    t2 = (b.hi.y - o.y) * inv.y;
+    tmin = fmx(tmin, fmn(t1, t2));
+    tmax = fmn(tmax, fmx(t1, t2));
+    if (DIM == 3) {
+        t1 = (b.lo.z - o.z) * inv.z;
+        t2 = (b.hi.z - o.z) * inv.z;
+        tmin = fmx(tmin, fmn(t1, t2));
+        tmax = fmn(tmax, fmx(t1, t2));
+    }
+    return tmax >= tmin;
+}
+
+template <int BS, int DIM, int KIND>
+__global__ __launch_bounds__(BS) void k_generic(TreeView t, const float* __restrict__ qa, const float* __restrict__ qb,
+                                                uint32_t n, uint32_t max_out, uint32_t* __restrict__ out_count,
+                                                uint32_t* __restrict__ out_idx, float* __restrict__ out_t,
+                                                uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t stk[SS * BS];
+    const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (qi >= n)
+        return;
+    // KIND 0: the box [qa, qb]; KIND 1: the line through qa along qb
+    const v3 a = load_point<DIM>(qa, qi), b = load_point<DIM>(qb, qi);
+    box3 qbox;
+    qbox.lo = a;
+    qbox.hi = b;
+    const v3 inv = mk(1.0f / b.x, 1.0f / b.y, DIM == 3 ? 1.0f / b.z : 0.0f); // line(o, dir) (aabb.cuh:288-294)
+    uint32_t found = 0;
+    const size_t row = (size_t)qi * max_out;
+    auto hit_box = [&](const box3& bx) { return KIND == 0 ? overlap_box<DIM>(bx, qbox) : line_box<DIM>(bx, a, inv); };
+    auto emit = [&](uint32_t j) {
+        if (j >= t.n_leaves)
+            return;
+        const float4 la = t.tleaf[3 * j], lb = t.tleaf[3 * j + 1];
+        float tt = 0.0f;
+        bool ok = true;
+        if (KIND == 1) {
+            float uh, vh;
+            if (DIM == 3) {
+                const float4 lc = t.tleaf[3 * j + 2];
+                ok = line_triangle(a, b, mk(la.x, la.y, la.z), mk(lb.x, lb.y, lb.z), mk(lc.x, lc.y, lc.z), tt, uh, vh);
+            } else {
+                ok = line_segment(a, b, mk(la.x, la.y, 0.0f), mk(lb.x, lb.y, 0.0f), tt, uh);
+            }
+        }
+        if (!ok)
+            return;
+        if (found < max_out) {
+            out_idx[row + found] = __float_as_uint(la.w);
+            if (KIND == 1 && out_t)
+                out_t[row + found] = tt;
+        }
+        ++found;
+    };
+    if (t.n_leaves == 1) {
+        if (hit_box(ldbox2(t.nbox, 0)))
+            emit(0);
+    } else {
+        IdStack<BS, SS> st;
+        st.init(stk + threadIdx.x);
+        st.push(0);
+        // each internal node is pushed at most once
+        for (uint32_t it = 0; it < t.n_leaves && st.sp > 0; ++it) {
+            const NodeRec nr = load_node(t.tnode, st.pop());
+            if (hit_box(nr.L)) {
+                if (nr.lref & LEAF_BIT)
+                    emit(nr.lref & ~LEAF_BIT);
+                else
+                    st.push(nr.lref);
+            }
+            if (hit_box(nr.R)) {
+                if (nr.rref & LEAF_BIT)
+                    emit(nr.rref & ~LEAF_BIT);
+                else
+                    st.push(nr.rref);
+            }
+        }
+        if (st.overflow && err)
+            atomicOr(err, 1u);
+    }
+    out_count[qi] = found;
+}
+
 // Morton code of each query point in the root box (clamped to it), used to
 // visit queries in spatial order: neighbouring lanes then walk mostly the
 // same nodes, which keeps a wave's memory accesses and branches together
@@ -1298,6 +1442,23 @@ hipError_t launch_sample(const TreeView& t, int dim, const float4* V, const int3
         return hipSuccess;
     SNCH_DISPATCH_DIM(dim, hipLaunchKernelGGL(k_sample<DIM>, dim3(nblocks(n, 256)), dim3(256), 0, st, t, V, F, c, r,
                                               u, n, idx, pdf, pt));
+    return hipGetLastError();
+}
+
+hipError_t launch_generic(const TreeView& t, int dim, int kind, const float* qa, const float* qb, uint32_t n,
+                          uint32_t max_out, uint32_t* count, uint32_t* idx, float* tout, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    uint32_t* err = traversal_error_word();
+    SNCH_DISPATCH_DIM(dim, {
+        if (kind == 0)
+            hipLaunchKernelGGL((k_generic<64, DIM, 0>), dim3(nblocks(n, 64)), dim3(64), 0, st, t, qa, qb, n, max_out,
+                               count, idx, tout, err);
+        else
+            hipLaunchKernelGGL((k_generic<64, DIM, 1>), dim3(nblocks(n, 64)), dim3(64), 0, st, t, qa, qb, n, max_out,
+                               count, idx, tout, err);
+    });
     return hipGetLastError();
 }
 
