@@ -293,8 +293,8 @@ def workload_batched(args):
 
     def launch(tune):
         opts = b._opts(tune, 0)
-        S.check(S.lib().snch_batch_closest_silhouette(b._h, S._ptr(q), qo.ctypes.data, 0, S._ptr(sil), S._ptr(dist),
-                                                      ctypes.byref(opts), S._stream()))
+        S.check(S.lib().snch_batch_closest_silhouette(b._h, S._ptr(q), qo.ctypes.data, 0, None, S._ptr(sil),
+                                                      S._ptr(dist), ctypes.byref(opts), S._stream()))
 
     wall0, ev0 = run_timed(torch, lambda: launch(None), max(2, args.steps // 3), 1)
     t0 = time.perf_counter()

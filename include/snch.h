@@ -194,8 +194,12 @@ int snch_closest_silhouette_ex(const snch_scene *scene, const float *q, uint32_t
  * more primitives first).  Results equal the per-mesh single calls. */
 int snch_batch_closest_point(const snch_scene *scene, const float *q, const uint32_t *q_off, uint32_t *out_idx,
                              float *out_dist, const snch_query_opts *opts, void *stream);
+/* seed_radius (may be NULL): per-query pruning radius (DEVICE, n floats), as
+ * for snch_closest_silhouette: the search within it is exact and a query not
+ * settled there resumes without it, so outputs equal the unseeded call's. */
 int snch_batch_closest_silhouette(const snch_scene *scene, const float *q, const uint32_t *q_off, int flip,
-                                  uint32_t *out_sil, float *out_dist, const snch_query_opts *opts, void *stream);
+                                  const float *seed_radius, uint32_t *out_sil, float *out_dist,
+                                  const snch_query_opts *opts, void *stream);
 
 /* Autotuner for the batched queries (kind 0 = closest point, 1 = closest
  * silhouette): coordinate descent from leaf 1 / block 64 / stack 16 / 64

@@ -90,7 +90,7 @@ def _declare(L):
     L.snch_scene_create_tri_batch.argtypes = [vp, u32, vp, vp, vp, vp, ctypes.POINTER(vp)]
     L.snch_scene_mesh_count.argtypes = [vp, vp]
     L.snch_batch_closest_point.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-    L.snch_batch_closest_silhouette.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp]
+    L.snch_batch_closest_silhouette.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp, vp]
     L.snch_batch_autotune.argtypes = [vp, u32, vp, vp, u32, c_int, vp, vp]
     L.snch_scene_export_mesh.argtypes = [vp, u32, vp, vp, vp, vp, vp]
     L.snch_scene_destroy.argtypes = [vp]
@@ -471,18 +471,22 @@ class BatchScene:
                                              ctypes.byref(opts), _stream()))
         return idx, dist
 
-    def closest_silhouette(self, q, q_off, flip=False, tune=None, coherent=0):
+    def closest_silhouette(self, q, q_off, flip=False, tune=None, coherent=0, seed_radius=None):
+        """seed_radius: optional per-query pruning radius (a learned bound);
+        results equal the unseeded call's."""
         import torch
 
         q = _dev_f32(q, 3, self.device)
         q_off = np.ascontiguousarray(q_off, np.uint32)
         n = q.shape[0]
         _check_qoff(q_off, self.n_meshes, n)
+        seed = _dev_f32(seed_radius, 0, self.device).reshape(-1) if seed_radius is not None else None
+        _need_rows(seed, n, "seed_radius")
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
         opts = self._opts(tune, coherent)
-        check(lib().snch_batch_closest_silhouette(self._h, _ptr(q), q_off.ctypes.data, int(bool(flip)), _ptr(sil),
-                                                  _ptr(dist), ctypes.byref(opts), _stream()))
+        check(lib().snch_batch_closest_silhouette(self._h, _ptr(q), q_off.ctypes.data, int(bool(flip)), _ptr(seed),
+                                                  _ptr(sil), _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist
 
     def autotune(self, q, q_off, kind="silhouette", sample_per_mesh=0, flip=False):

@@ -801,7 +801,8 @@ int snch_batch_closest_point(const snch_scene* s, const float* q, const uint32_t
 }
 
 int snch_batch_closest_silhouette(const snch_scene* s, const float* q, const uint32_t* q_off, int flip,
-                                  uint32_t* out_sil, float* out_dist, const snch_query_opts* opts, void* stream)
+                                  const float* seed_radius, uint32_t* out_sil, float* out_dist,
+                                  const snch_query_opts* opts, void* stream)
 {
     SNCH_NEED_BUILT(s, "snch_batch_closest_silhouette");
     SNCH_CHECK_ARG(s->dim == 3, "snch_batch_closest_silhouette: batches hold triangle meshes");
@@ -815,7 +816,8 @@ int snch_batch_closest_silhouette(const snch_scene* s, const float* q, const uin
         return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     SNCH_HIP(scratch_begin(s, st), "snch_batch_closest_silhouette: stream wait");
-    SNCH_HIP(launch_batch(s, 1, q, q_off, flip, out_sil, out_dist, cfg, st), "snch_batch_closest_silhouette");
+    SNCH_HIP(launch_batch(s, 1, q, q_off, flip, out_sil, out_dist, cfg, st, seed_radius),
+             "snch_batch_closest_silhouette");
     SNCH_HIP(scratch_end(s, st), "snch_batch_closest_silhouette: event");
     return SNCH_OK;
 }

@@ -228,8 +228,10 @@ hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, u
 hipError_t launch_generic(const TreeView& t, int dim, int kind, const float* qa, const float* qb, uint32_t n,
                           uint32_t max_out, uint32_t* count, uint32_t* idx, float* tout, hipStream_t st);
 // batched closest point (kind 0) / closest silhouette (kind 1) over all meshes
+// seed (closest silhouette, nullable): per-query pruning radius; results equal the unseeded ones
 hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
-                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st);
+                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st,
+                        const float* seed = nullptr);
 uint32_t* traversal_error_word(); // [0] traversal overflow, [1] sort wait bound (capi.hip)
 // noalloc.hip: the device address of the static scratch array on `device`
 // (current), resolved once per device

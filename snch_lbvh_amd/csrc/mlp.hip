@@ -12,9 +12,18 @@
 // traversal verifies every answer (query.hip), so the network only changes
 // how fast a query runs, never its result.
 //
-// Kernels (one wave = 16 points = one MFMA M-tile; v_mfma_f32_16x16x16_bf16):
-//   k_mlp_infer   all BF16 weights staged in LDS once per block (112 KB),
-//                 per-wave ping-pong activation tiles, FP32 accumulate
+// Kernels:
+//   k_mlp_infer   out^T = W x^T per layer on v_mfma_f32_16x16x32_bf16 (gfx950's
+//                 K = 32 BF16 MFMA): the weights are the A operand, staged in
+//                 LDS once per block (112 KB) and shared by its 16 waves; the
+//                 activations are the B operand and never leave registers: a
+//                 layer's accumulator tiles are turned into the next layer's
+//                 B fragments in place, because the hidden features are
+//                 numbered so that the output layout of two 16-row tiles is
+//                 the B layout of one 32-deep K block (see infer_index);
+//                 each wave takes P tiles of 16 points, so every weight
+//                 fragment read from LDS feeds P MFMAs (one wave = 16 P points)
+//   (the training kernels use v_mfma_f32_16x16x16_bf16, one wave = 16 points)
 //   k_mlp_train   forward + backward for a 64-point tile: weight gradients
 //                 dW = delta^T a_prev and input deltas delta W both on MFMA,
 //                 added into an FP32 gradient with one atomic per weight
@@ -42,9 +51,17 @@ constexpr int OFF_W5 = 57856, OFF_B5 = 57984, NPARAM = 57985;
 constexpr int SW_OFF[4] = {0, 8192, 24576, 40960}; // offsets in the bf16 fragment arrays
 constexpr int NSW = 57344;
 constexpr float LOG_EPS = 1e-3f;
+#ifndef MLP_INFER_WAVES
+#define MLP_INFER_WAVES 16 // waves per inference block (they share one LDS copy of the weights)
+#endif
+#ifndef MLP_INFER_TILES
+#define MLP_INFER_TILES 2 // 16-point tiles per wave at a time (reuse of each weight fragment)
+#endif
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ unsigned short f2bf(float f)
 {
@@ -63,6 +80,12 @@ __device__ __forceinline__ float dsilu(float z)
 }
 
 __device__ __forceinline__ v4f mma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// D (16 x 16) += A (16 x 32) B (32 x 16); lane l = 16 g + r holds A[r][8 g + j],
+// B[8 g + j][r] (j = 0..7) and D[4 g + i][r] (i = 0..3)
+__device__ __forceinline__ v4f mma32(v8bf a, v8bf b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 // fragment index of W_l[n][k] (forward B operand) and of the transposed use
 __host__ __device__ __forceinline__ int fwd_index(int l, int n, int k)
@@ -73,6 +96,26 @@ __host__ __device__ __forceinline__ int fwd_index(int l, int n, int k)
 __host__ __device__ __forceinline__ int bwd_index(int l, int n, int k)
 {
     return SW_OFF[l] + ((((k >> 4) * (H >> 4) + (n >> 4)) * 64 + ((n & 15) >> 2) * 16 + (k & 15)) << 2) + (n & 3);
+}
+// fragment index of W_l[n][k] for k_mlp_infer: the A operand of tile (mt, kb)
+// (16 output rows x 32 inputs) is 64 lanes x 8 BF16, lane 16 g + r holding
+// row 16 mt + r.  Layer 0 reads the features in order (position 8 g + j of
+// block kb is input 32 kb + 8 g + j).  Layers 1-3 read the previous layer's
+// outputs as its accumulators leave them: lane 16 g + r of output tiles 2 kb
+// and 2 kb + 1 holds rows 4 g + i of each, so position 8 g + j of block kb is
+// input 32 kb + 4 g + j (j < 4) or 32 kb + 16 + 4 g + j - 4 (j >= 4).
+__host__ __device__ __forceinline__ int infer_index(int l, int n, int k)
+{
+    const int KB = KIN[l] >> 5, mt = n >> 4, r = n & 15, kb = k >> 5, kl = k & 31;
+    int g, j;
+    if (l == 0) {
+        g = kl >> 3;
+        j = kl & 7;
+    } else {
+        g = (kl & 15) >> 2;
+        j = (kl & 3) + (kl >= 16 ? 4 : 0);
+    }
+    return SW_OFF[l] + (((mt * KB + kb) * 64 + 16 * g + r) << 3) + j;
 }
 
 struct Frame {
@@ -99,82 +142,135 @@ __device__ __forceinline__ void features(const float* __restrict__ fourier, cons
 }
 
 // ------------------------------------------------------------------ infer
-// mode 0: predicted distance; mode 1: seed radius
-__global__ __launch_bounds__(256) void k_mlp_infer(const unsigned short* __restrict__ wsw,
-                                                   const float* __restrict__ theta,
-                                                   const float* __restrict__ fourier, Frame fr,
-                                                   const float* __restrict__ pts, uint32_t n, int mode, float rel,
-                                                   float absm, float D, float* __restrict__ out)
+// One hidden layer L for P tiles of 16 points: acc = W_L bop (+ bias, SiLU);
+// the results become the next layer's B operand in place (L < 3) or are dotted
+// with the output weights (L = 3: o = partial w5 . a4 of this lane's rows).
+// A function per layer keeps every register array statically indexed.
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+// two floats as one word of two round-to-nearest-even BF16 halves (low first):
+// one v_cvt_pk_bf16_f32 on gfx950
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 {
-    __shared__ v4s sW[NSW / 4];
-    __shared__ unsigned short sAct[4][2][16 * ACT_STRIDE];
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v2f{lo, hi}, v2bf));
+}
+// SiLU for the inference path: the hardware exp and reciprocal (about 1 ulp,
+// far inside the BF16 rounding its result gets)
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+template <int L, int P>
+__device__ __forceinline__ void infer_layer(const v4u* sW, const float* sB, v4u (&bop)[P][4], float (&o)[P],
+                                            int lane, int g)
+{
+    constexpr int KB = KIN[L] >> 5;
+    v4f acc[P][8];
+#pragma unroll
+    for (int pt = 0; pt < P; ++pt)
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+            acc[pt][mt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+            const v8bf a = __builtin_bit_cast(v8bf, sW[(SW_OFF[L] >> 3) + (mt * KB + kb) * 64 + lane]);
+#pragma unroll
+            for (int pt = 0; pt < P; ++pt)
+                acc[pt][mt] = mma32(a, __builtin_bit_cast(v8bf, bop[pt][kb]), acc[pt][mt]);
+        }
+    // bias + SiLU; lane (g, r) holds outputs 16 mt + 4 g + i of point r
+#pragma unroll
+    for (int pt = 0; pt < P; ++pt) {
+        float s = 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * mt + 4 * g + i;
+                acc[pt][mt][i] = silu_fast(acc[pt][mt][i] + sB[L * H + row]);
+                if (L == 3)
+                    s += sB[4 * H + row] * acc[pt][mt][i];
+            }
+        o[pt] = s;
+        if (L < 3) { // tiles 2 kb and 2 kb + 1 become K block kb of the next layer
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const v4f a0 = acc[pt][2 * kb], a1 = acc[pt][2 * kb + 1];
+                bop[pt][kb] = v4u{pack_bf16(a0[0], a0[1]), pack_bf16(a0[2], a0[3]), pack_bf16(a1[0], a1[1]),
+                                  pack_bf16(a1[2], a1[3])};
+            }
+        }
+    }
+}
+
+// mode 0: predicted distance; mode 1: seed radius.  NW waves per block share
+// the LDS weights; each wave takes P tiles of 16 points at a time.
+template <int NW, int P>
+__global__ __launch_bounds__(NW * 64) void k_mlp_infer(const v4u* __restrict__ wisw, const float* __restrict__ theta,
+                                                       const float* __restrict__ fourier, Frame fr,
+                                                       const float* __restrict__ pts, uint32_t n, int mode, float rel,
+                                                       float absm, float D, float* __restrict__ out)
+{
+    __shared__ v4u sW[NSW / 8];
     __shared__ float sB[4 * H + H + 1];
     __shared__ float sFour[3 * NFREQ];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
-    for (int i = tid; i < NSW / 4; i += 256)
-        sW[i] = reinterpret_cast<const v4s*>(wsw)[i];
-    for (int i = tid; i < 4 * H; i += 256)
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r = lane & 15, g = lane >> 4;
+    for (int i = tid; i < NSW / 8; i += NW * 64)
+        sW[i] = wisw[i];
+    for (int i = tid; i < 4 * H; i += NW * 64)
         sB[i] = theta[OFF_B[i / H] + (i % H)];
-    for (int i = tid; i < H; i += 256)
+    for (int i = tid; i < H; i += NW * 64)
         sB[4 * H + i] = theta[OFF_W5 + i];
     if (tid == 0)
         sB[5 * H] = theta[OFF_B5];
-    for (int i = tid; i < 3 * NFREQ; i += 256)
+    for (int i = tid; i < 3 * NFREQ; i += NW * 64)
         sFour[i] = fourier[i];
     __syncthreads();
-    const uint32_t ntiles = (n + 63) / 64;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint32_t p = tile * 64 + w * 16 + r; // this lane's point for features / output
-        const bool valid = p < n;
-        unsigned short* in = sAct[w][0];
-        unsigned short* outb = sAct[w][1];
-        {
+    const uint32_t ntiles = (n + 15) / 16;
+    for (uint32_t t0 = ((uint32_t)blockIdx.x * NW + w) * P; t0 < ntiles; t0 += gridDim.x * NW * P) {
+        // a compiler-level memory barrier: the weight fragments are the same in
+        // every iteration, and without it the compiler hoists all 112 of their
+        // LDS reads out of the loop (448 registers, spilled to scratch)
+        asm volatile("" ::: "memory");
+        // layer-0 B operand: features of point r of each tile, frequencies 8 g .. 8 g + 7
+        v4u bop[P][4];
+#pragma unroll
+        for (int pt = 0; pt < P; ++pt) {
+            const uint32_t p = (t0 + pt) * 16 + r;
+            const bool valid = p < n;
             const float x = valid ? pts[3 * p] : 0.0f, y = valid ? pts[3 * p + 1] : 0.0f,
                         z = valid ? pts[3 * p + 2] : 0.0f;
-            features(sFour, fr, x, y, z, q, in + r * ACT_STRIDE);
-        }
-        __syncthreads();
-        for (int l = 0; l < 4; ++l) {
-            const int K = KIN[l];
-            v4f acc[8];
+            const float xh = (x - fr.cx) * fr.ihx, yh = (y - fr.cy) * fr.ihy, zh = (z - fr.cz) * fr.ihz;
+            float sn[8], cs[8];
 #pragma unroll
-            for (int nt = 0; nt < 8; ++nt)
-                acc[nt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-            for (int ks = 0; ks < K / 16; ++ks) {
-                const v4s a = *reinterpret_cast<const v4s*>(in + r * ACT_STRIDE + ks * 16 + 4 * q);
-#pragma unroll
-                for (int nt = 0; nt < 8; ++nt) {
-                    const v4s b = sW[(SW_OFF[l] >> 2) + (nt * (K >> 4) + ks) * 64 + lane];
-                    acc[nt] = mma(a, b, acc[nt]);
-                }
+            for (int j = 0; j < 8; ++j) {
+                const int fi = 8 * g + j;
+                const float arg = sFour[3 * fi] * xh + sFour[3 * fi + 1] * yh + sFour[3 * fi + 2] * zh;
+                sn[j] = __sinf(arg); // hardware sine and cosine (|arg| stays below ~40 rad):
+                cs[j] = __cosf(arg); // errors far below the BF16 rounding of the features
             }
-            __syncthreads();
+            bop[pt][0] = v4u{pack_bf16(sn[0], sn[1]), pack_bf16(sn[2], sn[3]), pack_bf16(sn[4], sn[5]),
+                             pack_bf16(sn[6], sn[7])};
+            bop[pt][1] = v4u{pack_bf16(cs[0], cs[1]), pack_bf16(cs[2], cs[3]), pack_bf16(cs[4], cs[5]),
+                             pack_bf16(cs[6], cs[7])};
+        }
+        float o[P];
+        infer_layer<0, P>(sW, sB, bop, o, lane, g);
+        infer_layer<1, P>(sW, sB, bop, o, lane, g);
+        infer_layer<2, P>(sW, sB, bop, o, lane, g);
+        infer_layer<3, P>(sW, sB, bop, o, lane, g);
 #pragma unroll
-            for (int nt = 0; nt < 8; ++nt) {
-                const int col = nt * 16 + r;
-                const float bias = sB[l * H + col];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    outb[(4 * q + i) * ACT_STRIDE + col] = f2bf(silu(acc[nt][i] + bias));
+        for (int pt = 0; pt < P; ++pt) {
+            float s = o[pt];
+            s += __shfl_xor(s, 16, 64);
+            s += __shfl_xor(s, 32, 64);
+            const uint32_t p = (t0 + pt) * 16 + r;
+            if (g == 0 && p < n) {
+                float d = __expf(s + sB[5 * H]) - LOG_EPS;
+                d = d > 0.0f ? d : 0.0f;
+                out[p] = mode == 0 ? d * D : (d * (1.0f + rel) + absm) * D;
             }
-            __syncthreads();
-            unsigned short* t = in;
-            in = outb;
-            outb = t;
         }
-        // o = w5 . a4 + b5: lane (r, q) sums k in [32 q, 32 q + 32) for point r
-        float s = 0.0f;
-        for (int k = 32 * q; k < 32 * q + 32; ++k)
-            s += sB[4 * H + k] * bf2f(in[r * ACT_STRIDE + k]);
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        if (q == 0 && valid) {
-            const float o = s + sB[5 * H];
-            float d = (__expf(o) - LOG_EPS);
-            d = d > 0.0f ? d : 0.0f;
-            out[p] = mode == 0 ? d * D : (d * (1.0f + rel) + absm) * D;
-        }
-        __syncthreads();
     }
 }
 
@@ -353,8 +449,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(const unsigned short* __restr
 __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ theta, float* __restrict__ grad,
                                                   float* __restrict__ m1, float* __restrict__ m2,
                                                   unsigned short* __restrict__ wsw, unsigned short* __restrict__ wtsw,
-                                                  float lr, float b1, float b2, float eps, float c1, float c2,
-                                                  int update)
+                                                  unsigned short* __restrict__ wisw, float lr, float b1, float b2,
+                                                  float eps, float c1, float c2, int update)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= NPARAM)
@@ -378,6 +474,7 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ theta, flo
             const unsigned short h = f2bf(th);
             wsw[fwd_index(l, n, k)] = h;
             wtsw[bwd_index(l, n, k)] = h;
+            wisw[infer_index(l, n, k)] = h;
         }
     }
 }
@@ -399,7 +496,7 @@ struct snch_mlp {
     snch_context* ctx = nullptr;
     Frame fr{};
     float D = 1.0f;
-    DeviceBuffer theta, grad, m1, m2, wsw, wtsw, fourier, loss;
+    DeviceBuffer theta, grad, m1, m2, wsw, wtsw, wisw, fourier, loss;
     std::vector<float> h_fourier;
     int step = 0;
 };
@@ -422,7 +519,8 @@ int upload_params(snch_mlp* m, const float* params, hipStream_t st)
         return hip_fail(e, "snch_mlp: upload");
     hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, m->theta.as<float>(),
                        m->grad.as<float>(), m->m1.as<float>(), m->m2.as<float>(), m->wsw.as<unsigned short>(),
-                       m->wtsw.as<unsigned short>(), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0);
+                       m->wtsw.as<unsigned short>(), m->wisw.as<unsigned short>(), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+                       0);
     e = hipStreamSynchronize(st);
     return e == hipSuccess ? SNCH_OK : hip_fail(e, "snch_mlp: refresh");
 }
@@ -473,6 +571,7 @@ int snch_mlp_create(snch_context* ctx, const float* lo, const float* hi, float s
     if ((e = m->theta.ensure(sizeof(float) * NPARAM)) != hipSuccess || (e = m->grad.ensure(sizeof(float) * NPARAM)) ||
         (e = m->m1.ensure(sizeof(float) * NPARAM)) || (e = m->m2.ensure(sizeof(float) * NPARAM)) ||
         (e = m->wsw.ensure(sizeof(unsigned short) * NSW)) || (e = m->wtsw.ensure(sizeof(unsigned short) * NSW)) ||
+        (e = m->wisw.ensure(sizeof(unsigned short) * NSW)) ||
         (e = m->fourier.ensure(sizeof(float) * 3 * NFREQ)) || (e = m->loss.ensure(sizeof(float) * 4)) ||
         (e = hipMemcpy(m->fourier.p, m->h_fourier.data(), sizeof(float) * 3 * NFREQ, hipMemcpyHostToDevice))) {
         delete m;
@@ -582,7 +681,8 @@ int snch_mlp_train(snch_mlp* m, const float* pts, const float* dist, uint32_t n,
             const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->step));
             hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, m->theta.as<float>(),
                                m->grad.as<float>(), m->m1.as<float>(), m->m2.as<float>(),
-                               m->wsw.as<unsigned short>(), m->wtsw.as<unsigned short>(), lr, b1, b2, eps, c1, c2, 1);
+                               m->wsw.as<unsigned short>(), m->wtsw.as<unsigned short>(), m->wisw.as<unsigned short>(),
+                               lr, b1, b2, eps, c1, c2, 1);
         }
         float l = 0.0f;
         hipError_t e = hipGetLastError();
@@ -607,11 +707,12 @@ int snch_mlp_predict(snch_mlp* m, const float* pts, uint32_t n, int mode, float 
     }
     if (n == 0)
         return SNCH_OK;
-    const uint32_t ntiles = (n + 63) / 64;
-    const uint32_t grid = ntiles < 1024 ? ntiles : 1024;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       m->wsw.as<unsigned short>(), m->theta.as<float>(), m->fourier.as<float>(), m->fr, pts, n, mode,
-                       rel, absm, m->D, out);
+    constexpr int NW = MLP_INFER_WAVES, P = MLP_INFER_TILES;
+    const uint32_t groups = (n + 16 * NW * P - 1) / (16 * NW * P);
+    const uint32_t grid = groups < 2048 ? groups : 2048;
+    hipLaunchKernelGGL((k_mlp_infer<NW, P>), dim3(grid), dim3(NW * 64), 0, static_cast<hipStream_t>(stream),
+                       m->wisw.as<v4u>(), m->theta.as<float>(), m->fourier.as<float>(), m->fr, pts, n, mode, rel, absm,
+                       m->D, out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SNCH_OK : hip_fail(e, "snch_mlp_predict");
 }

@@ -634,12 +634,14 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
         const float nb2 = best * best;
         bool gL = hL && !lLeaf && Lmd <= nb2;
         bool gR = hR && !rLeaf && Rmd <= nb2;
-        if (gL && Lmd > cap2) { // beyond the cap: set aside
-            dlost = dlost || !st.defer(lref, Lmd);
+        if (gL && Lmd > cap2) { // beyond the cap: set aside (a hard cap drops it)
+            if (HINT)
+                dlost = dlost || !st.defer(lref, Lmd);
             gL = false;
         }
         if (gR && Rmd > cap2) {
-            dlost = dlost || !st.defer(rref, Rmd);
+            if (HINT)
+                dlost = dlost || !st.defer(rref, Rmd);
             gR = false;
         }
         if (gL && gR) {
@@ -887,10 +889,13 @@ struct BatchTable {
     uint32_t n_sched, total_chunks, qpw;
 };
 
-template <int BS, int SSZ, int KIND>
+// SEEDED (closest silhouette): seed[qi] is the query's pruning radius (a
+// learned bound); the search within it is exact, a query not settled there
+// resumes from the nodes it set aside, so results equal the unseeded ones
+template <int BS, int SSZ, int KIND, bool SEEDED>
 __global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
-                                              uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
-                                              uint32_t* __restrict__ err)
+                                              const float* __restrict__ seed, uint32_t* __restrict__ out_idx,
+                                              float* __restrict__ out_dist, uint32_t* __restrict__ err)
 {
     __shared__ uint32_t stk[SSZ * BS];
     __shared__ EdgeLds elds[BS / 64];
@@ -928,8 +933,11 @@ __global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __r
                 cp_query<BS, SSZ, false, 3>(tv, p, stk + threadIdx.x, best, bi, cnt, ovf);
                 best = dm::sqrt_(best);
             } else {
-                sil_query<BS, SSZ, false, 3, false, true>(tv, p, flip != 0, F_INF, stk + threadIdx.x,
-                                                          &elds[threadIdx.x / 64], best, bi, cnt, ovf);
+                float cap = SEEDED ? seed[qi] : F_INF;
+                if (!(cap >= 0.0f))
+                    cap = F_INF;
+                sil_query<BS, SSZ, false, 3, SEEDED, true>(tv, p, flip != 0, cap, stk + threadIdx.x,
+                                                           &elds[threadIdx.x / 64], best, bi, cnt, ovf);
             }
             if (out_idx)
                 out_idx[qi] = bi;
@@ -1472,49 +1480,51 @@ hipError_t launch_sample_on_object(int dim, const float4* V, const int32_t* F, u
     return hipGetLastError();
 }
 
-template <int BS, int SSZ, int KIND>
-static hipError_t launch_batch_cfg(const BatchTable& bt, const float* q, int flip, uint32_t* out_idx,
-                                   float* out_dist, uint32_t n_cu, hipStream_t st)
+template <int BS, int SSZ, int KIND, bool SEEDED>
+static hipError_t launch_batch_cfg(const BatchTable& bt, const float* q, int flip, const float* seed,
+                                   uint32_t* out_idx, float* out_dist, uint32_t n_cu, hipStream_t st)
 {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<BS, SSZ, KIND>, BS, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<BS, SSZ, KIND, SEEDED>, BS, 0);
     if (e != hipSuccess)
         return e;
     const uint32_t waves = n_cu * (uint32_t)std::max(per_cu, 1) * (BS / 64);
     // no more waves than chunks
     const uint32_t grid = std::max(1u, std::min(n_cu * (uint32_t)std::max(per_cu, 1),
                                                 (std::min(waves, bt.total_chunks) + BS / 64 - 1) / (BS / 64)));
-    hipLaunchKernelGGL((k_batch<BS, SSZ, KIND>), dim3(grid), dim3(BS), 0, st, bt, q, flip, out_idx, out_dist,
-                       traversal_error_word());
+    hipLaunchKernelGGL((k_batch<BS, SSZ, KIND, SEEDED>), dim3(grid), dim3(BS), 0, st, bt, q, flip, seed, out_idx,
+                       out_dist, traversal_error_word());
     return hipGetLastError();
 }
 
-template <int BS, int KIND>
-static hipError_t launch_batch_bs(const BatchTable& bt, uint32_t ssz, const float* q, int flip, uint32_t* out_idx,
-                                  float* out_dist, uint32_t n_cu, hipStream_t st)
+template <int BS, int KIND, bool SEEDED>
+static hipError_t launch_batch_bs(const BatchTable& bt, uint32_t ssz, const float* q, int flip, const float* seed,
+                                  uint32_t* out_idx, float* out_dist, uint32_t n_cu, hipStream_t st)
 {
     switch (ssz) {
-    case 8: return launch_batch_cfg<BS, 8, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
-    case 24: return launch_batch_cfg<BS, 24, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
-    case 32: return launch_batch_cfg<BS, 32, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
-    default: return launch_batch_cfg<BS, 16, KIND>(bt, q, flip, out_idx, out_dist, n_cu, st);
+    case 8: return launch_batch_cfg<BS, 8, KIND, SEEDED>(bt, q, flip, seed, out_idx, out_dist, n_cu, st);
+    case 24: return launch_batch_cfg<BS, 24, KIND, SEEDED>(bt, q, flip, seed, out_idx, out_dist, n_cu, st);
+    case 32: return launch_batch_cfg<BS, 32, KIND, SEEDED>(bt, q, flip, seed, out_idx, out_dist, n_cu, st);
+    default: return launch_batch_cfg<BS, 16, KIND, SEEDED>(bt, q, flip, seed, out_idx, out_dist, n_cu, st);
     }
 }
 
-template <int KIND>
+template <int KIND, bool SEEDED>
 static hipError_t launch_batch_kind(const BatchTable& bt, uint32_t bs, uint32_t ssz, const float* q, int flip,
-                                    uint32_t* out_idx, float* out_dist, uint32_t n_cu, hipStream_t st)
+                                    const float* seed, uint32_t* out_idx, float* out_dist, uint32_t n_cu,
+                                    hipStream_t st)
 {
     switch (bs) {
-    case 64: return launch_batch_bs<64, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
-    case 128: return launch_batch_bs<128, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
-    case 512: return launch_batch_bs<512, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
-    default: return launch_batch_bs<256, KIND>(bt, ssz, q, flip, out_idx, out_dist, n_cu, st);
+    case 64: return launch_batch_bs<64, KIND, SEEDED>(bt, ssz, q, flip, seed, out_idx, out_dist, n_cu, st);
+    case 128: return launch_batch_bs<128, KIND, SEEDED>(bt, ssz, q, flip, seed, out_idx, out_dist, n_cu, st);
+    case 512: return launch_batch_bs<512, KIND, SEEDED>(bt, ssz, q, flip, seed, out_idx, out_dist, n_cu, st);
+    default: return launch_batch_bs<256, KIND, SEEDED>(bt, ssz, q, flip, seed, out_idx, out_dist, n_cu, st);
     }
 }
 
 hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uint32_t* q_off_host, int flip,
-                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st)
+                        uint32_t* out_idx, float* out_dist, const BatchConfig& cfg, hipStream_t st,
+                        const float* seed)
 {
     const uint32_t M = s->n_meshes;
     BatchState& b = s->batch;
@@ -1590,8 +1600,13 @@ hipError_t launch_batch(const snch_scene* s, int kind, const float* q, const uin
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, bt.counter);
     const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
     if (kind == 0)
-        return launch_batch_kind<0>(bt, cfg.block_size, cfg.stack_depth, q, flip, out_idx, out_dist, n_cu, st);
-    return launch_batch_kind<1>(bt, cfg.block_size, cfg.stack_depth, q, flip, out_idx, out_dist, n_cu, st);
+        return launch_batch_kind<0, false>(bt, cfg.block_size, cfg.stack_depth, q, flip, nullptr, out_idx, out_dist,
+                                           n_cu, st);
+    if (seed)
+        return launch_batch_kind<1, true>(bt, cfg.block_size, cfg.stack_depth, q, flip, seed, out_idx, out_dist, n_cu,
+                                          st);
+    return launch_batch_kind<1, false>(bt, cfg.block_size, cfg.stack_depth, q, flip, nullptr, out_idx, out_dist, n_cu,
+                                       st);
 }
 
 } // namespace snch

@@ -145,3 +145,24 @@ def test_batch_errors(S):
         b.closest_silhouette(q, np.array([0, 10, 30], np.uint32))
     with pytest.raises(ValueError):
         b.autotune(q, np.array([1, 10, 20], np.uint32))
+
+
+def test_batch_seeded_identical(S):
+    """Seeded batched silhouette queries (exact, loose, too small, none, NaN)
+    equal the unseeded ones."""
+    import torch
+
+    meshes = _meshes()
+    b = S.BatchScene(meshes).build_bvh()
+    counts = [3000, 500, 70, 900, 130, 10, 2000]
+    q, q_off = _queries(meshes, counts, 77)
+    e0, d0 = b.closest_silhouette(q, q_off)
+    rng = np.random.default_rng(3)
+    d = d0.cpu().numpy()
+    fin = np.isfinite(d)
+    for seed in (np.where(fin, d, 1.0), np.where(fin, d * 1.05, 0.1), np.zeros(len(q)), rng.random(len(q)) * 0.5,
+                 np.full(len(q), np.nan)):
+        e1, d1 = b.closest_silhouette(q, q_off, seed_radius=seed.astype(np.float32))
+        assert torch.equal(e1, e0) and torch.equal(d1, d0)
+        e2, d2 = b.closest_silhouette(q, q_off, seed_radius=seed.astype(np.float32), tune={"block_size": 256})
+        assert torch.equal(e2, e0) and torch.equal(d2, d0)

@@ -166,6 +166,12 @@ def test_c3_batched(S, oracle):
     ci, cd = (t.cpu().numpy() for t in b.closest_point(qd, q_off))
     se, sd = (t.cpu().numpy() for t in b.closest_silhouette(qd, q_off))
     ci, se = ci.view(np.uint32), se.view(np.uint32)
+    # the learned-bound form of the batch (a seed radius per query) changes no result
+    sdd = torch.from_numpy(sd).cuda()
+    seed = torch.where(torch.isfinite(sdd), sdd * 1.1, torch.zeros_like(sdd))
+    e2, d2 = (t.cpu().numpy() for t in b.closest_silhouette(qd, q_off, seed_radius=seed))
+    np.testing.assert_array_equal(e2.view(np.uint32), se)
+    np.testing.assert_array_equal(d2, sd)
     rng = np.random.default_rng(5)
     for m, (V, F) in enumerate(meshes):
         a, z = int(q_off[m]), int(q_off[m + 1])
