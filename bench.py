@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 15)
     ap.add_argument("--build-reps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-build-scaling", action="store_true",
+                    help="skip the builds of the 64-mesh batch and of an 8.4M-triangle torus")
     ap.add_argument("--input-order", action="store_true", help="also time the kernel over unordered queries")
     ap.add_argument("--sweep", action="store_true", help="also time block size x stack depth launch shapes")
     ap.add_argument("--workload", default="silhouette", choices=["silhouette", "small", "batched", "wost", "mlp"],
@@ -439,6 +441,29 @@ def build_phase_roofline(sc, n_prims, reps):
     return out
 
 
+def build_scaling(reps):
+    """Builds/s and the phase roofline where launch latency weighs less: the
+    64-mesh batch (4.76M triangles, one build) and an 8.4M-triangle torus."""
+    import snch_lbvh_amd as S
+    from snch_lbvh_amd import workloads as W
+
+    out = {}
+    meshes, _, _ = W.batched_meshes()
+    V8, F8 = W.cut_torus(2896, 1448, seed=3)
+    for name, sc, n in (("batch_64_meshes", S.BatchScene(meshes), int(sum(len(f) for _, f in meshes))),
+                        ("torus_8M", S.Scene(V8, F8), int(F8.shape[0]))):
+        ms = []
+        for _ in range(reps + 1):
+            sc.build_bvh()
+            ms.append(sc.build_ms())
+        build_ms = float(np.median(ms[1:]))
+        out[name] = {"triangles": n, "build_ms": build_ms, "builds_per_s": 1e3 / build_ms,
+                     "whole_build_frac": sum(BUILD_PHASE_BYTES.values()) * n / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "phases": build_phase_roofline(sc, n, reps)}
+        del sc
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -468,6 +493,7 @@ def main():
         build_ms.append(sc.build_ms())
     build_ms = float(np.median(build_ms[1:]))
     build_roofline = build_phase_roofline(sc, F.shape[0], args.build_reps)
+    build_larger = None if args.no_build_scaling else build_scaling(args.build_reps)
 
     q = torch.from_numpy(q_np).to(dev)
     n = q.shape[0]
@@ -684,6 +710,7 @@ def main():
         "builds_per_s": 1000.0 / build_ms,
         "build_ms": build_ms,
         "build_roofline": build_roofline,
+        "build_larger": build_larger,
         "scene_create_s": t_scene,
         "learned_bound": learned,
         **extra,

@@ -434,8 +434,9 @@ class BatchScene:
             _lib.snch_scene_destroy(h)
             self._h = None
 
-    def build_bvh(self, reference_radius=False, leaf_size=1):
-        opts = BuildOpts(BUILD_REFERENCE_RADIUS if reference_radius else 0, leaf_size)
+    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False):
+        flags = (BUILD_REFERENCE_RADIUS if reference_radius else 0) | (BUILD_TIME_PHASES if time_phases else 0)
+        opts = BuildOpts(flags, leaf_size)
         check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
         return self
 
@@ -443,6 +444,12 @@ class BatchScene:
         ms = ctypes.c_float()
         check(lib().snch_scene_build_time(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def build_phases(self):
+        """GPU milliseconds of the last build's phases (see Scene.build_phases)."""
+        ms = (ctypes.c_float * 4)()
+        check(lib().snch_scene_build_phases(self._h, ms))
+        return {"leaves_ms": ms[0], "sort_ms": ms[1], "karras_ms": ms[2], "refit_ms": ms[3]}
 
     @staticmethod
     def _qcat(queries):
