@@ -46,6 +46,12 @@ typedef struct snch_scene snch_scene;
                                           true nearest silhouette */
 #define SNCH_BUILD_TIME_PHASES 2u      /* record GPU events between the build's phases (read them
                                           with snch_scene_build_phases); the tree is unchanged */
+/* Layout of the refit above 16384-leaf subtrees (one workgroup): by default
+ * the nodes' boxes sit in LDS while they fit, their topology while it fits,
+ * else the merge climbs with arrival counters in memory.  These two force the
+ * second or third layout at any size (for tests); the tree is unchanged. */
+#define SNCH_BUILD_TOPS_IN_MEMORY 4u
+#define SNCH_BUILD_TOPS_CLIMB 8u
 
 typedef struct snch_build_opts {
     uint32_t flags;     /* SNCH_BUILD_* */

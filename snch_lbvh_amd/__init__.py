@@ -26,6 +26,9 @@ SNCH_OK = 0
 ERRORS = {1: "invalid argument", 2: "empty mesh", 3: "HIP error", 4: "not built", 5: "stack overflow"}
 BUILD_REFERENCE_RADIUS = 1
 BUILD_TIME_PHASES = 2
+BUILD_TOPS_IN_MEMORY = 4  # layouts of the refit above 16384-leaf subtrees, forced for tests
+BUILD_TOPS_CLIMB = 8
+_TOP_LAYOUT = {"auto": 0, "memory": BUILD_TOPS_IN_MEMORY, "climb": BUILD_TOPS_CLIMB}
 
 
 class SnchError(RuntimeError):
@@ -233,10 +236,13 @@ class Scene:
     def compute_silhouettes(self):
         return self
 
-    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False):
+    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False, top_layout="auto"):
         """scene<3>::build_bvh (scene.cuh:1205-1229) -> bvh::construct on the GPU.
-        time_phases: also record the GPU time of each build phase (build_phases)."""
+        time_phases: also record the GPU time of each build phase (build_phases).
+        top_layout: "auto", or "memory" / "climb" to force the layout of the
+        refit above 16384-leaf subtrees (the tree is the same)."""
         flags = (BUILD_REFERENCE_RADIUS if reference_radius else 0) | (BUILD_TIME_PHASES if time_phases else 0)
+        flags |= _TOP_LAYOUT[top_layout]
         opts = BuildOpts(flags, leaf_size)
         check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
         self.built = True
@@ -434,8 +440,9 @@ class BatchScene:
             _lib.snch_scene_destroy(h)
             self._h = None
 
-    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False):
+    def build_bvh(self, reference_radius=False, leaf_size=1, time_phases=False, top_layout="auto"):
         flags = (BUILD_REFERENCE_RADIUS if reference_radius else 0) | (BUILD_TIME_PHASES if time_phases else 0)
+        flags |= _TOP_LAYOUT[top_layout]
         opts = BuildOpts(flags, leaf_size)
         check(lib().snch_scene_build(self._h, ctypes.byref(opts), _stream()))
         return self

@@ -78,10 +78,10 @@ __device__ __forceinline__ uint32_t mesh_search(const MeshDesc* __restrict__ d, 
 }
 
 // scene-box accumulators (+inf, -inf) of every mesh, 8 ints per mesh
-// (and clears the two list counters k_morton and k_karras append to, and
-// the radix sort's digit totals and tile counters)
-__global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32_t* list_b, uint32_t* sort_words,
-                                 uint32_t n_sort_words)
+// (and clears the list counters k_morton and k_karras append to, and the
+// radix sort's digit totals and tile counters)
+__global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32_t* list_b, uint32_t* list_c,
+                                 uint32_t* list_d, uint32_t* sort_words, uint32_t n_sort_words)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) // the radix sort's digit totals and tile counters
@@ -91,6 +91,8 @@ __global__ void k_init_scene_box(int* bits, uint32_t M, uint32_t* list_a, uint32
         list_a[0] = 0u;
         list_a[1] = 0u; // k_refit's next cut
         *list_b = 0u;
+        *list_c = 0u;
+        *list_d = 0u;
     }
     if (t >= 8 * M)
         return;
@@ -625,20 +627,23 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes_all, co
     return 32 + __clz((idx_all[off + a] - off) ^ (idx_all[off + b] - off));
 }
 
-constexpr uint32_t RT = 1024; // leaves of one refit subtree ("cut")
-constexpr uint32_t RB = 512;  // threads of one k_refit workgroup (RT / RB leaves each)
+constexpr uint32_t RT = 1024;  // leaves of one refit subtree ("cut")
+constexpr uint32_t RB = 512;   // threads of one k_refit workgroup (RT / RB leaves each)
+constexpr uint32_t RM = 16384; // leaves of a mid set's subtree (the tops below it: k_refit_mid)
 
 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
 // for one internal node: children, leaf range and parent links; the cuts it
-// roots (maximal subtrees of at most RT leaves below a node of more) and
-// whether it is a top (a node of more than RT leaves)
+// roots (maximal subtrees of at most RT leaves below a node of more), the mid
+// sets it roots (maximal subtrees of at most RM leaves below a node of more,
+// when they hold more than RT), and its class: an upper (more than RM
+// leaves: `top`) or a mid-set member (more than RT: `midtop`)
 __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_all,
                                             const uint32_t* __restrict__ idx_all, const CodeWindow& w, uint32_t t,
                                             const MeshDesc* __restrict__ desc, uint32_t M,
                                             uint32_t* __restrict__ parent, uint2* __restrict__ child,
                                             uint2* __restrict__ range, uint32_t& n_cut, uint32_t* cut,
-                                            uint32_t& top)
+                                            uint32_t& top, uint32_t& n_mid, uint32_t* mid, uint32_t& midtop)
 {
     const uint32_t m = mesh_search<false>(desc, M, t);
     const MeshDesc D = desc[m];
@@ -696,11 +701,24 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
     range[self] = make_uint2(off + first, off + last);
     parent[left] = self;
     parent[right] = self;
-    if (last - first + 1 > RT) {
+    const uint32_t size = last - first + 1, lsize = g - first + 1, rsize = last - g;
+    if (size > RM) {
         top = self;
-        if (g - first + 1 <= RT)
+        if (lsize <= RT)
             cut[n_cut++] = left;
-        if (last - g <= RT)
+        else if (lsize <= RM)
+            mid[n_mid++] = left;
+        if (rsize <= RT)
+            cut[n_cut++] = right;
+        else if (rsize <= RM)
+            mid[n_mid++] = right;
+    } else if (size > RT) {
+        midtop = self;
+        if (i == 0)
+            mid[n_mid++] = self;
+        if (lsize <= RT)
+            cut[n_cut++] = left;
+        if (rsize <= RT)
             cut[n_cut++] = right;
     } else if (i == 0) {
         cut[n_cut++] = self;
@@ -709,19 +727,25 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
 
 // One thread per internal node of every mesh (internal-record slot t);
 // nodes are written with global (mesh-major) ids.  Lists the refit subtrees
-// ("cuts") and the nodes above them ("tops"); flags[top] = its list slot.
+// ("cuts"), the uppers ("tops", flags[top] = its list slot), the mid-set
+// roots ("mids", flags[root] = its slot, its member count cleared) and the
+// mid-set members ("midtops").  mids[mid_cap] counts the roots and
+// mids[mid_cap + 1 + c] the members of set c.
 __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ codes_all,
                                                const uint32_t* __restrict__ idx_all, uint32_t n_internal,
                                                uint32_t n_all, const MeshDesc* __restrict__ desc, uint32_t M,
                                                uint32_t* __restrict__ parent, uint2* __restrict__ child,
                                                uint2* __restrict__ range, uint32_t* __restrict__ flags,
                                                uint32_t* __restrict__ cuts, uint32_t cut_cap,
-                                               uint32_t* __restrict__ tops, uint32_t top_cap)
+                                               uint32_t* __restrict__ tops, uint32_t top_cap,
+                                               uint32_t* __restrict__ mids, uint32_t mid_cap,
+                                               uint32_t* __restrict__ midtops)
 {
     static_assert(BT == 256, "the code window spans BT nodes");
-    __shared__ uint32_t s_n[2], s_base[2]; // cuts and tops of this block: counts, then list offsets
+    // cuts, tops, mids and midtops of this block: counts, then list offsets
+    __shared__ uint32_t s_n[4], s_base[4];
     __shared__ uint32_t win[KWL];
-    if (threadIdx.x < 2)
+    if (threadIdx.x < 4)
         s_n[threadIdx.x] = 0u;
     // internal-record slot t of mesh m is the node at sorted position t + m
     const uint32_t t0 = blockIdx.x * BT;
@@ -734,17 +758,37 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     const CodeWindow w{win, lo};
     const uint32_t t = t0 + threadIdx.x;
     uint32_t n_cut = 0, cut[2] = {INVALID, INVALID}, self = INVALID; // self: set for a top
+    uint32_t n_mid = 0, mid[2] = {INVALID, INVALID}, midtop = INVALID;
     if (t < n_internal)
-        karras_node(codes_all, idx_all, w, t, desc, M, parent, child, range, n_cut, cut, self);
+        karras_node(codes_all, idx_all, w, t, desc, M, parent, child, range, n_cut, cut, self, n_mid, mid, midtop);
     // one list append per block and list (the counters are shared by all blocks)
     const uint32_t cut_k = n_cut ? atomicAdd(&s_n[0], n_cut) : 0u;
     const uint32_t top_k = self != INVALID ? atomicAdd(&s_n[1], 1u) : 0u;
+    const uint32_t mid_k = n_mid ? atomicAdd(&s_n[2], n_mid) : 0u;
+    const uint32_t mt_k = midtop != INVALID ? atomicAdd(&s_n[3], 1u) : 0u;
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 4) {
         const uint32_t n = s_n[threadIdx.x];
-        s_base[threadIdx.x] = n ? atomicAdd(threadIdx.x == 0 ? cuts + cut_cap : tops + top_cap, n) : 0u;
+        uint32_t* ctr = threadIdx.x == 0 ? cuts + cut_cap
+                        : threadIdx.x == 1 ? tops + top_cap
+                        : threadIdx.x == 2 ? mids + mid_cap
+                                           : midtops + top_cap;
+        s_base[threadIdx.x] = n ? atomicAdd(ctr, n) : 0u;
     }
     __syncthreads();
+    for (uint32_t j = 0; j < n_mid; ++j) {
+        const uint32_t k = s_base[2] + mid_k + j;
+        if (k < mid_cap) {
+            mids[k] = mid[j];
+            mids[mid_cap + 1 + k] = 0u;
+            flags[mid[j]] = k;
+        }
+    }
+    if (midtop != INVALID) {
+        const uint32_t k = s_base[3] + mt_k;
+        if (k < top_cap)
+            midtops[k] = midtop;
+    }
     for (uint32_t j = 0; j < n_cut; ++j) {
         const uint32_t k = s_base[0] + cut_k + j;
         if (k < cut_cap)
@@ -1080,226 +1124,307 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
     }
 }
 
-// k_refit_top keeps, when the scene has at most TOP_BOX cuts and tops, the
-// box and cone of every live node in LDS (slot: the cut's index; a top takes
-// over its left child's slot) with each top's node, mesh, parent slot and
-// arrival count, children and children's slots and a queue entry, 26 B per
-// top, so a round reads nothing from memory but the mesh's offsets.  Up to
-// TOP_LDS tops, the topology alone (14 B per top) sits in LDS and the boxes
-// are read from memory.  Beyond that it climbs from the cut roots with
-// arrival counters in memory.
-constexpr uint32_t TOP_BOX = 2048;
-constexpr uint32_t TOP_LDS = 11264;
-constexpr uint32_t TOP_ROUNDS = 128; // a Karras tree over 64-bit keys is at most 64 levels deep
-constexpr uint32_t TOP_SMEM = TOP_LDS * 14;
-static_assert(TOP_BOX * (44 + 26) <= TOP_SMEM, "LDS layouts of k_refit_top");
+// Above the cuts, the tops are refit as "sets": a set is a group of tops
+// merged by one workgroup, with the topology of its members in LDS.  A
+// member's children are members (merged in an earlier round of the same
+// workgroup) or nodes whose box and cone are already in memory (nbox/ncone:
+// cut roots written by k_refit, mid-set roots written by k_refit_mid).
+//
+//   mid sets  the tops of one subtree of at most RM leaves whose parent has
+//             more (or a mesh root of at most RM leaves): k_refit_mid, one
+//             workgroup per set, many sets at once;
+//   uppers    the tops of more than RM leaves: k_refit_top, one workgroup.
+//
+// Size of a mid set: the common prefix of a node's 64-bit keys grows by at
+// least one bit from a node to its children, so a root-to-leaf path holds at
+// most 64 internal nodes; the members without member children own disjoint
+// subtrees of more than RT leaves, at most RM / (RT + 1) = 15 of them, and
+// every member lies on a path from the set root to one of them: at most
+// 15 * 64 = 960 members, so MID_CAP members always fit.
+constexpr uint32_t MID_CAP = 1024;
+constexpr uint32_t MB = 256;              // threads of one k_refit_mid workgroup
+constexpr uint32_t NO_MEMBER = 0xFFFFu;
+constexpr uint32_t SET_ROUNDS = 128;      // a set is at most 64 levels deep
+static_assert(RM / (RT + 1) * 64 <= MID_CAP, "mid-set bound");
 
-// topology of the tops: tword[k] = parent slot << 16 (0xFFFF: a mesh root),
-// tch[k] = children, tnode_id[k] (optional) = the node
-__device__ __forceinline__ void load_tops(const uint32_t* __restrict__ parent, const uint2* __restrict__ child,
-                                          const uint32_t* flags, const uint32_t* __restrict__ tops, uint32_t n_top,
-                                          uint32_t n_nodes, uint32_t* tword, uint2* tch, uint32_t* tnode_id = nullptr)
+// LDS arrays of a set, `cap` entries each (box: 11 rows, boxes in LDS only)
+struct SetLds {
+    uint32_t* node;  // member's node id
+    uint2* ch;       // its children's node ids
+    uint32_t* cidx;  // children's member indices, left | right << 16 (NO_MEMBER: in memory)
+    uint32_t* word;  // parent's member index << 16 | member children not merged yet
+    uint32_t* mesh;  // its mesh
+    uint16_t* queue;
+    float* box;
+    uint32_t cap;
+    __device__ __forceinline__ void put(uint32_t k, const box3& b, const cone3& c) const
+    {
+        float* v = box + k;
+        v[0 * cap] = b.lo.x; v[1 * cap] = b.lo.y; v[2 * cap] = b.lo.z;
+        v[3 * cap] = b.hi.x; v[4 * cap] = b.hi.y; v[5 * cap] = b.hi.z;
+        v[6 * cap] = c.axis.x; v[7 * cap] = c.axis.y; v[8 * cap] = c.axis.z;
+        v[9 * cap] = c.half_angle; v[10 * cap] = c.radius;
+    }
+    __device__ __forceinline__ void get(uint32_t k, box3& b, cone3& c) const
+    {
+        const float* v = box + k;
+        b.lo = mk(v[0 * cap], v[1 * cap], v[2 * cap]);
+        b.hi = mk(v[3 * cap], v[4 * cap], v[5 * cap]);
+        c.axis = mk(v[6 * cap], v[7 * cap], v[8 * cap]);
+        c.half_angle = v[9 * cap];
+        c.radius = v[10 * cap];
+    }
+};
+
+__device__ __forceinline__ uint32_t range_size(const uint2* __restrict__ range, uint32_t node)
 {
-    for (uint32_t base = threadIdx.x; base < n_top; base += 4 * 1024) {
-        uint32_t p[4], pp[4];
-        uint2 ch[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t k = base + 1024u * u;
-            p[u] = k < n_top ? tops[k] : INVALID;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            pp[u] = p[u] < n_nodes ? parent[p[u]] : INVALID;
-            ch[u] = p[u] < n_nodes ? child[p[u]] : make_uint2(INVALID, INVALID);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t k = base + 1024u * u;
-            if (k < n_top) {
-                const uint32_t ps = pp[u] < n_nodes ? flags[pp[u]] : 0xFFFFu;
-                tword[k] = min(ps, 0xFFFFu) << 16;
-                tch[k] = ch[u];
-                if (tnode_id)
-                    tnode_id[k] = p[u];
+    const uint2 rg = range[node];
+    return rg.y - rg.x + 1;
+}
+
+// Refit the n members of a set, listed in s.node with flags[node] = member
+// index (written before the call and ordered by a barrier).  A child is a
+// member when it is internal with more than lo_thr leaves; a parent when it
+// is internal with at most hi_thr leaves.  NT threads; rq cleared.
+template <int DIM, bool BOX_LDS, uint32_t NT>
+__device__ void refit_set(const SetLds& s, uint32_t n, uint32_t lo_thr, uint32_t hi_thr, RoundQueue& rq,
+                          const uint32_t* __restrict__ parent, const uint2* __restrict__ child,
+                          const uint2* __restrict__ range, const uint32_t* flags, const MeshDesc* __restrict__ desc,
+                          uint32_t M, int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
+                          float4* __restrict__ tnode, float4* __restrict__ tcone)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = t; k < n; k += NT) {
+        const uint32_t p = s.node[k];
+        const uint32_t m = mesh_search<true>(desc, M, p);
+        const MeshDesc D = desc[m];
+        const MeshIds id{D.off, D.n, D.nb};
+        const uint2 ch = child[p];
+        const bool lm = id.internal(ch.x) && range_size(range, ch.x) > lo_thr;
+        const bool rm = id.internal(ch.y) && range_size(range, ch.y) > lo_thr;
+        const uint32_t li = lm ? flags[ch.x] & 0xFFFFu : NO_MEMBER;
+        const uint32_t ri = rm ? flags[ch.y] & 0xFFFFu : NO_MEMBER;
+        const uint32_t par = parent[p];
+        const bool pm = par != INVALID && id.internal(par) && range_size(range, par) <= hi_thr;
+        s.ch[k] = ch;
+        s.cidx[k] = li | (ri << 16);
+        s.word[k] = ((pm ? flags[par] & 0xFFFFu : NO_MEMBER) << 16) | ((lm ? 1u : 0u) + (rm ? 1u : 0u));
+        s.mesh[k] = m;
+    }
+    __syncthreads();
+    // each member is queued once (when its last member child is merged), so
+    // the queue holds at most n entries; the bounds below only guard LDS
+    // against an inconsistent tree
+    auto push = [&](uint32_t qe, uint32_t round, uint32_t k) {
+        const uint32_t at = qe + atomicAdd(&rq.n[round % 3], 1u);
+        if (at < n)
+            s.queue[at] = (uint16_t)k;
+    };
+    for (uint32_t k = t; k < n; k += NT)
+        if ((s.word[k] & 0xFFFFu) == 0u)
+            push(0u, 0u, k);
+    __syncthreads();
+    uint32_t qs = 0, qe = min(rq.n[0], n);
+    for (uint32_t round = 1; qs < qe && round < SET_ROUNDS; ++round) {
+        if (t == 0)
+            rq.n[(round + 1) % 3] = 0u;
+        for (uint32_t i = qs + t; i < qe; i += NT) {
+            const uint32_t k = s.queue[i];
+            const uint32_t p = s.node[k];
+            const uint2 ch = s.ch[k];
+            const uint32_t ci = s.cidx[k];
+            const MeshDesc D = desc[min(s.mesh[k], M - 1)];
+            const MeshIds id{D.off, D.n, D.nb};
+            box3 lb, rb, pb;
+            cone3 lc, rc, pc;
+            if (BOX_LDS && (ci & 0xFFFFu) != NO_MEMBER) {
+                s.get(min(ci & 0xFFFFu, s.cap - 1), lb, lc);
+            } else {
+                lb = ldbox(nbox, ch.x);
+                lc = ldcone(ncone, ch.x);
             }
+            if (BOX_LDS && (ci >> 16) != NO_MEMBER) {
+                s.get(min(ci >> 16, s.cap - 1), rb, rc);
+            } else {
+                rb = ldbox(nbox, ch.y);
+                rc = ldcone(ncone, ch.y);
+            }
+            refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone, tnode,
+                             tcone);
+            if (BOX_LDS)
+                s.put(k, pb, pc);
+            const uint32_t pk = s.word[k] >> 16;
+            if (pk < n && (atomicSub(&s.word[pk], 1u) & 0xFFFFu) == 1u)
+                push(qe, round, pk);
+        }
+        __syncthreads(); // this round's boxes (LDS or memory) precede the next round's reads
+        const uint32_t added = rq.n[round % 3];
+        qs = qe;
+        qe = min(qe + added, n);
+    }
+    // the sin and cos of the children's half angles, once every record is written
+    for (uint32_t k = t; k < n; k += NT)
+        tcone_trig(tcone, s.node[k] - desc[min(s.mesh[k], M - 1)].off);
+}
+
+// Members of the mid sets: each top of at most RM leaves climbs to the root
+// of its set (the last ancestor of at most RM leaves; flags[root] = the set's
+// slot, written by k_karras) and appends itself to the set's member list.
+__global__ __launch_bounds__(BT) void k_mid_members(const uint32_t* __restrict__ midtops, uint32_t midtop_cap,
+                                                    const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ range,
+                                                    const uint32_t* __restrict__ flags,
+                                                    uint32_t* __restrict__ mids, uint32_t mid_cap,
+                                                    uint32_t* __restrict__ members)
+{
+    const uint32_t n = min(midtops[midtop_cap], midtop_cap);
+    const uint32_t n_mid = min(mids[mid_cap], mid_cap);
+    uint32_t* mcount = mids + mid_cap + 1;
+    for (uint32_t k = blockIdx.x * BT + threadIdx.x; k < n; k += gridDim.x * BT) {
+        const uint32_t t = midtops[k];
+        uint32_t node = t;
+        for (uint32_t guard = 0; guard < 64; ++guard) {
+            const uint32_t p = parent[node];
+            if (p == INVALID || range_size(range, p) > RM)
+                break;
+            node = p;
+        }
+        const uint32_t c = flags[node];
+        if (c < n_mid) {
+            const uint32_t j = atomicAdd(mcount + c, 1u);
+            if (j < MID_CAP)
+                members[c * MID_CAP + j] = t;
         }
     }
 }
+
+// One workgroup per mid set (a grid-stride loop over the sets), boxes in LDS.
+template <int DIM>
+__global__ __launch_bounds__(MB) void k_refit_mid(const uint32_t* __restrict__ parent,
+                                                  const uint2* __restrict__ child, const uint2* __restrict__ range,
+                                                  uint32_t* flags, const uint32_t* __restrict__ mids,
+                                                  uint32_t mid_cap, const uint32_t* __restrict__ members,
+                                                  const MeshDesc* __restrict__ desc, uint32_t M, int conservative,
+                                                  uint32_t leaf_size, float4* nbox, float4* ncone,
+                                                  float4* __restrict__ tnode, float4* __restrict__ tcone)
+{
+    __shared__ uint32_t node[MID_CAP], cidx[MID_CAP], word[MID_CAP], mesh[MID_CAP];
+    __shared__ uint2 ch[MID_CAP];
+    __shared__ uint16_t queue[MID_CAP];
+    __shared__ float box[11 * MID_CAP];
+    __shared__ RoundQueue rq;
+    const SetLds s{node, ch, cidx, word, mesh, queue, box, MID_CAP};
+    const uint32_t t = threadIdx.x;
+    const uint32_t n_mid = min(mids[mid_cap], mid_cap);
+    const uint32_t* mcount = mids + mid_cap + 1;
+    for (uint32_t c = blockIdx.x; c < n_mid; c += gridDim.x) {
+        const uint32_t n = min(mcount[c], MID_CAP);
+        if (t < 3)
+            rq.n[t] = 0u;
+        for (uint32_t k = t; k < n; k += MB) {
+            const uint32_t p = members[c * MID_CAP + k];
+            node[k] = p;
+            flags[p] = k; // this workgroup's members only: read back after the barrier
+        }
+        __syncthreads();
+        refit_set<DIM, true, MB>(s, n, RT, RM, rq, parent, child, range, flags, desc, M, conservative, leaf_size,
+                                 nbox, ncone, tnode, tcone);
+        __syncthreads(); // the next set reuses the LDS
+    }
+}
+
+// The uppers (tops of more than RM leaves, flags[top] = its list slot from
+// k_karras) in one workgroup: boxes in LDS up to TOP_BOX members, topology
+// alone in LDS up to TOP_TOPO (boxes read back from memory), beyond that a
+// climb with arrival counters in memory.  force: 1 or 2 picks the second or
+// third layout whatever the count (they give the same records).
+constexpr uint32_t TOP_BOX = 2048;
+constexpr uint32_t TOP_TOPO = 6016;
+constexpr uint32_t TOP_SMEM = TOP_TOPO * 26;
+static_assert(TOP_BOX * (26 + 44) <= TOP_SMEM, "LDS layouts of k_refit_top");
 
 template <int DIM>
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ parent,
                                                     const uint2* __restrict__ child, const uint2* __restrict__ range,
                                                     uint32_t* flags, const uint32_t* __restrict__ tops,
                                                     uint32_t top_cap, const MeshDesc* __restrict__ desc, uint32_t M,
-                                                    uint32_t n_nodes, int conservative, uint32_t leaf_size,
-                                                    float4* nbox, float4* ncone, float4* __restrict__ tnode,
-                                                    float4* __restrict__ tcone, const uint32_t* __restrict__ cuts,
-                                                    uint32_t cut_cap)
+                                                    int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
+                                                    float4* __restrict__ tnode, float4* __restrict__ tcone,
+                                                    uint32_t force)
 {
     __shared__ __align__(16) unsigned char smem[TOP_SMEM];
     __shared__ RoundQueue rq;
     const uint32_t t = threadIdx.x;
-    const uint32_t n_top = min(tops[top_cap], top_cap);
-    const uint32_t n_cut = min(cuts[cut_cap], cut_cap);
+    const uint32_t n = min(tops[top_cap], top_cap);
     if (t < 3)
         rq.n[t] = 0u;
-    // once every top is merged: the sin and cos of its children's half angles
-    auto trig_tops = [&]() {
-        __syncthreads(); // the records' writes precede their reads
-        for (uint32_t k = t; k < n_top; k += 1024) {
-            const uint32_t p = tops[k];
-            if (p < n_nodes)
-                tcone_trig(tcone, p - desc[mesh_search<true>(desc, M, p)].off);
-        }
+    auto layout = [&](uint32_t cap, bool boxes) {
+        // 8-byte children first, then 4-byte arrays, the boxes, the 2-byte queue
+        SetLds s;
+        s.cap = cap;
+        s.ch = reinterpret_cast<uint2*>(smem);
+        s.node = reinterpret_cast<uint32_t*>(smem + 8 * cap);
+        s.cidx = s.node + cap;
+        s.word = s.cidx + cap;
+        s.mesh = s.word + cap;
+        s.box = reinterpret_cast<float*>(s.mesh + cap);
+        s.queue = reinterpret_cast<uint16_t*>(s.box + (boxes ? 11 * cap : 0));
+        return s;
     };
-    if (n_top <= TOP_BOX && n_cut <= TOP_BOX) {
-        float(*bx)[TOP_BOX] = reinterpret_cast<float(*)[TOP_BOX]>(smem); // 11 rows: box, cone
-        uint2* tch = reinterpret_cast<uint2*>(smem + 44 * TOP_BOX);
-        uint32_t* tword = reinterpret_cast<uint32_t*>(smem + 52 * TOP_BOX);
-        uint32_t* tslot = reinterpret_cast<uint32_t*>(smem + 56 * TOP_BOX); // children's slots, left | right << 16
-        uint16_t* queue = reinterpret_cast<uint16_t*>(smem + 60 * TOP_BOX);
-        uint32_t* tp = reinterpret_cast<uint32_t*>(smem + 62 * TOP_BOX); // node of each top
-        uint32_t* tm = reinterpret_cast<uint32_t*>(smem + 66 * TOP_BOX); // its mesh
-        auto put = [&](uint32_t sl, const box3& b, const cone3& c) {
-            bx[0][sl] = b.lo.x; bx[1][sl] = b.lo.y; bx[2][sl] = b.lo.z;
-            bx[3][sl] = b.hi.x; bx[4][sl] = b.hi.y; bx[5][sl] = b.hi.z;
-            bx[6][sl] = c.axis.x; bx[7][sl] = c.axis.y; bx[8][sl] = c.axis.z;
-            bx[9][sl] = c.half_angle; bx[10][sl] = c.radius;
-        };
-        auto get = [&](uint32_t sl, box3& b, cone3& c) {
-            b.lo = mk(bx[0][sl], bx[1][sl], bx[2][sl]);
-            b.hi = mk(bx[3][sl], bx[4][sl], bx[5][sl]);
-            c.axis = mk(bx[6][sl], bx[7][sl], bx[8][sl]);
-            c.half_angle = bx[9][sl];
-            c.radius = bx[10][sl];
-        };
-        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch, tp);
-        for (uint32_t k = t; k < n_top; k += 1024) {
-            tslot[k] = 0u;
-            tm[k] = mesh_search<true>(desc, M, tp[k]); // each thread reads back its own tp[k]
-        }
+    if ((n <= TOP_BOX && force == 0) || (n <= TOP_TOPO && force <= 1)) {
+        const bool boxes = n <= TOP_BOX && force == 0;
+        const SetLds s = layout(boxes ? TOP_BOX : TOP_TOPO, boxes);
+        for (uint32_t k = t; k < n; k += 1024)
+            s.node[k] = tops[k];
         __syncthreads();
-        // the cut roots (written by k_refit) take their slots and arrive
-        for (uint32_t c = t; c < n_cut; c += 1024) {
-            const uint32_t r = cuts[c];
-            if (r >= n_nodes)
-                continue;
-            put(c, ldbox(nbox, r), ldcone(ncone, r));
-            const uint32_t pp = parent[r];
-            const uint32_t kp = pp < n_nodes ? flags[pp] : INVALID;
-            if (kp >= n_top)
-                continue; // a mesh that is one cut
-            atomicOr(&tslot[kp], tch[kp].x == r ? c : c << 16);
-            if ((atomicAdd(&tword[kp], 1u) & 0xFFFFu) == 1u)
-                rq.push(queue, 0u, 0u, kp);
-        }
-        __syncthreads();
-        uint32_t qs = 0, qe = rq.n[0];
-        for (uint32_t round = 1; qs < qe && round < TOP_ROUNDS; ++round) {
-            if (t == 0)
-                rq.n[(round + 1) % 3] = 0u;
-            for (uint32_t i = qs + t; i < qe; i += 1024) {
-                const uint32_t k = queue[i];
-                const uint32_t p = tp[k];
-                const uint2 ch = tch[k];
-                const uint32_t ls = min(tslot[k] & 0xFFFFu, TOP_BOX - 1), rs = min(tslot[k] >> 16, TOP_BOX - 1);
-                const MeshDesc D = desc[min(tm[k], M - 1)];
-                const MeshIds id{D.off, D.n, D.nb};
-                box3 lb, rb, pb;
-                cone3 lc, rc, pc;
-                get(ls, lb, lc);
-                get(rs, rb, rc);
-                refit_write<DIM>(id, p, ch, lb, rb, lc, rc, conservative, range, leaf_size, pb, pc, nbox, ncone,
-                                 tnode, tcone);
-                put(ls, pb, pc);
-                const uint32_t kp = tword[k] >> 16;
-                if (kp < n_top) {
-                    atomicOr(&tslot[kp], tch[kp].x == p ? ls : ls << 16);
-                    if ((atomicAdd(&tword[kp], 1u) & 0xFFFFu) == 1u)
-                        rq.push(queue, qe, round, kp);
-                }
-            }
-            __syncthreads(); // this round's slots precede the next round's reads
-            const uint32_t added = rq.n[round % 3];
-            qs = qe;
-            qe += added;
-        }
-        trig_tops();
+        if (boxes)
+            refit_set<DIM, true, 1024>(s, n, RM, 0xFFFFFFFFu, rq, parent, child, range, flags, desc, M,
+                                       conservative, leaf_size, nbox, ncone, tnode, tcone);
+        else
+            refit_set<DIM, false, 1024>(s, n, RM, 0xFFFFFFFFu, rq, parent, child, range, flags, desc, M,
+                                        conservative, leaf_size, nbox, ncone, tnode, tcone);
         return;
     }
-    if (n_top <= TOP_LDS) {
-        // level by level with the boxes in memory: a round writes the tops
-        // whose top children are written (the cut roots below them were
-        // written by k_refit); tword counts the top children not yet written
-        uint32_t* tword = reinterpret_cast<uint32_t*>(smem);
-        uint2* tch = reinterpret_cast<uint2*>(smem + 4 * TOP_LDS);
-        uint16_t* queue = reinterpret_cast<uint16_t*>(smem + 12 * TOP_LDS);
-        load_tops(parent, child, flags, tops, n_top, n_nodes, tword, tch);
-        __syncthreads();
-        for (uint32_t k = t; k < n_top; k += 1024) {
-            const uint32_t ps = tword[k] >> 16;
-            if (ps < n_top)
-                atomicAdd(&tword[ps], 1u);
-        }
-        __syncthreads();
-        for (uint32_t k = t; k < n_top; k += 1024)
-            if ((tword[k] & 0xFFFFu) == 0u)
-                rq.push(queue, 0u, 0u, k);
-        __syncthreads();
-        uint32_t qs = 0, qe = rq.n[0];
-        for (uint32_t round = 1; qs < qe && round < TOP_ROUNDS; ++round) {
-            if (t == 0)
-                rq.n[(round + 1) % 3] = 0u;
-            for (uint32_t i = qs + t; i < qe; i += 1024) {
-                const uint32_t k = queue[i];
-                const uint32_t p = tops[k];
-                const uint2 ch = tch[k];
-                const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
-                const MeshIds id{D.off, D.n, D.nb};
-                box3 pb;
-                cone3 pc;
-                refit_write<DIM>(id, p, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
-                                 ldcone(ncone, ch.y), conservative, range, leaf_size, pb, pc, nbox, ncone, tnode,
-                                 tcone);
-                const uint32_t ps = tword[k] >> 16;
-                if (ps < n_top && (atomicSub(&tword[ps], 1u) & 0xFFFFu) == 1u)
-                    rq.push(queue, qe, round, ps);
-            }
-            __syncthreads(); // this round's node writes precede the next round's reads
-            const uint32_t added = rq.n[round % 3];
-            qs = qe;
-            qe += added;
-        }
-        trig_tops();
-        return;
+    // climb: flags[top] = its upper children not merged yet; a top with none
+    // starts a climb, and the thread that merges a top's last child goes on
+    // to merge the top
+    auto upper_children = [&](uint32_t p, const MeshIds& id) {
+        const uint2 ch = child[p];
+        return (id.internal(ch.x) && range_size(range, ch.x) > RM ? 1u : 0u) +
+               (id.internal(ch.y) && range_size(range, ch.y) > RM ? 1u : 0u);
+    };
+    for (uint32_t k = t; k < n; k += 1024) {
+        const uint32_t p = tops[k];
+        const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
+        flags[p] = upper_children(p, MeshIds{D.off, D.n, D.nb});
     }
-    // more tops than LDS holds: climb from the cut roots with arrival
-    // counters in memory
-    for (uint32_t k = t; k < n_top; k += 1024)
-        flags[tops[k]] = 0u;
     __syncthreads();
-    for (uint32_t k = t; k < n_cut; k += 1024) {
-        uint32_t node = cuts[k];
-        if (node >= n_nodes)
-            continue;
-        const MeshDesc D = desc[mesh_search<true>(desc, M, node)];
+    for (uint32_t k = t; k < n; k += 1024) {
+        uint32_t p = tops[k];
+        const MeshDesc D = desc[mesh_search<true>(desc, M, p)];
         const MeshIds id{D.off, D.n, D.nb};
-        uint32_t p = parent[node];
-        for (int guard = 0; p != INVALID && id.internal(p) && guard < 96; ++guard) {
-            const uint32_t old = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0u)
-                break;
+        if (upper_children(p, id) != 0u)
+            continue;
+        for (uint32_t guard = 0; guard < SET_ROUNDS; ++guard) {
             const uint2 ch = child[p];
             box3 pb;
             cone3 pc;
             refit_write<DIM>(id, p, ch, ldbox(nbox, ch.x), ldbox(nbox, ch.y), ldcone(ncone, ch.x),
                              ldcone(ncone, ch.y), conservative, range, leaf_size, pb, pc, nbox, ncone, tnode, tcone);
-            node = p;
-            p = parent[p];
+            const uint32_t par = parent[p];
+            if (par == INVALID || !id.internal(par))
+                break;
+            // release this top's records, acquire the sibling's
+            if (__hip_atomic_fetch_add(flags + par, 0xFFFFFFFFu, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
+                break;
+            p = par;
         }
     }
-    trig_tops();
+    __syncthreads(); // the records' writes precede their reads
+    for (uint32_t k = t; k < n; k += 1024) {
+        const uint32_t p = tops[k];
+        tcone_trig(tcone, p - desc[mesh_search<true>(desc, M, p)].off);
+    }
 }
 
 
@@ -1354,6 +1479,13 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     const uint32_t top_cap = NI > 0 ? NI : 1u; // tops are internal nodes
     SNCH_TRY(w.tops.ensure(sizeof(uint32_t) * (top_cap + 1)));
     uint32_t* tops = w.tops.as<uint32_t>();
+    SNCH_TRY(w.midtops.ensure(sizeof(uint32_t) * (top_cap + 1)));
+    uint32_t* midtops = w.midtops.as<uint32_t>();
+    // mid sets are disjoint subtrees of more than RT leaves
+    const uint32_t mid_cap = N / (RT + 1) + 1;
+    SNCH_TRY(w.mids.ensure(sizeof(uint32_t) * (2 * mid_cap + 1)));
+    uint32_t* mids = w.mids.as<uint32_t>();
+    SNCH_TRY(w.members.ensure(sizeof(uint32_t) * mid_cap * MID_CAP));
     SNCH_TRY(s->tnode.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tcone.ensure(sizeof(float4) * 4 * (NI > 0 ? NI : 1)));
     SNCH_TRY(s->tleaf.ensure(sizeof(float4) * 3 * N));
@@ -1369,7 +1501,8 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     SNCH_TRY(w.hist.ensure(radix_sort_scratch_bytes(N, 4 + (pmesh ? (mesh_bits + 7) / 8 : 0))));
     mark(0);
     hipLaunchKernelGGL(k_init_scene_box, dim3((8 * M + 255) / 256), dim3(256), 0, st, w.scene_bits.as<int>(), M,
-                       cuts + cut_cap, tops + top_cap, w.hist.as<uint32_t>(), (uint32_t)radix_sort_clear_words());
+                       cuts + cut_cap, tops + top_cap, mids + mid_cap, midtops + top_cap, w.hist.as<uint32_t>(),
+                       (uint32_t)radix_sort_clear_words());
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
@@ -1402,14 +1535,21 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     if (NI > 0)
         hipLaunchKernelGGL(k_karras, dim3(blocks(NI)), dim3(BT), 0, st, codes, sidx, NI, N, desc, M,
                            w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(),
-                           cuts, cut_cap, tops, top_cap);
+                           cuts, cut_cap, tops, top_cap, mids, mid_cap, midtops);
     // about 2N / RT cuts (each holds at least RT / 2 leaves but for one per
     // side of a large node, plus one per small mesh); workgroups loop over them
     const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
-    bool any_big = false;
-    for (uint32_t m = 0; m < M && !any_big; ++m)
-        any_big = s->mesh_off[m + 1] - s->mesh_off[m] > RT;
+    bool any_big = false, any_huge = false; // a mesh of more than RT (RM) leaves
+    for (uint32_t m = 0; m < M; ++m) {
+        any_big = any_big || s->mesh_off[m + 1] - s->mesh_off[m] > RT;
+        any_huge = any_huge || s->mesh_off[m + 1] - s->mesh_off[m] > RM;
+    }
+    const uint32_t top_layout = (flags & SNCH_BUILD_TOPS_CLIMB) ? 2u : (flags & SNCH_BUILD_TOPS_IN_MEMORY) ? 1u : 0u;
     mark(3);
+    if (any_big)
+        hipLaunchKernelGGL(k_mid_members, dim3(std::min(blocks(top_cap), 4u * n_cu)), dim3(BT), 0, st, midtops,
+                           top_cap, w.parent.as<uint32_t>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), mids,
+                           mid_cap, w.members.as<uint32_t>());
     SNCH_DISPATCH_REFIT(s->dim, {
         // as many workgroups as are resident at once (LDS-bound), or one per cut
         static int per_cu = 0;
@@ -1424,11 +1564,16 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                            desc, M, N, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
                            s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
         if (any_big)
+            hipLaunchKernelGGL(k_refit_mid<DIM>, dim3(std::min(mid_cap, 2u * n_cu)), dim3(MB), 0, st,
+                               w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
+                               w.flags.as<uint32_t>(), mids, mid_cap, w.members.as<uint32_t>(), desc, M,
+                               conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
+                               s->tnode.as<float4>(), s->tcone.as<float4>());
+        if (any_huge)
             hipLaunchKernelGGL(k_refit_top<DIM>, dim3(1), dim3(1024), 0, st, w.parent.as<uint32_t>(),
                                w.child.as<uint2>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), tops, top_cap,
-                               desc, M, NN,
-                               conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
-                               s->tnode.as<float4>(), s->tcone.as<float4>(), cuts, cut_cap);
+                               desc, M, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
+                               s->tnode.as<float4>(), s->tcone.as<float4>(), top_layout);
     });
     mark(4);
     s->sorted_in_alt = in_alt;

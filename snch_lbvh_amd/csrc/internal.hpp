@@ -103,7 +103,7 @@ struct BatchConfig {
 
 struct BuildScratch {
     DeviceBuffer leafbox, leafcone, scene_bits, boxpart, codes, idx, codes_alt, idx_alt, hist, parent, child, range,
-        flags, top, tops, status;
+        flags, top, tops, status, mids, midtops, members;
 };
 
 } // namespace snch

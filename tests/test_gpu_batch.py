@@ -62,6 +62,29 @@ def test_batch_build_matches_oracle(S, oracle):
             np.testing.assert_array_equal(g[k], ov, err_msg=f"mesh {m} {k}")
 
 
+def test_batch_build_refit_levels(S, oracle):
+    """Meshes of every refit class in one batch: one refit subtree (at most
+    1,024 triangles), a mid set only (1,025 to 16,384), mid sets and nodes
+    above them (more); each mesh's tree equals the oracle's with every layout
+    of the pass above the mid sets."""
+    meshes = [
+        W.cut_torus(200, 100, seed=11),           # 40,000
+        W.icosphere(2, 0.0),                      # 320
+        W.cut_torus(96, 48, seed=12),             # 9,216
+        W.icosphere(5, 0.02, seed=13),            # 20,480
+        (np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.int32([[0, 1, 2]])),
+        W.uv_sphere(17000, seed=14),
+    ]
+    orc = [oracle.build_tree(V, F) for V, F in meshes]
+    b = S.BatchScene(meshes)
+    for layout in ("auto", "memory", "climb"):
+        b.build_bvh(top_layout=layout)
+        for m, o in enumerate(orc):
+            g = b.export(m)
+            for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
+                np.testing.assert_array_equal(g[k], ov, err_msg=f"{layout}: mesh {m} {k}")
+
+
 @pytest.mark.parametrize("tune", [None, {"block_size": 64, "stack_depth": 24, "queries_per_wave": 128},
                                   {"block_size": 512, "stack_depth": 32, "queries_per_wave": 256},
                                   {"block_size": 128, "stack_depth": 16, "queries_per_wave": 64}])

@@ -66,8 +66,9 @@ def test_build_bit_exact(S, oracle, name, reference_radius):
 
 
 def test_build_bit_exact_large(S, oracle):
-    """327,680 triangles: the radix sort's multi-block histogram scan and the
-    refit's top-level pass both run; the whole tree must equal the oracle's.
+    """327,680 triangles: the radix sort's multi-block histogram scan and all
+    three refit levels (subtrees of 1,024 leaves, mid sets of 16,384, the
+    nodes above) run; the whole tree must equal the oracle's.
     Then 300,000 queries in Morton order against input order."""
     V, F = W.cut_torus(512, 320)
     sc = S.Scene(V, F).build_bvh()
@@ -86,20 +87,22 @@ def test_build_bit_exact_large(S, oracle):
 
 
 def test_build_bit_exact_many_tops(S, oracle):
-    """2,096,704 triangles: about 3,500 refit subtrees of up to 1,024 leaves,
-    more than the top pass keeps boxes for in LDS, so it reads them from
-    memory; the tree must still equal the oracle's."""
+    """2,096,704 triangles: about 3,500 refit subtrees of up to 1,024 leaves
+    and a few hundred mid sets; the tree must equal the oracle's with every
+    layout of the top pass (boxes in LDS, boxes in memory, the climb with
+    arrival counters that very large scenes use)."""
     V, F = W.cut_torus(1448, 724)
-    g = S.Scene(V, F).build_bvh().export()
     o = oracle.build_tree(V, F)
-    for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
-        np.testing.assert_array_equal(g[k], ov, err_msg=k)
+    sc = S.Scene(V, F)
+    for layout in ("auto", "memory", "climb"):
+        g = sc.build_bvh(top_layout=layout).export()
+        for k, ov in (("sorted_idx", o.sorted_idx), ("nodes", o.nodes), ("boxes", o.boxes), ("cones", o.cones)):
+            np.testing.assert_array_equal(g[k], ov, err_msg=f"{layout}: {k}")
 
 
-def test_build_bit_exact_top_climb(S, oracle):
-    """8,386,816 triangles: about 14,000 nodes lie above the refit subtrees,
-    more than the top pass keeps in LDS at all, so it climbs from the subtree
-    roots with arrival counters in memory; the tree must equal the oracle's."""
+def test_build_bit_exact_8m(S, oracle):
+    """8,386,816 triangles: about 14,000 refit subtrees, about 1,000 mid sets
+    and the nodes above them; the tree must equal the oracle's."""
     V, F = W.cut_torus(2896, 1448)
     g = S.Scene(V, F).build_bvh().export()
     o = oracle.build_tree(V, F)
