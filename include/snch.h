@@ -35,7 +35,8 @@ extern "C" {
 #define SNCH_ERR_EMPTY 2    /* empty mesh: the reference silently returns (bvh.cuh:383-386) */
 #define SNCH_ERR_HIP 3      /* HIP runtime failure */
 #define SNCH_ERR_STATE 4    /* "BVH is not built yet." (scene.cuh:686, :1250) */
-#define SNCH_ERR_OVERFLOW 5 /* a traversal exceeded its stack (query.cuh:315, :420 assert) */
+#define SNCH_ERR_OVERFLOW 5 /* a traversal exceeded its stack or its step bound, which only a
+                               malformed tree can do (query.cuh:315, :420 assert) */
 
 typedef struct snch_context snch_context;
 typedef struct snch_scene snch_scene;
@@ -202,7 +203,7 @@ int snch_batch_closest_point(const snch_scene *scene, const float *q, const uint
                              float *out_dist, const snch_query_opts *opts, void *stream);
 /* seed_radius (may be NULL): per-query pruning radius (DEVICE, n floats), as
  * for snch_closest_silhouette: the search within it is exact and a query not
- * settled there resumes without it, so outputs equal the unseeded call's. */
+ * settled there searches again without it, so outputs equal the unseeded call's. */
 int snch_batch_closest_silhouette(const snch_scene *scene, const float *q, const uint32_t *q_off, int flip,
                                   const float *seed_radius, uint32_t *out_sil, float *out_dist,
                                   const snch_query_opts *opts, void *stream);

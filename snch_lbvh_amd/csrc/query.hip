@@ -7,10 +7,12 @@
 //  * Node layout: one 64-B record per internal node holds both children's
 //    boxes and references; the silhouette query reads a 48-B cone record from
 //    a separate stream, so the closest-point and ray queries never touch it.
-//  * Stack: a short per-lane stack in LDS (16 entries of {node, key}, column
-//    per lane so a wave's 64 lanes hit distinct banks) backed by a private
-//    overflow array; the tree depth is at most 64, so the stack never loses
-//    an entry.
+//  * Stack: closest point and closest silhouette keep a short per-lane stack
+//    in LDS (16 packed entries, a column per lane so a wave's 64 lanes hit
+//    distinct banks) with a restart trail (TrailStack): nothing in private
+//    memory; a far child pushed out of the full stack is found again from
+//    the trail.  Rays keep the 16 LDS entries over a private overflow array
+//    (the tree depth is at most 64, so that stack never loses an entry).
 //  * Order: the nearer child is descended first and the farther is pushed,
 //    and a child is skipped when its lower bound exceeds the best answer so
 //    far.  The reference's own pruning predicates (minmaxdist for closest
@@ -94,53 +96,50 @@ struct LaneStack {
     }
 };
 
-// The same stack with 4-B entries, node << 8 | q, for trees of at most 2^24
-// leaves (internal node ids < 2^24).  q is an 8-bit lower bound of the key (a
-// non-negative squared distance): the key's float bits >> 20 (exponent and 3
-// mantissa bits, i.e. the key truncated to 8 steps per octave) minus a base
-// chosen per tree so that the 255 levels span 31 octaves below the squared
-// scene diagonal; keys below that range store 0 (decoded as 0), keys above
-// saturate at 255, so the decoded value never exceeds the key and a popped
-// entry is never skipped wrongly.  Half the LDS and spill traffic of uint2
-// entries.
-// The private array also holds, from its top end down, up to DEFER entries
-// set aside by a capped search (see sil_query).
-constexpr int DEFER = 24;
-
+// A short stack with a restart trail (Laine 2010, "Restart trail for
+// stackless BVH traversal"), for searches whose pruning bound only tightens:
+// SSZ 4-B entries in a circular LDS column, and nothing in private memory.
+// An entry is node << 8 | q for trees of at most 2^24 leaves (internal node
+// ids < 2^24); q is an 8-bit lower bound of the key (a non-negative squared
+// distance): the key's float bits >> 20 (exponent and 3 mantissa bits, the
+// key truncated to 8 steps per octave) minus a base chosen per tree so that
+// the 255 levels span 31 octaves below the squared scene diagonal; keys
+// below that range store 0 (decoded as 0), keys above saturate at 255, so
+// the decoded value never exceeds the key and a popped entry is never
+// skipped wrongly.  A push onto a full stack overwrites the oldest
+// entry (the shallowest level); the lost far children are found again
+// through the trail and revisited from the root.  Per lane, one bit per tree
+// level (root 0; a path holds at most 64 internal nodes):
+//   T  at this level the near child is finished and the far child taken;
+//   S  at this level only one child was descended (nothing left behind);
+//   M  the levels whose far child sits in the stack (levels grow from the
+//      stack's bottom to its top, so the top entry's level is M's highest).
+// A level of the current path with neither T nor S set, and not in M, left
+// its far child behind; when the stack runs empty the deepest such level j
+// gets T and the search restarts from the root: levels above j follow the
+// trail (T: far child; else near, re-pushing a surviving far child), and
+// level j takes its far child.  Children survive only if they pass the
+// pruning tests, which can only fail more often as the bound tightens, so a
+// subtree left by the trail holds nothing new.
 template <int BS, int SSZ>
-struct PackedStack {
-    using Entry = uint32_t;
-    Entry* lds;
-    int sp;
-    int nd; // set-aside entries at ov[OVF2 - 1], ov[OVF2 - 2], ...
-    static constexpr int OVF2 = OVF + DEFER;
-    uint32_t ov[OVF2];
-    bool overflow;
+struct TrailStack {
+    uint32_t* lds;
+    uint32_t top; // position of the next push (mod SSZ)
     uint32_t qbase;
+    uint64_t T, S, M;
+    uint32_t lv; // level of the current node
 
-    __device__ __forceinline__ void init(Entry* base, uint32_t qb)
+    __device__ __forceinline__ void init(uint32_t* base, uint32_t qb)
     {
         lds = base;
-        sp = 0;
-        nd = 0;
-        overflow = false;
         qbase = qb;
+        reset();
     }
-    __device__ __forceinline__ bool defer(uint32_t n, float d)
+    __device__ __forceinline__ void reset()
     {
-        if (nd >= DEFER)
-            return false;
-        ov[OVF2 - 1 - nd] = enc(n, d);
-        ++nd;
-        return true;
-    }
-    __device__ __forceinline__ void undefer(uint32_t& n, float& d)
-    {
-        --nd;
-        const uint32_t e = ov[OVF2 - 1 - nd];
-        n = e >> 8;
-        const uint32_t q = e & 255u;
-        d = q ? __uint_as_float((q + qbase) << 20) : 0.0f;
+        top = 0;
+        T = S = M = 0ull;
+        lv = 0;
     }
     __device__ __forceinline__ uint32_t enc(uint32_t n, float d) const
     {
@@ -148,31 +147,74 @@ struct PackedStack {
         const uint32_t q = v > qbase ? min(v - qbase, 255u) : 0u;
         return (n << 8) | q;
     }
-    __device__ __forceinline__ void push(uint32_t n, float d)
+    __device__ __forceinline__ void push(uint32_t n, float d, uint32_t k)
     {
-        const int s = sp % SSZ;
-        if (sp >= SSZ) {
-            const int o = sp - SSZ;
-            if (o < OVF) {
-                ov[o] = lds[s * BS];
-            } else {
-                overflow = true;
-                return;
-            }
-        }
-        lds[s * BS] = enc(n, d);
-        ++sp;
+        lds[(top % SSZ) * BS] = enc(n, d);
+        ++top;
+        if (__popcll(M) >= SSZ)
+            M &= M - 1ull; // the oldest entry was just overwritten
+        M |= 1ull << k;
     }
+    // pop the top entry (M != 0): its node and decoded key; the current
+    // path ends at its level, whose near child is finished
     __device__ __forceinline__ void pop(uint32_t& n, float& d)
     {
-        --sp;
-        const int s = sp % SSZ;
-        const uint32_t e = lds[s * BS];
+        --top;
+        const uint32_t e = lds[(top % SSZ) * BS];
         n = e >> 8;
         const uint32_t q = e & 255u;
         d = q ? __uint_as_float((q + qbase) << 20) : 0.0f;
-        if (sp >= SSZ)
-            lds[s * BS] = ov[sp - SSZ];
+        const uint32_t k = 63u - (uint32_t)__clzll(M);
+        const uint64_t kb = 1ull << k;
+        M &= ~kb;
+        T = (T & (kb - 1ull)) | kb;
+        S &= kb - 1ull;
+        lv = k;
+    }
+    // the stack is empty: restart from the root for the deepest level of the
+    // current path that left its far child behind; false when there is none
+    __device__ __forceinline__ bool restart()
+    {
+        const uint64_t path = lv >= 63u ? ~1ull : ((2ull << lv) - 1ull) & ~1ull; // levels 1..lv
+        const uint64_t Z = ~(T | S) & path;
+        if (!Z)
+            return false;
+        const uint32_t j = 63u - (uint32_t)__clzll(Z);
+        const uint64_t jb = 1ull << j;
+        T = (T & (jb - 1ull)) | jb;
+        S &= jb - 1ull;
+        top = 0;
+        lv = 0;
+        return true;
+    }
+    // choose the child to descend from the current node (children at level
+    // lv + 1), given which of the near (N) and far (F) children survive;
+    // false: nothing to descend
+    __device__ __forceinline__ bool descend(bool gN, bool gF, uint32_t nN, uint32_t nF, float dF, uint32_t& node)
+    {
+        if (!gN && !gF)
+            return false;
+        const uint32_t k = lv + 1u; // at most 63: the caller stops deeper paths
+        const uint64_t kb = 1ull << k;
+        if (T & kb) { // near finished before: the far child, if it survives
+            if (!gF)
+                return false;
+            node = nF;
+        } else if (gN && gF) {
+            push(nF, dF, k);
+            node = nN;
+        } else if (gN) {
+            S |= kb;
+            node = nN;
+        } else if (gF) { // off the trail: the near child no longer survives
+            T = (T & (kb - 1ull)) | kb;
+            S &= kb - 1ull;
+            node = nF;
+        } else {
+            return false;
+        }
+        lv = k;
+        return true;
     }
 };
 
@@ -282,13 +324,15 @@ __device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint32_t* stk_
         cp_leaf<DIM>(t, 0, p, best, best_idx);
         return;
     }
-    PackedStack<BS, SSZ> st;
+    TrailStack<BS, SSZ> st;
     st.init(stk_col, t.qbase);
     uint32_t node = 0;
-    // each internal node is visited at most once; the cap and the range
-    // checks only guard against a malformed tree
-    const uint32_t cap_iters = 2 * t.n_leaves + 64;
-    for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+    // a pass visits each internal node at most once, restarts add the paths
+    // above lost far children; the cap and the range checks only guard
+    // against a malformed tree (hitting the cap is reported as an overflow)
+    const uint32_t cap_iters = 8 * t.n_leaves + 8192;
+    uint32_t it = 0;
+    for (; it < cap_iters && node < t.n_leaves - 1; ++it) {
         const NodeRec nr = load_node(t.tnode, node);
         const float Lmd = box_mindist(nr.L, p), Rmd = box_mindist(nr.R, p);
         const float Lmm = box_minmaxdist(nr.L, p), Rmm = box_minmaxdist(nr.R, p);
@@ -313,18 +357,16 @@ __device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint32_t* stk_
         }
         const bool gL = cL && !lLeaf && Lmd <= best;
         const bool gR = cR && !rLeaf && Rmd <= best;
-        if (gL && gR) {
-            const bool lfirst = Lmd <= Rmd;
-            st.push(lfirst ? nr.rref : nr.lref, lfirst ? Rmd : Lmd);
-            node = lfirst ? nr.lref : nr.rref;
-            continue;
+        if ((gL || gR) && st.lv >= 63u) {
+            overflow = true; // deeper than a Karras tree can be
+            break;
         }
-        if (gL || gR) {
-            node = gL ? nr.lref : nr.rref;
+        const bool lfirst = Lmd <= Rmd;
+        if (st.descend(lfirst ? gL : gR, lfirst ? gR : gL, lfirst ? nr.lref : nr.rref, lfirst ? nr.rref : nr.lref,
+                       lfirst ? Rmd : Lmd, node))
             continue;
-        }
         bool found = false;
-        while (st.sp > 0) {
+        while (st.M) {
             uint32_t nn;
             float dd;
             st.pop(nn, dd);
@@ -334,10 +376,13 @@ __device__ __forceinline__ void cp_query(const TreeView& t, v3 p, uint32_t* stk_
                 break;
             }
         }
-        if (!found)
+        if (found)
+            continue;
+        if (!st.restart())
             break;
+        node = 0;
     }
-    overflow = overflow || st.overflow;
+    overflow = overflow || it >= cap_iters;
 }
 
 template <int BS, bool STATS, int DIM, int SSZ>
@@ -529,16 +574,13 @@ __device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p,
 }
 
 // query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
-// best / best_e receive the answer.  A finite radius `cap` changes only the
-// visiting order: internal nodes farther than cap (that pass the reference's
-// tests) are set aside instead of being descended into.  When the search
-// within cap ends with best <= cap, every node that could hold a silhouette
-// at least as close has been visited and the set-aside nodes are dropped;
-// otherwise (HINT) the search resumes from them with no cap, so nothing is
-// visited twice, or (hard cap, !HINT) it stops.  If more nodes are set aside
-// than the stack keeps, the resumed search restarts from the root with the
-// bound found so far.  With index tie-breaking the answer equals the
-// uncapped search.
+// best / best_e receive the answer.  The stack is a TrailStack: SSZ entries
+// in LDS and no private memory.  A finite radius `cap` prunes internal nodes
+// farther than cap as well.  When the search within cap ends with
+// best <= cap, every node that could hold a silhouette at least as close has
+// been visited; otherwise (HINT) it searches again from the root with no cap
+// and the bound found so far, or (hard cap, !HINT) it stops.  With index
+// tie-breaking the answer equals the uncapped search.
 template <int BS, int SSZ, bool STATS, int DIM, bool HINT, bool PARK>
 __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, float cap, uint32_t* stk_col,
                                           EdgeLds* elds, float& best, uint32_t& best_e, Counts& cnt,
@@ -556,11 +598,14 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     if (!(cap >= 0.0f))
         cap = F_INF;
     float cap2 = cap * cap;
-    PackedStack<BS, SSZ> st;
+    TrailStack<BS, SSZ> st;
     st.init(stk_col, t.qbase);
-    bool dlost = false;
     uint32_t node = 0;
-    const uint32_t cap_iters = 4 * t.n_leaves + 64;
+    // a pass visits each internal node at most once; restarts add the paths
+    // above lost far children (a few per query); the bound only guards
+    // against an inconsistent tree, and hitting it is reported as an overflow
+    const uint32_t cap_iters = 8 * t.n_leaves + 8192;
+    uint32_t it = 0;
     // A lane whose node has leaf children to evaluate parks them (phase
     // `leaves`) until a quarter of the wave's looping lanes, or all of them,
     // are parked; the parked lanes then evaluate their leaves together and
@@ -573,7 +618,7 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
     float Lmd = 0.0f, Rmd = 0.0f;
     bool hL = false, hR = false, lLeaf = false, rLeaf = false;
     uint32_t lref = 0, rref = 0, lf = 0, lc = 0, rf = 0, rc = 0;
-    for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1;) {
+    for (; it < cap_iters && node < t.n_leaves - 1;) {
         bool decide = false;
         if (!leaves) {
             ++it;
@@ -631,75 +676,56 @@ __device__ __forceinline__ void sil_query(const TreeView& t, v3 p, bool flip, fl
         }
         if (!decide)
             continue;
+        // internal children to descend: within the bound and the cap
         const float nb2 = best * best;
-        bool gL = hL && !lLeaf && Lmd <= nb2;
-        bool gR = hR && !rLeaf && Rmd <= nb2;
-        if (gL && Lmd > cap2) { // beyond the cap: set aside (a hard cap drops it)
-            if (HINT)
-                dlost = dlost || !st.defer(lref, Lmd);
-            gL = false;
+        const bool gL = hL && !lLeaf && Lmd <= nb2 && Lmd <= cap2;
+        const bool gR = hR && !rLeaf && Rmd <= nb2 && Rmd <= cap2;
+        if ((gL || gR) && st.lv >= 63u) {
+            overflow = true; // deeper than a Karras tree can be
+            break;
         }
-        if (gR && Rmd > cap2) {
-            if (HINT)
-                dlost = dlost || !st.defer(rref, Rmd);
-            gR = false;
-        }
-        if (gL && gR) {
-            const bool lfirst = Lmd <= Rmd;
-            st.push(lfirst ? rref : lref, lfirst ? Rmd : Lmd);
-            node = lfirst ? lref : rref;
+        const bool lfirst = Lmd <= Rmd;
+        if (st.descend(lfirst ? gL : gR, lfirst ? gR : gL, lfirst ? lref : rref, lfirst ? rref : lref,
+                       lfirst ? Rmd : Lmd, node))
             continue;
-        }
-        if (gL || gR) {
-            node = gL ? lref : rref;
-            continue;
-        }
         bool found = false;
-        for (int phase = 0; phase < 2 && !found; ++phase) {
-            while (st.sp > 0) {
-                uint32_t nn;
-                float dd;
-                st.pop(nn, dd);
-                if (dd <= best * best) { // query.cuh:347
-                    node = nn;
-                    found = true;
-                    break;
-                }
-            }
-            if (found || phase == 1)
-                break;
-            // the search within the cap is over
-            if (cap2 == F_INF || (best_e != INVALID && best <= cap) || !HINT)
-                break;
-            cap2 = F_INF;
-            if (dlost) { // not every set-aside node was kept: restart with the bound found so far
-                node = 0;
+        while (st.M) {
+            uint32_t nn;
+            float dd;
+            st.pop(nn, dd);
+            if (dd <= best * best) { // query.cuh:347
+                node = nn;
                 found = true;
                 break;
             }
-            while (st.nd > 0) { // resume from the set-aside nodes
-                uint32_t nn;
-                float dd;
-                st.undefer(nn, dd);
-                if (dd <= best * best)
-                    st.push(nn, dd);
-            }
         }
-        if (!found)
+        if (found)
+            continue;
+        if (st.restart()) {
+            node = 0;
+            continue;
+        }
+        // the search within the cap is over: settled, or (HINT) search again
+        // from the root with no cap and the bound found so far
+        if (cap2 == F_INF || (best_e != INVALID && best <= cap) || !HINT)
             break;
+        cap2 = F_INF;
+        st.reset();
+        node = 0;
     }
-    overflow = overflow || st.overflow;
+    overflow = overflow || it >= cap_iters;
 }
 
 // MODE 0: plain search.  MODE 1: search within the radius seed[qi] first; a
-// query not settled there resumes from the nodes it set aside (bit-identical
-// to MODE 0).  MODE 2: search within the radius only (a hard cap).  Keeping
-// the resumption in the same launch lets the rare unsettled queries (the
-// costliest ones) overlap with the rest of the grid; a separate second
-// launch over them measured ~30 ms slower on the 1M-triangle torus, all of
-// it the tail of those few queries.
+// query not settled there searches again from the root with no cap and the
+// bound found so far (bit-identical to MODE 0).  MODE 2: search within the
+// radius only (a hard cap).  Keeping the second search in the same launch
+// lets the rare unsettled queries (the costliest ones) overlap with the rest
+// of the grid; a separate second launch over them measured ~30 ms slower on
+// the 1M-triangle torus, all of it the tail of those few queries.  The plain
+// kernel is held to 80 VGPRs (6 waves per SIMD), the others to 96.
 template <int BS, bool STATS, int DIM, int MODE, int SSZ>
-__global__ __launch_bounds__(BS) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
+__global__ __launch_bounds__(BS, MODE == 0 && !STATS ? 6 : 5) void k_closest_silhouette(TreeView t, const float* __restrict__ q, uint32_t n,
                                                            uint32_t nq, int flip, const float* __restrict__ seed,
                                                            uint32_t* __restrict__ out_e, float* __restrict__ out_d,
                                                            const uint32_t* __restrict__ perm,
@@ -891,9 +917,9 @@ struct BatchTable {
 
 // SEEDED (closest silhouette): seed[qi] is the query's pruning radius (a
 // learned bound); the search within it is exact, a query not settled there
-// resumes from the nodes it set aside, so results equal the unseeded ones
+// searches again from the root, so results equal the unseeded ones
 template <int BS, int SSZ, int KIND, bool SEEDED>
-__global__ __launch_bounds__(BS, 7) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
+__global__ __launch_bounds__(BS, 5) void k_batch(BatchTable bt, const float* __restrict__ q, int flip,
                                               const float* __restrict__ seed, uint32_t* __restrict__ out_idx,
                                               float* __restrict__ out_dist, uint32_t* __restrict__ err)
 {
