@@ -21,10 +21,13 @@
 //                     (bvh.cuh:459-476), so no uniqueness pass or host
 //                     round trip is needed.
 //   k_refit           bottom-up AABB + cone merge over subtrees of at most
-//                     1024 leaves in LDS, then k_refit_top above them (the
-//                     parent box centroid feeds the cone merge, as in
-//                     bvh.cuh:598); writes the reference-layout node data and
-//                     the packed traversal records.
+//                     1024 leaves in LDS (the parent box centroid feeds the
+//                     cone merge, as in bvh.cuh:598); writes the
+//                     reference-layout node data and the packed traversal
+//                     records
+//   k_refit_mid       the same above them, up to subtrees of 16384 leaves,
+//                     one workgroup per subtree
+//   k_refit_top       the nodes above those, one workgroup
 //
 // A scene may hold several meshes: every kernel runs over all of them at
 // once.  Each mesh gets its own scene box, Morton codes in that box, and its
@@ -630,6 +633,7 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes_all, co
 constexpr uint32_t RT = 1024;  // leaves of one refit subtree ("cut")
 constexpr uint32_t RB = 512;   // threads of one k_refit workgroup (RT / RB leaves each)
 constexpr uint32_t RM = 16384; // leaves of a mid set's subtree (the tops below it: k_refit_mid)
+constexpr uint32_t MID_CAP = 1024; // members of a mid set (at most 960, see k_refit_mid)
 
 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
@@ -968,6 +972,41 @@ __device__ __forceinline__ uint32_t cut_child_global(const MeshIds& id, uint32_t
     return lc & (1u << 10) ? id.nb + (id.n - 1) + j : id.nb + j;
 }
 
+// Members of the mid sets: each top of at most RM leaves climbs to the root
+// of its set (the last ancestor of at most RM leaves; flags[root] = the set's
+// slot, written by k_karras) and appends itself to the set's member list.
+// Run by k_refit's threads (index k0, stride) before they take cuts; the
+// lists are read by k_refit_mid, the next launch.
+__device__ __forceinline__ void collect_mid_members(uint32_t k0, uint32_t stride,
+                                                    const uint32_t* __restrict__ midtops, uint32_t midtop_cap,
+                                                    const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ range,
+                                                    const uint32_t* __restrict__ flags, uint32_t* __restrict__ mids,
+                                                    uint32_t mid_cap, uint32_t* __restrict__ members)
+{
+    const uint32_t n = min(midtops[midtop_cap], midtop_cap);
+    if (k0 >= n)
+        return;
+    const uint32_t n_mid = min(mids[mid_cap], mid_cap);
+    uint32_t* mcount = mids + mid_cap + 1;
+    for (uint32_t k = k0; k < n; k += stride) {
+        const uint32_t t = midtops[k];
+        uint32_t node = t;
+        for (uint32_t guard = 0; guard < 64; ++guard) {
+            const uint32_t p = parent[node];
+            if (p == INVALID || range[p].y - range[p].x + 1 > RM)
+                break;
+            node = p;
+        }
+        const uint32_t c = flags[node];
+        if (c < n_mid) {
+            const uint32_t j = atomicAdd(mcount + c, 1u);
+            if (j < MID_CAP)
+                members[c * MID_CAP + j] = t;
+        }
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
@@ -978,9 +1017,13 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
                                               int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
                                               float4* __restrict__ tnode, float4* __restrict__ tcone,
                                               float4* __restrict__ tleaf, uint32_t* __restrict__ cuts,
-                                              uint32_t cut_cap)
+                                              uint32_t cut_cap, const uint32_t* __restrict__ midtops,
+                                              uint32_t midtop_cap, uint32_t* __restrict__ mids, uint32_t mid_cap,
+                                              uint32_t* __restrict__ members, const uint32_t* __restrict__ flags)
 {
     static_assert(RT == 1024, "10-bit local positions");
+    collect_mid_members(blockIdx.x * RB + threadIdx.x, gridDim.x * RB, midtops, midtop_cap, parent, range, flags,
+                        mids, mid_cap, members);
     constexpr uint32_t LPT = RT / RB; // leaves per thread
     __shared__ NodeLds nd;
     __shared__ uint32_t lnode[RT];    // children and slot of the internal node at local position k
@@ -1132,7 +1175,8 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
 //
 //   mid sets  the tops of one subtree of at most RM leaves whose parent has
 //             more (or a mesh root of at most RM leaves): k_refit_mid, one
-//             workgroup per set, many sets at once;
+//             workgroup per set, many sets at once (k_refit lists the
+//             members first);
 //   uppers    the tops of more than RM leaves: k_refit_top, one workgroup.
 //
 // Size of a mid set: the common prefix of a node's 64-bit keys grows by at
@@ -1141,7 +1185,6 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
 // subtrees of more than RT leaves, at most RM / (RT + 1) = 15 of them, and
 // every member lies on a path from the set root to one of them: at most
 // 15 * 64 = 960 members, so MID_CAP members always fit.
-constexpr uint32_t MID_CAP = 1024;
 constexpr uint32_t MB = 256;              // threads of one k_refit_mid workgroup
 constexpr uint32_t NO_MEMBER = 0xFFFFu;
 constexpr uint32_t SET_ROUNDS = 128;      // a set is at most 64 levels deep
@@ -1265,37 +1308,6 @@ __device__ void refit_set(const SetLds& s, uint32_t n, uint32_t lo_thr, uint32_t
     // the sin and cos of the children's half angles, once every record is written
     for (uint32_t k = t; k < n; k += NT)
         tcone_trig(tcone, s.node[k] - desc[min(s.mesh[k], M - 1)].off);
-}
-
-// Members of the mid sets: each top of at most RM leaves climbs to the root
-// of its set (the last ancestor of at most RM leaves; flags[root] = the set's
-// slot, written by k_karras) and appends itself to the set's member list.
-__global__ __launch_bounds__(BT) void k_mid_members(const uint32_t* __restrict__ midtops, uint32_t midtop_cap,
-                                                    const uint32_t* __restrict__ parent,
-                                                    const uint2* __restrict__ range,
-                                                    const uint32_t* __restrict__ flags,
-                                                    uint32_t* __restrict__ mids, uint32_t mid_cap,
-                                                    uint32_t* __restrict__ members)
-{
-    const uint32_t n = min(midtops[midtop_cap], midtop_cap);
-    const uint32_t n_mid = min(mids[mid_cap], mid_cap);
-    uint32_t* mcount = mids + mid_cap + 1;
-    for (uint32_t k = blockIdx.x * BT + threadIdx.x; k < n; k += gridDim.x * BT) {
-        const uint32_t t = midtops[k];
-        uint32_t node = t;
-        for (uint32_t guard = 0; guard < 64; ++guard) {
-            const uint32_t p = parent[node];
-            if (p == INVALID || range_size(range, p) > RM)
-                break;
-            node = p;
-        }
-        const uint32_t c = flags[node];
-        if (c < n_mid) {
-            const uint32_t j = atomicAdd(mcount + c, 1u);
-            if (j < MID_CAP)
-                members[c * MID_CAP + j] = t;
-        }
-    }
 }
 
 // One workgroup per mid set (a grid-stride loop over the sets), boxes in LDS.
@@ -1546,10 +1558,6 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     }
     const uint32_t top_layout = (flags & SNCH_BUILD_TOPS_CLIMB) ? 2u : (flags & SNCH_BUILD_TOPS_IN_MEMORY) ? 1u : 0u;
     mark(3);
-    if (any_big)
-        hipLaunchKernelGGL(k_mid_members, dim3(std::min(blocks(top_cap), 4u * n_cu)), dim3(BT), 0, st, midtops,
-                           top_cap, w.parent.as<uint32_t>(), w.range.as<uint2>(), w.flags.as<uint32_t>(), mids,
-                           mid_cap, w.members.as<uint32_t>());
     SNCH_DISPATCH_REFIT(s->dim, {
         // as many workgroups as are resident at once (LDS-bound), or one per cut
         static int per_cu = 0;
@@ -1562,7 +1570,8 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
                            w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
                            desc, M, N, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
-                           s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap);
+                           s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap,
+                           midtops, top_cap, mids, mid_cap, w.members.as<uint32_t>(), w.flags.as<uint32_t>());
         if (any_big)
             hipLaunchKernelGGL(k_refit_mid<DIM>, dim3(std::min(mid_cap, 2u * n_cu)), dim3(MB), 0, st,
                                w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),

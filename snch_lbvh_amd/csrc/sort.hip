@@ -27,6 +27,8 @@
 // started before it; the wait is bounded all the same.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace snch {
@@ -42,6 +44,7 @@ constexpr int MAX_PASSES = 8;    // up to 4 key digits and 4 segment digits
 constexpr uint32_t LB_AGG = 1u << 30, LB_INC = 2u << 30, LB_COUNT = (1u << 30) - 1u;
 constexpr uint32_t LB_SPIN = 1u << 22; // bound on polls of one look-back word
 constexpr uint32_t LB_WIN = 8;         // look-back words read at once
+constexpr uint32_t HIST_BLOCKS = 512;   // k_hist_all workgroups (2 per CU); each loops over tiles
 
 __global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__ tile_ctr)
 {
@@ -52,7 +55,12 @@ __global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__
 }
 
 // digit counts of every pass (pass p < key_passes: key byte p; then byte
-// p - key_passes of seg[value]); clears this tile's look-back words
+// p - key_passes of seg[value]); clears this tile's look-back words.  Each
+// thread counts SI consecutive keys and adds a run of equal digits with one
+// LDS atomic; when every lane of the wave ends on the same digit (Morton
+// codes of neighbouring primitives mostly share their high bytes), one lane
+// adds the wave's total.  Same-address LDS atomics from many lanes run one
+// lane at a time, and the LDS is shared by the CU's four SIMDs.
 __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                  const uint32_t* __restrict__ seg, uint32_t n, int key_passes,
                                                  int seg_passes, uint32_t ntiles, uint32_t* __restrict__ ghist,
@@ -64,28 +72,68 @@ __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ ke
     for (uint32_t i = tid; i < MAX_PASSES * RADIX; i += ST)
         h[i / RADIX][i % RADIX] = 0u;
     __syncthreads();
-    const uint32_t base = blockIdx.x * TILE;
-#pragma unroll 4
-    for (int i = 0; i < SI; ++i) {
-        const uint32_t pos = base + i * ST + tid;
-        if (pos >= n)
-            continue;
-        const uint32_t k = keys[pos];
-        for (int p = 0; p < key_passes; ++p)
-            atomicAdd(&h[p][(k >> (8 * p)) & 0xFFu], 1u);
-        if (seg_passes > 0) {
-            const uint32_t sg = seg[vals[pos]];
-            for (int q = 0; q < seg_passes; ++q)
-                atomicAdd(&h[key_passes + q][(sg >> (8 * q)) & 0xFFu], 1u);
+    // a grid-stride loop over the tiles: one flush into the global totals
+    // per workgroup (the totals are few words hit by every workgroup)
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t base = tile * TILE + tid * SI;
+        uint32_t kk[SI], sg[SI];
+        uint32_t nv = 0; // valid keys of this thread (a prefix of its run)
+        if (base + SI <= n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0u) {
+            const uint4 a = *reinterpret_cast<const uint4*>(keys + base);
+            const uint4 b = *reinterpret_cast<const uint4*>(keys + base + 4);
+            kk[0] = a.x; kk[1] = a.y; kk[2] = a.z; kk[3] = a.w;
+            kk[4] = b.x; kk[5] = b.y; kk[6] = b.z; kk[7] = b.w;
+            nv = SI;
+        } else {
+#pragma unroll
+            for (int i = 0; i < SI; ++i)
+                kk[i] = base + i < n ? keys[base + i] : 0u;
+            nv = base < n ? min((uint32_t)SI, n - base) : 0u;
         }
+#pragma unroll
+        for (int i = 0; i < SI; ++i)
+            sg[i] = seg_passes > 0 && (uint32_t)i < nv ? seg[vals[base + i]] : 0u;
+        static_assert(SI == 8, "two 16-byte loads per thread");
+        for (int p = 0; p < passes; ++p) {
+            const bool on_key = p < key_passes;
+            const int sh = 8 * (on_key ? p : p - key_passes);
+            uint32_t cur = ((on_key ? kk[0] : sg[0]) >> sh) & 0xFFu, cnt = 0;
+#pragma unroll
+            for (int i = 0; i < SI; ++i) {
+                const uint32_t d = ((on_key ? kk[i] : sg[i]) >> sh) & 0xFFu;
+                if ((uint32_t)i < nv) {
+                    if (d != cur) {
+                        atomicAdd(&h[p][cur], cnt);
+                        cur = d;
+                        cnt = 0;
+                    }
+                    ++cnt;
+                }
+            }
+            // the last run of each lane: one atomic for the whole wave when
+            // every lane with keys ends on the same digit
+            const uint64_t live = __ballot(cnt != 0u);
+            const uint32_t first = __shfl(cur, live ? (uint32_t)__ffsll((unsigned long long)live) - 1u : 0u, 64);
+            if (live && __ballot(cnt != 0u && cur == first) == live) {
+                uint32_t sum = cnt;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1)
+                    sum += __shfl_xor(sum, o, 64);
+                if ((threadIdx.x & 63u) == (uint32_t)__ffsll((unsigned long long)live) - 1u)
+                    atomicAdd(&h[p][first], sum);
+            } else if (cnt != 0u) {
+                atomicAdd(&h[p][cur], cnt);
+            }
+        }
+        if (tid < RADIX)
+            for (int p = 0; p < passes; ++p)
+                status[((size_t)p * ntiles + tile) * RADIX + tid] = 0u;
     }
     __syncthreads();
     if (tid < RADIX)
-        for (int p = 0; p < passes; ++p) {
+        for (int p = 0; p < passes; ++p)
             if (h[p][tid] != 0u)
                 atomicAdd(&ghist[p * RADIX + tid], h[p][tid]);
-            status[((size_t)p * ntiles + blockIdx.x) * RADIX + tid] = 0u;
-        }
 }
 
 // one pass.  SEG: the digit comes from seg[value] instead of the key
@@ -310,8 +358,8 @@ hipError_t radix_sort_pairs_in(uint32_t* keys, uint32_t* vals, uint32_t* keys_al
     uint32_t* status = tile_ctr + 2 * MAX_PASSES;
     if (!pre_cleared)
         hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
-    hipLaunchKernelGGL(k_hist_all, dim3(ntiles), dim3(ST), 0, st, keys, vals, seg, n, key_passes, seg_passes, ntiles,
-                       ghist, status);
+    hipLaunchKernelGGL(k_hist_all, dim3(std::min(ntiles, HIST_BLOCKS)), dim3(ST), 0, st, keys, vals, seg, n,
+                       key_passes, seg_passes, ntiles, ghist, status);
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
     for (int pass = 0; pass < passes; ++pass) {
         const bool sp = pass >= key_passes;
