@@ -81,6 +81,11 @@ typedef struct snch_query_opts {
     uint32_t seed_is_cap;      /* closest silhouette: 1 = seed_radius is a hard search radius
                                   (the answer is exact when it lies within the radius; otherwise
                                   the returned distance exceeds the radius, possibly +inf) */
+    uint32_t seed_retry;       /* closest silhouette with seed_radius (not a cap): 0 = queries not
+                                  settled within their radius are queued and searched again 64 at a
+                                  time by whole wavefronts of the same persistent launch (default;
+                                  block size 64 and stack depth 16 only), 1 = each is searched again
+                                  at once by its own lane.  Outputs are the same. */
 } snch_query_opts;
 
 const char *snch_last_error(void);

@@ -58,7 +58,7 @@ class BuildOpts(ctypes.Structure):
 class QueryOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("coherent", ctypes.c_uint32), ("stats", ctypes.c_void_p),
                 ("order", ctypes.c_void_p), ("stack_depth", ctypes.c_uint32), ("queries_per_wave", ctypes.c_uint32),
-                ("seed_is_cap", ctypes.c_uint32)]
+                ("seed_is_cap", ctypes.c_uint32), ("seed_retry", ctypes.c_uint32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -285,7 +285,13 @@ class Scene:
         return idx, dist
 
     def closest_silhouette(self, q, flip=False, seed_radius=None, block_size=0, stats=None, coherent=0, order=None,
-                           stack_depth=0, seed_is_cap=False):
+                           stack_depth=0, seed_is_cap=False, seed_in_place=False):
+        """query_device(nearest_silhouette) (query.cuh:325-423) over q (n x dim).
+        seed_radius: optional per-query pruning radius (a learned bound); the
+        outputs equal the unseeded call's.  Queries not settled within it are
+        re-queued for whole wavefronts of the same launch, or with
+        seed_in_place searched again at once by their own lane.  seed_is_cap
+        makes the radius a hard search radius."""
         import torch
 
         q = _dev_f32(q, self.dim, self.device)
@@ -295,7 +301,7 @@ class Scene:
         sil = torch.empty(n, dtype=torch.int32, device=q.device)
         dist = torch.empty(n, dtype=torch.float32, device=q.device)
         opts = QueryOpts(block_size, coherent, _stats_ptr(stats), _order_ptr(order, n), stack_depth, 0,
-                         int(bool(seed_is_cap)))
+                         int(bool(seed_is_cap)), int(bool(seed_in_place)))
         check(lib().snch_closest_silhouette_ex(self._h, _ptr(q), n, int(bool(flip)), _ptr(seed), _ptr(sil),
                                                _ptr(dist), ctypes.byref(opts), _stream()))
         return sil, dist

@@ -665,16 +665,29 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t* perm = opts ? opts->order : nullptr;
     const bool own_order = !perm && want_order(opts, n);
-    if (own_order) {
+    const bool capped = opts && opts->seed_is_cap;
+    // a seed radius, not a hard cap: unsettled queries are re-queued for whole
+    // waves unless asked to search again in place (default launch shape only)
+    const bool requeue = seed && !capped && n > 0 && !(opts && opts->seed_retry == 1) && block == 64 && depth == 16;
+    if (own_order || requeue)
         SNCH_HIP(scratch_begin(s, st), "snch_closest_silhouette: stream wait");
+    if (own_order)
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
+    if (requeue) {
+        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
+        SNCH_HIP(s->rq_queue.ensure(sizeof(uint32_t) * (size_t)n), "snch_closest_silhouette: allocation");
+        const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
+        SNCH_HIP(launch_closest_silhouette_requeue(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, perm,
+                                                   opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(),
+                                                   s->rq_queue.as<uint32_t>(), n_cu, st),
+                 "snch_closest_silhouette");
+    } else {
+        SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, out_sil, out_dist,
+                                           block, depth, perm, opts ? opts->stats : nullptr, st),
+                 "snch_closest_silhouette");
     }
-    const bool capped = opts && opts->seed_is_cap;
-    SNCH_HIP(launch_closest_silhouette(s->view(), (int)s->dim, q, n, flip, seed, capped, out_sil, out_dist, block,
-                                       depth, perm, opts ? opts->stats : nullptr, st),
-             "snch_closest_silhouette");
-    if (own_order)
+    if (own_order || requeue)
         SNCH_HIP(scratch_end(s, st), "snch_closest_silhouette: event");
     return SNCH_OK;
 }

@@ -141,6 +141,7 @@ struct snch_scene {
     snch::DeviceBuffer tnode, tcone, tleaf, tedge, nbox, ncone;
     snch::BuildScratch scratch;
     mutable snch::QueryScratch qscratch; // used by const query entry points
+    mutable snch::DeviceBuffer rq_ctr, rq_queue; // re-queued seeded search (counters, queue)
     mutable snch::BatchState batch;
     bool built = false;
     bool sorted_in_alt = false; // sorted codes/indices live in scratch.codes_alt/idx_alt
@@ -215,6 +216,12 @@ hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q,
                                      const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
                                      uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st,
                                      uint32_t* err = nullptr);
+// seeded search with unsettled queries re-queued for whole waves (ctr: 10
+// words, queue: n words; both rewritten at launch)
+hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
+                                             const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
+                                             uint64_t* stats, uint32_t* ctr, uint32_t* queue, uint32_t n_cu,
+                                             hipStream_t st, uint32_t* err = nullptr);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

@@ -762,6 +762,152 @@ __global__ __launch_bounds__(BS, MODE == 0 && !STATS ? 6 : 5) void k_closest_sil
         flush_counts(cnt, stats);
 }
 
+// Seeded search with re-queued unsettled queries.  A wave's time is that of
+// its slowest lane, so a query not settled within its seed radius, searched
+// again in place, holds 63 settled lanes idle.  Here the launch is
+// persistent: each wave takes 64-query chunks of the visiting order (each
+// XCD a contiguous range of chunks first, then the other XCDs' ranges) and
+// searches them within their seeds (a hard cap); a query not settled there is
+// appended to a queue.  A wave that finds no chunk left takes up to 64 queued
+// queries at a time and searches them with no cap, until it finds the queue
+// empty.  Every appended entry is taken: its appender itself goes on to drain
+// the queue after its last append.  A taken slot may still be being written
+// by its appender, which has reserved it and is running, so the wait for it
+// is short (and bounded).  ctr: [0, 8) next chunk of each XCD's range,
+// [8] queue head, [9] queue tail; queue: n entries, RQ_EMPTY at launch.
+// Results equal the unseeded search's.
+constexpr uint32_t RQ_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t RQ_SPIN = 1u << 20;
+
+__global__ __launch_bounds__(256) void k_requeue_init(uint32_t* __restrict__ ctr, uint32_t* __restrict__ queue,
+                                                      uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 10)
+        ctr[i] = 0u;
+    for (uint32_t k = i; k < n; k += gridDim.x * 256)
+        queue[k] = RQ_EMPTY;
+}
+
+// wave-uniform copy of lane 0's value (a scalar register)
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int SSZ, bool STATS, int DIM>
+__global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, const float* __restrict__ q,
+                                                                 uint32_t n, int flip, const float* __restrict__ seed,
+                                                                 uint32_t* __restrict__ out_e,
+                                                                 float* __restrict__ out_d,
+                                                                 const uint32_t* __restrict__ perm, uint32_t* ctr,
+                                                                 uint32_t* queue, uint32_t* __restrict__ err,
+                                                                 unsigned long long* stats)
+{
+    __shared__ uint32_t stk[SSZ * 64];
+    __shared__ EdgeLds elds[1];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nchunks = (n + 63) / 64;
+    const uint32_t x0 = blockIdx.x % 8u; // blocks are dispatched round-robin over the 8 XCDs
+    uint32_t xi = 0;                     // ranges exhausted so far, own range first (wave-uniform)
+    bool draining = false, ovf = false;
+    Counts cnt;
+    for (;;) {
+        uint32_t qi = INVALID;
+        float cap = F_INF;
+        if (!draining) {
+            uint32_t c = INVALID;
+            while (xi < 8u) {
+                const uint32_t xx = (x0 + xi) & 7u;
+                const uint32_t lo = (uint32_t)((uint64_t)nchunks * xx / 8u);
+                const uint32_t hi = (uint32_t)((uint64_t)nchunks * (xx + 1u) / 8u);
+                uint32_t k = 0;
+                if (lane == 0)
+                    k = atomicAdd(ctr + xx, 1u);
+                k = uniform(k);
+                if (k < hi - lo) {
+                    c = lo + k;
+                    break;
+                }
+                ++xi;
+            }
+            if (c == INVALID) {
+                draining = true;
+                continue;
+            }
+            const uint32_t slot = c * 64u + lane;
+            if (slot < n) {
+                qi = perm ? perm[slot] : slot;
+                cap = seed[qi];
+                if (!(cap >= 0.0f))
+                    cap = F_INF;
+            }
+        } else {
+            // take up to 64 queued queries: [base, base + k) below the tail
+            uint32_t base = 0, k = 0, empty = 0;
+            if (lane == 0) {
+                uint32_t h = __hip_atomic_load(ctr + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (uint32_t tries = 0; tries < 64u; ++tries) {
+                    const uint32_t tl = __hip_atomic_load(ctr + 9, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (h >= tl) {
+                        empty = 1u;
+                        break;
+                    }
+                    const uint32_t want = min(64u, tl - h);
+                    const uint32_t old = atomicCAS(ctr + 8, h, h + want); // a failure means another wave took some
+                    if (old == h) {
+                        base = h;
+                        k = want;
+                        break;
+                    }
+                    h = old;
+                }
+            }
+            if (uniform(empty))
+                break;
+            base = uniform(base);
+            k = uniform(k);
+            if (k == 0u)
+                continue; // contended: try again
+            if (lane < k) {
+                uint32_t v = RQ_EMPTY;
+                for (uint32_t s = 0; s < RQ_SPIN && v == RQ_EMPTY; ++s) {
+                    v = __hip_atomic_load(queue + base + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (v == RQ_EMPTY)
+                        __builtin_amdgcn_s_sleep(2);
+                }
+                if (v < n)
+                    qi = v;
+                else
+                    ovf = true; // not expected: the slot's writer had reserved it
+            }
+        }
+        float best = F_INF;
+        uint32_t best_e = INVALID;
+        if (qi < n)
+            sil_query<64, SSZ, STATS, DIM, false, true>(t, load_point<DIM>(q, qi), flip != 0, cap, stk + lane,
+                                                        DIM == 3 ? elds : nullptr, best, best_e, cnt, ovf);
+        const bool unsettled = qi < n && cap < F_INF && !(best_e != INVALID && best <= cap);
+        if (qi < n && !unsettled) {
+            if (out_e)
+                out_e[qi] = best_e;
+            out_d[qi] = best;
+        }
+        const uint64_t um = __ballot(unsettled);
+        if (um) {
+            uint32_t b = 0;
+            if (lane == 0)
+                b = atomicAdd(ctr + 9, (uint32_t)__popcll(um));
+            b = uniform(b);
+            // rank among the unsettled lanes below this one
+            const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(um >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)um, 0u));
+            if (unsettled)
+                __hip_atomic_store(queue + b + r, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (ovf && err)
+        atomicOr(err, 1u);
+    if (STATS)
+        flush_counts(cnt, stats);
+}
+
 // ------------------------------------------------------------ ray
 template <int BS, bool STATS, int DIM>
 __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict__ o, const float* __restrict__ d,
@@ -1446,6 +1592,41 @@ hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q,
         case 512: launch_sil_bs<512, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
         default: launch_sil_bs<64, DIM>(t, q, n, flip, seed, capped, sil, dist, depth, perm, sp, err, st); break;
         }
+    });
+    return hipGetLastError();
+}
+
+template <int DIM, bool STATS>
+static void launch_sil_requeue_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
+                                   uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
+                                   uint32_t* ctr, uint32_t* queue, uint32_t n_cu, uint32_t* err, hipStream_t st)
+{
+    // as many one-wave blocks as are resident at once; more would only wait
+    static int per_cu = 0;
+    if (per_cu == 0 &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sil_requeue<16, STATS, DIM>, 64, 0) != hipSuccess)
+        per_cu = 1;
+    const uint32_t grid = std::max(1u, std::min(n_cu * (uint32_t)std::max(per_cu, 1), (n + 63) / 64));
+    hipLaunchKernelGGL((k_sil_requeue<16, STATS, DIM>), dim3(grid), dim3(64), 0, st, t, q, n, flip, seed, sil, dist,
+                       perm, ctr, queue, err, stats);
+}
+
+hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
+                                             const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
+                                             uint64_t* stats, uint32_t* ctr, uint32_t* queue, uint32_t n_cu,
+                                             hipStream_t st, uint32_t* err)
+{
+    if (n == 0)
+        return hipSuccess;
+    auto* sp = reinterpret_cast<unsigned long long*>(stats);
+    if (!err)
+        err = traversal_error_word();
+    hipLaunchKernelGGL(k_requeue_init, dim3(std::min(nblocks(n, 256), 1024u)), dim3(256), 0, st, ctr, queue, n);
+    SNCH_DISPATCH_DIM(dim, {
+        if (sp)
+            launch_sil_requeue_cfg<DIM, true>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, queue, n_cu, err, st);
+        else
+            launch_sil_requeue_cfg<DIM, false>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, queue, n_cu, err, st);
     });
     return hipGetLastError();
 }

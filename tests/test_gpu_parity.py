@@ -252,11 +252,20 @@ def test_seeded_silhouette_identical(S):
     sc = S.Scene(V, F).build_bvh()
     e0, d0 = (t.cpu().numpy() for t in sc.closest_silhouette(q))
     rng = np.random.default_rng(1)
+    mixed = np.where(rng.random(len(q)) < 0.02, d0 * 0.5, d0 * 1.1)  # 2 % unsettled, scattered
     for seed in (np.zeros(len(q)), d0 * 0.5, d0, d0 * 1.0001 + 1e-3, rng.random(len(q)) * 0.3,
-                 np.full(len(q), -1.0), np.full(len(q), np.nan)):
-        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=seed.astype(np.float32)))
-        np.testing.assert_array_equal(e1, e0)
-        np.testing.assert_array_equal(d1, d0)
+                 np.full(len(q), -1.0), np.full(len(q), np.nan), mixed, np.full(len(q), np.inf)):
+        # unsettled queries re-queued for whole waves (default), or searched again in place
+        for in_place in (False, True):
+            e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=seed.astype(np.float32),
+                                                                     seed_in_place=in_place))
+            np.testing.assert_array_equal(e1, e0)
+            np.testing.assert_array_equal(d1, d0)
+    # a query count that is not a multiple of the 64-query chunk, and a single query
+    for m in (1, 63, 4097):
+        e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q[:m], seed_radius=mixed[:m].astype(np.float32)))
+        np.testing.assert_array_equal(e1, e0[:m])
+        np.testing.assert_array_equal(d1, d0[:m])
     for bs in (64, 128, 256, 512):
         for depth in (8, 16, 24, 32):
             e1, d1 = (t.cpu().numpy() for t in sc.closest_silhouette(q, block_size=bs, stack_depth=depth))
