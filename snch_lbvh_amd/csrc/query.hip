@@ -7,12 +7,13 @@
 //  * Node layout: one 64-B record per internal node holds both children's
 //    boxes and references; the silhouette query reads a 48-B cone record from
 //    a separate stream, so the closest-point and ray queries never touch it.
-//  * Stack: closest point and closest silhouette keep a short per-lane stack
-//    in LDS (16 packed entries, a column per lane so a wave's 64 lanes hit
-//    distinct banks) with a restart trail (TrailStack): nothing in private
-//    memory; a far child pushed out of the full stack is found again from
-//    the trail.  Rays keep the 16 LDS entries over a private overflow array
-//    (the tree depth is at most 64, so that stack never loses an entry).
+//  * Stack: closest point, closest silhouette and rays keep a short per-lane
+//    stack in LDS (16 packed entries, a column per lane so a wave's 64 lanes
+//    hit distinct banks) with a restart trail (TrailStack): nothing in
+//    private memory; a far child pushed out of the full stack is found again
+//    from the trail.  The generic overlap and line queries, which must emit
+//    hits in the reference's visit order, keep 16 LDS entries over a private
+//    overflow array (the tree depth is at most 64, so it never loses one).
 //  * Order: the nearer child is descended first and the farther is pushed,
 //    and a child is skipped when its lower bound exceeds the best answer so
 //    far.  The reference's own pruning predicates (minmaxdist for closest
@@ -37,64 +38,13 @@ namespace snch {
 namespace {
 
 constexpr int SS = 16;  // default LDS stack entries per lane
-constexpr int OVF = 64; // private overflow entries per lane
+constexpr int OVF = 64; // private overflow entries per lane (generic queries)
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
 {
     const uint32_t X = 8, q = nb / X, r = nb % X, x = b % X, k = b / X;
     return x * q + (x < r ? x : r) + k;
 }
-
-// per-lane traversal stack: the top SSZ entries in LDS (a column per lane, so
-// the lanes of a wave hit distinct banks), older entries spilled to a private
-// array; the tree depth is at most 64, so the stack never loses an entry.
-// An entry is {node, key} (8 B), key being the lower bound used to skip the
-// entry when it is popped.
-template <int BS, int SSZ>
-struct LaneStack {
-    using Entry = uint2;
-    Entry* lds; // this lane's column: entry s at lds[s * BS]
-    int sp;
-    uint32_t ovn[OVF];
-    float ovd[OVF];
-    bool overflow;
-
-    __device__ __forceinline__ void init(Entry* base)
-    {
-        lds = base;
-        sp = 0;
-        overflow = false;
-    }
-    __device__ __forceinline__ void push(uint32_t n, float d)
-    {
-        const int s = sp % SSZ;
-        if (sp >= SSZ) {
-            const int o = sp - SSZ;
-            if (o < OVF) {
-                const uint2 old = lds[s * BS];
-                ovn[o] = old.x;
-                ovd[o] = __uint_as_float(old.y);
-            } else {
-                overflow = true;
-                return;
-            }
-        }
-        lds[s * BS] = make_uint2(n, __float_as_uint(d));
-        ++sp;
-    }
-    __device__ __forceinline__ void pop(uint32_t& n, float& d)
-    {
-        --sp;
-        const int s = sp % SSZ;
-        const uint2 e = lds[s * BS];
-        n = e.x;
-        d = __uint_as_float(e.y);
-        if (sp >= SSZ) {
-            const int o = sp - SSZ;
-            lds[s * BS] = make_uint2(ovn[o], __float_as_uint(ovd[o]));
-        }
-    }
-};
 
 // A short stack with a restart trail (Laine 2010, "Restart trail for
 // stackless BVH traversal"), for searches whose pruning bound only tightens:
@@ -916,7 +866,7 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
                                             float* __restrict__ out_uv, uint32_t* __restrict__ out_idx,
                                             uint32_t* __restrict__ err, unsigned long long* stats)
 {
-    __shared__ uint2 stk[SS * BS];
+    __shared__ uint32_t stk[SS * BS];
     const uint32_t qi = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     Counts cnt;
     if (t.n_dev)
@@ -959,19 +909,23 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
     if (t.n_leaves == 1) {
         leaf(0);
     } else {
-        LaneStack<BS, SS> st;
-        st.init(stk + threadIdx.x);
+        // the closest hit only shrinks, so the restart-trail stack applies
+        TrailStack<BS, SS> st;
+        st.init(stk + threadIdx.x, t.qbase);
         uint32_t node = 0;
-        const uint32_t cap_iters = 2 * t.n_leaves + 64;
-        for (uint32_t it = 0; it < cap_iters && node < t.n_leaves - 1; ++it) {
+        const uint32_t cap_iters = 8 * t.n_leaves + 8192;
+        uint32_t it = 0;
+        bool deep = false;
+        for (; it < cap_iters && node < t.n_leaves - 1; ++it) {
             const NodeRec nr = load_node(t.tnode, node);
-            float Ld, Rd;
+            float Ld = F_INF, Rd = F_INF;
             if (STATS) {
                 cnt.node += 1;
                 cnt.box += 2;
             }
-            bool Lh = box_ray(nr.L, ro, inv, max_dist, Ld) && Ld <= tbest;
-            bool Rh = box_ray(nr.R, ro, inv, max_dist, Rd) && Rd <= tbest;
+            const bool bL = box_ray(nr.L, ro, inv, max_dist, Ld), bR = box_ray(nr.R, ro, inv, max_dist, Rd);
+            const bool Lh = bL && Ld <= tbest;
+            const bool Rh = bR && Rd <= tbest;
             uint32_t lf, lc, rf, rc;
             const bool lLeaf = as_leaf(nr.lref, nr.lrun, lf, lc), rLeaf = as_leaf(nr.rref, nr.rrun, rf, rc);
             // nearer child first, as query.cuh:136-149
@@ -995,18 +949,19 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
                 break;
             const bool gL = Lh && !lLeaf && Ld <= tbest;
             const bool gR = Rh && !rLeaf && Rd <= tbest;
-            if (gL && gR) {
-                const bool lnear = !(Rd < Ld);
-                st.push(lnear ? nr.rref : nr.lref, lnear ? Rd : Ld);
-                node = lnear ? nr.lref : nr.rref;
-                continue;
+            if ((gL || gR) && st.lv >= 63u) {
+                deep = true; // deeper than a Karras tree can be
+                break;
             }
-            if (gL || gR) {
-                node = gL ? nr.lref : nr.rref;
+            // near and far by geometry alone (a box the ray misses is far), so
+            // every pass orders a node's children alike; with both boxes hit
+            // this is query.cuh:136-149's order
+            const bool lnear = bL && (!bR || !(Rd < Ld));
+            if (st.descend(lnear ? gL : gR, lnear ? gR : gL, lnear ? nr.lref : nr.rref, lnear ? nr.rref : nr.lref,
+                           lnear ? Rd : Ld, node))
                 continue;
-            }
             bool more = false;
-            while (st.sp > 0) {
+            while (st.M) {
                 uint32_t nn;
                 float dd;
                 st.pop(nn, dd);
@@ -1016,12 +971,14 @@ __global__ __launch_bounds__(BS) void k_ray(TreeView t, const float* __restrict_
                     break;
                 }
             }
-            if (!more)
+            if (more)
+                continue;
+            if (!st.restart())
                 break;
+            node = 0;
         }
-        if (st.overflow)
-            if (err)
-                atomicOr(err, 1u);
+        if ((deep || it >= cap_iters) && err)
+            atomicOr(err, 1u);
     }
     if (out_hit)
         out_hit[qi] = found ? 1 : 0;
