@@ -243,6 +243,23 @@ int snch_scene_export(const snch_scene *scene, uint32_t *nodes, float *boxes, fl
 /* The same for mesh `mesh` of a batch (ids local to the mesh). */
 int snch_scene_export_mesh(const snch_scene *scene, uint32_t mesh, uint32_t *nodes, float *boxes, float *cones,
                            uint32_t *sorted_codes, uint32_t *sorted_idx);
+/* A built mesh's tree for users' own HIP kernels: the reference's
+ * get_bvh_device_ptr() (scene.cuh:1242-1252) feeding query_device calls from
+ * device code (query.cuh:79-423, sample.cuh:7-92).  Pass it to
+ * lbvh::device::bvh_view (include/snch_lbvh/device.hpp).  All pointers are
+ * device pointers into the scene, valid until its next build or destruction.
+ * For a batch, `mesh` picks the mesh (ids local to it); else pass 0. */
+typedef struct snch_device_view {
+    uint32_t dim;      /* 2 or 3 */
+    uint32_t n_leaves; /* primitives of the mesh */
+    uint32_t flags;    /* traversal options recorded by the build */
+    uint32_t qbase;    /* stack key quantisation base of the tree */
+    const void *tnode, *tcone, *tleaf, *tedge, *nbox, *ncone; /* float4 records */
+    const void *V;     /* vertices, float4 */
+    const int32_t *F;  /* the mesh's primitives' vertex indices into V */
+} snch_device_view;
+int snch_scene_device_view(const snch_scene *scene, uint32_t mesh, snch_device_view *out);
+
 /* Edge (3D) or vertex (2D) silhouette table: slots (opp0, lo, hi, opp1) per
  * edge / (prev, v, next, -1) per vertex, and per-primitive owned ids (3 per
  * triangle, 2 per segment, -1 padded).  For a batch: the tables of the meshes

@@ -61,6 +61,15 @@ class QueryOpts(ctypes.Structure):
                 ("seed_is_cap", ctypes.c_uint32), ("seed_retry", ctypes.c_uint32)]
 
 
+class DeviceView(ctypes.Structure):
+    """snch_device_view: a built mesh's tree for users' own HIP kernels
+    (include/snch_lbvh/device.hpp)."""
+    _fields_ = [("dim", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("qbase", ctypes.c_uint32), ("tnode", ctypes.c_void_p), ("tcone", ctypes.c_void_p),
+                ("tleaf", ctypes.c_void_p), ("tedge", ctypes.c_void_p), ("nbox", ctypes.c_void_p),
+                ("ncone", ctypes.c_void_p), ("V", ctypes.c_void_p), ("F", ctypes.c_void_p)]
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_uint32), ("block_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32),
                 ("queries_per_wave", ctypes.c_uint32), ("ms", ctypes.c_float), ("ms_default", ctypes.c_float),
@@ -72,7 +81,7 @@ EXPORTED = [
     "snch_scene_create_tri_batch", "snch_scene_create_seg", "snch_scene_destroy", "snch_scene_mesh_count",
     "snch_scene_info", "snch_scene_build", "snch_scene_build_time", "snch_scene_build_phases",
     "snch_batch_closest_point",
-    "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh",
+    "snch_batch_closest_silhouette", "snch_batch_autotune", "snch_scene_export_mesh", "snch_scene_device_view",
     "snch_query_order", "snch_closest_point", "snch_closest_point_ex", "snch_closest_silhouette",
     "snch_closest_silhouette_ex", "snch_closest_silhouette_noalloc", "snch_ray", "snch_ray_ex",
     "snch_sample_in_sphere", "snch_sample_on_object", "snch_overlap", "snch_line_intersect", "snch_scene_export", "snch_scene_silhouettes",
@@ -96,6 +105,7 @@ def _declare(L):
     L.snch_batch_closest_silhouette.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp, vp]
     L.snch_batch_autotune.argtypes = [vp, u32, vp, vp, u32, c_int, vp, vp]
     L.snch_scene_export_mesh.argtypes = [vp, u32, vp, vp, vp, vp, vp]
+    L.snch_scene_device_view.argtypes = [vp, u32, vp]
     L.snch_scene_destroy.argtypes = [vp]
     L.snch_scene_info.argtypes = [vp, vp, vp, vp]
     L.snch_scene_build.argtypes = [vp, vp, vp]
@@ -384,6 +394,13 @@ class Scene:
                                         _stream()))
         return cnt, idx, t
 
+    def device_view(self):
+        """snch_scene_device_view: the built tree for users' own HIP kernels
+        (include/snch_lbvh/device.hpp), a DeviceView of device pointers."""
+        v = DeviceView()
+        check(lib().snch_scene_device_view(self._h, 0, ctypes.byref(v)))
+        return v
+
     def export(self):
         """The tree in the reference layout (host numpy arrays)."""
         N = self.n_prims
@@ -519,6 +536,12 @@ class BatchScene:
         check(lib().snch_batch_autotune(self._h, 1 if kind == "silhouette" else 0, _ptr(q), q_off.ctypes.data,
                                         sample_per_mesh, int(bool(flip)), ctypes.byref(r), _stream()))
         return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+    def device_view(self, mesh):
+        """snch_scene_device_view for mesh `mesh` of the batch."""
+        v = DeviceView()
+        check(lib().snch_scene_device_view(self._h, mesh, ctypes.byref(v)))
+        return v
 
     def export(self, mesh):
         V, F = self.meshes[mesh]

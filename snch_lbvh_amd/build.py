@@ -48,6 +48,26 @@ def build(verbose=False):
     return OUT
 
 
+def build_device_api_test(verbose=False):
+    """Compile tests/cpp/device_api.hip (a user's kernel on the device header
+    include/snch_lbvh/device.hpp) into tests/cpp/libdevice_api.so, linked
+    against libsnch.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "device_api.hip")
+    out = os.path.join(root, "tests", "cpp", "libdevice_api.so")
+    if not os.path.exists(OUT):
+        build(verbose)
+    cmd = [HIPCC, *FLAGS, "-shared", "-I", os.path.join(root, "include"), src, "-L", HERE, "-lsnch",
+           "-Wl,-rpath,$ORIGIN/../../snch_lbvh_amd", "-o", out + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for device_api.hip:\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

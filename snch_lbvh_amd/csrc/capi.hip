@@ -963,6 +963,27 @@ int snch_scene_export(const snch_scene* s, uint32_t* nodes, float* boxes, float*
     return snch_scene_export_mesh(s, 0, nodes, boxes, cones, sorted_codes, sorted_idx);
 }
 
+int snch_scene_device_view(const snch_scene* s, uint32_t mesh, snch_device_view* out)
+{
+    SNCH_NEED_BUILT(s, "snch_scene_device_view");
+    SNCH_CHECK_ARG(out, "snch_scene_device_view: null pointer");
+    SNCH_CHECK_ARG(mesh < s->n_meshes, "snch_scene_device_view: mesh index out of range");
+    const TreeView t = s->view(mesh);
+    out->dim = s->dim;
+    out->n_leaves = t.n_leaves;
+    out->flags = t.flags;
+    out->qbase = t.qbase;
+    out->tnode = t.tnode;
+    out->tcone = t.tcone;
+    out->tleaf = t.tleaf;
+    out->tedge = t.tedge;
+    out->nbox = t.nbox;
+    out->ncone = t.ncone;
+    out->V = s->dV.p;
+    out->F = s->dF.as<int32_t>() + (size_t)s->dim * s->mesh_off[mesh];
+    return SNCH_OK;
+}
+
 int snch_scene_export_mesh(const snch_scene* s, uint32_t mesh, uint32_t* nodes, float* boxes, float* cones,
                            uint32_t* sorted_codes, uint32_t* sorted_idx)
 {

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "traverse.hpp"
 
 namespace snch {
 
@@ -168,10 +169,7 @@ struct TrailStack {
     }
 };
 
-// per-query visit counters, summed over the wave before one atomic per wave
-struct Counts {
-    uint32_t box = 0, leaf = 0, edge = 0, node = 0, cone = 0;
-};
+// per-query visit counters (Counts), summed over the wave before one atomic per wave
 __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long* stats)
 {
     unsigned long long v[5] = {c.box, c.leaf, c.edge, c.node, c.cone};
@@ -188,79 +186,7 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
     }
 }
 
-struct NodeRec {
-    box3 L, R;
-    uint32_t lref, rref, lrun, rrun;
-};
-
-__device__ __forceinline__ NodeRec load_node(const float4* __restrict__ tnode, uint32_t i)
-{
-    const float4 a = tnode[4 * i], b = tnode[4 * i + 1], c = tnode[4 * i + 2], r = tnode[4 * i + 3];
-    NodeRec n;
-    n.L.lo = mk(a.x, a.y, a.z);
-    n.L.hi = mk(a.w, b.x, b.y);
-    n.R.lo = mk(b.z, b.w, c.x);
-    n.R.hi = mk(c.y, c.z, c.w);
-    n.lref = __float_as_uint(r.x);
-    n.rref = __float_as_uint(r.y);
-    n.lrun = __float_as_uint(r.z);
-    n.rrun = __float_as_uint(r.w);
-    return n;
-}
-
-// a child the traversal treats as a leaf: a true leaf, or a run of sorted
-// leaves [first, first + count) under an internal node (leaf size > 1)
-__device__ __forceinline__ bool as_leaf(uint32_t ref, uint32_t run, uint32_t& first, uint32_t& count)
-{
-    if (ref & LEAF_BIT) {
-        first = ref & ~LEAF_BIT;
-        count = 1;
-        return true;
-    }
-    if (run != INVALID) {
-        first = run & RUN_FIRST_MASK;
-        count = (run >> RUN_SHIFT) + 1;
-        return true;
-    }
-    return false;
-}
-
-__device__ __forceinline__ box3 ldbox2(const float4* nb, uint32_t n)
-{
-    const float4 a = nb[2 * n], c = nb[2 * n + 1];
-    box3 b;
-    b.lo = mk(a.x, a.y, a.z);
-    b.hi = mk(c.x, c.y, c.z);
-    return b;
-}
-
-template <int DIM>
-__device__ __forceinline__ v3 load_point(const float* __restrict__ q, uint32_t i)
-{
-    return DIM == 3 ? mk(q[3 * i], q[3 * i + 1], q[3 * i + 2]) : mk(q[2 * i], q[2 * i + 1], 0.0f);
-}
-
 // ------------------------------------------------------------ closest point
-template <int DIM>
-__device__ __forceinline__ void cp_leaf(const TreeView& t, uint32_t j, v3 p, float& best, uint32_t& best_idx)
-{
-    if (j >= t.n_leaves)
-        return;
-    const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1];
-    float d;
-    if (DIM == 3) {
-        const float4 c = t.tleaf[3 * j + 2];
-        d = closest_point_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), p);
-    } else {
-        d = closest_point_segment2d(mk(a.x, a.y, 0.0f), mk(b.x, b.y, 0.0f), p); // scene.cuh:504-515
-    }
-    d *= d; // query.cuh:284-285
-    const uint32_t o = __float_as_uint(a.w);
-    if (d < best || (d == best && o < best_idx)) {
-        best = d;
-        best_idx = o;
-    }
-}
 
 // query_device(nearest) (query.cuh:238-318) for one query point; best is the
 // squared distance
@@ -365,56 +291,6 @@ __global__ __launch_bounds__(BS) void k_closest_point(TreeView t, const float* _
 }
 
 // ------------------------------------------------------------ silhouette
-// one leaf: its owned silhouettes (edges in 3D, scene.cuh:978-1003 and
-// :788-824; vertices in 2D, scene.cuh:517-541 and :354-386).  The record's
-// w holds the silhouette id reported to the caller.
-template <int DIM>
-__device__ __forceinline__ void sil_leaf(const TreeView& t, uint32_t j, v3 p, bool flip, float& best,
-                                         uint32_t& best_e, Counts& cnt)
-{
-    if (j >= t.n_leaves)
-        return;
-    const uint32_t es = __float_as_uint(t.tleaf[3 * j + 1].w);
-    const uint32_t ec = __float_as_uint(t.tleaf[3 * j + 2].w);
-    cnt.leaf += 1;
-    cnt.edge += ec;
-    for (uint32_t k = 0; k < ec; ++k) {
-        const uint32_t e = es + k;
-        const float mr2 = best * best;
-        if (!(0.0f < mr2))
-            break; // min_radius_squared (0) >= max_radius_squared
-        const float4 A = t.tedge[4 * e], B = t.tedge[4 * e + 1];
-        const uint32_t id = __float_as_uint(A.w);
-        const uint32_t fl = __float_as_uint(B.w);
-        float d;
-        bool sil = (fl & 3u) != 3u; // a boundary edge / end vertex is always a silhouette
-        if (DIM == 3) {
-            const v3 pa = mk(A.x, A.y, A.z), pb = mk(B.x, B.y, B.z);
-            v3 cp;
-            d = closest_point_segment(pa, pb, p, cp);
-            if (d * d > mr2)
-                continue;
-            if (!sil) {
-                const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
-                sil = is_silhouette_edge(pa, pb, mk(N0.x, N0.y, N0.z), mk(N1.x, N1.y, N1.z),
-                                         mk(p.x - cp.x, p.y - cp.y, p.z - cp.z), d, flip);
-            }
-        } else {
-            const float vx = p.x - A.x, vy = p.y - A.y;
-            d = len2(vx, vy);
-            if (d * d > mr2)
-                continue;
-            if (!sil) {
-                const float4 N0 = t.tedge[4 * e + 2], N1 = t.tedge[4 * e + 3];
-                sil = is_silhouette_vertex(mk(N0.x, N0.y, 0.0f), mk(N1.x, N1.y, 0.0f), vx, vy, d, flip);
-            }
-        }
-        if (sil && (d < best || (d == best && id < best_e))) {
-            best = d;
-            best_e = id;
-        }
-    }
-}
 
 // Per-wave LDS of the flattened leaf tests: one round of 64 edge slots (the
 // edge, then its distance with the sign bit set when it is not a silhouette),
@@ -516,12 +392,6 @@ __device__ __forceinline__ void sil_leaves_wave(const TreeView& t, EdgeLds& L, b
     }
 }
 
-template <int DIM>
-__device__ __forceinline__ bool overlap_dim(const float4& C, float radius, v3 p, const box3& b, float md)
-{
-    return DIM == 3 ? cone_overlap(mk(C.x, C.y, C.z), C.w, radius, p, b, md)
-                    : cone_overlap2d(mk(C.x, C.y, 0.0f), C.w, radius, p, b, md);
-}
 
 // query_device(nearest_silhouette) (query.cuh:325-423) for one query point.
 // best / best_e receive the answer.  The stack is a TrailStack: SSZ entries
@@ -1129,65 +999,13 @@ __global__ __launch_bounds__(256) void k_sample(TreeView t, const float4* __rest
     if (qi >= n)
         return;
     const v3 c = load_point<DIM>(cc, qi);
-    const float r = rr[qi];
-    float u = uu[3 * qi];
-    float npdf = 1.0f;
-    int32_t obj = -1;
-    float pdf = 0.0f;
-    uint32_t ref = t.n_leaves == 1 ? LEAF_BIT : 0u;
-    for (int depth = 0; depth < 96; ++depth) { // a single descent: at most the tree depth
-        if (ref & LEAF_BIT) {
-            const uint32_t j = ref & ~LEAF_BIT;
-            if (j >= t.n_leaves)
-                break;
-            const float4 a = t.tleaf[3 * j], b = t.tleaf[3 * j + 1], d = t.tleaf[3 * j + 2];
-            const v3 pa = mk(a.x, a.y, a.z), pb = mk(b.x, b.y, b.z), pc = mk(d.x, d.y, d.z);
-            if (DIM == 3 ? tri_sphere(pa, pb, pc, c, r) : sphere_segment(pa, pb, c, r)) {
-                obj = (int32_t)__float_as_uint(a.w);
-                pdf = npdf;
-                pdf /= DIM == 3 ? tri_area(pa, pb, pc) : len2(pa.x - pb.x, pa.y - pb.y); // scene.cuh:409-418
-            }
-            break;
-        }
-        if (ref >= t.n_leaves - 1)
-            break;
-        const NodeRec nr = load_node(t.tnode, ref);
-        const float Lw = box_sphere(nr.L, c, r)
-                             ? (DIM == 3 ? green_weight(c, box_centroid(nr.L)) : green_weight2d(c, box_centroid(nr.L)))
-                             : 0.0f;
-        const float Rw = box_sphere(nr.R, c, r)
-                             ? (DIM == 3 ? green_weight(c, box_centroid(nr.R)) : green_weight2d(c, box_centroid(nr.R)))
-                             : 0.0f;
-        const float tw = Lw + Rw;
-        if (!(tw > 0.0f))
-            break;
-        const float Lp = Lw / tw;
-        if (u < Lp) {
-            u /= Lp;
-            npdf = Lp * npdf;
-            ref = nr.lref;
-        } else {
-            const float Rp = 1.0f - Lp;
-            u = (u - Lp) / Rp;
-            npdf = Rp * npdf;
-            ref = nr.rref;
-        }
-    }
+    int32_t obj;
+    float pdf;
+    v3 pt;
+    sample_one<DIM>(t, V, F, c, rr[qi], uu[3 * qi], uu[3 * qi + 1], uu[3 * qi + 2], obj, pdf, pt);
     out_idx[qi] = obj;
-    out_pdf[qi] = obj >= 0 ? pdf : 0.0f;
+    out_pdf[qi] = pdf;
     if (out_pt) {
-        v3 pt = mk(0.0f, 0.0f, 0.0f);
-        if (obj >= 0) {
-            if (DIM == 3) {
-                const float4 a = V[F[3 * obj]], b = V[F[3 * obj + 1]], d = V[F[3 * obj + 2]];
-                pt = sample_triangle(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), uu[3 * qi + 1],
-                                     uu[3 * qi + 2]);
-            } else { // sample_line (scene.cuh:29-32)
-                const float4 a = V[F[2 * obj]], b = V[F[2 * obj + 1]];
-                const float u1 = uu[3 * qi + 1];
-                pt = mk(a.x + u1 * (b.x - a.x), a.y + u1 * (b.y - a.y), 0.0f);
-            }
-        }
         if (DIM == 3) {
             out_pt[3 * qi] = pt.x;
             out_pt[3 * qi + 1] = pt.y;
