@@ -321,7 +321,9 @@ int snch_wost_solve(snch_wost *wost, const float *x0, uint32_t n, uint32_t max_s
  * process ends) instead of per-scene buffers grown with hipMalloc; n above
  * SNCH_NOALLOC_MAX_QUERIES is rejected with SNCH_ERR_INVALID.  Every call on a
  * GPU shares that GPU's array, so calls that overlap on one GPU (other
- * streams, other threads) race and must be serialised by the caller. */
+ * streams, other threads) race and must be serialised by the caller.  With a
+ * seed radius, queries not settled within it are searched again at once by
+ * their own lane (snch_query_opts.seed_retry = 1), which needs no queue. */
 #define SNCH_NOALLOC_MAX_QUERIES (1u << 22)
 int snch_closest_silhouette_noalloc(const snch_scene *scene, const float *q, uint32_t n, int flip,
                                     const float *seed_radius, uint32_t *out_sil, float *out_dist, void *stream);
