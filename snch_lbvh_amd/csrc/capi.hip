@@ -669,12 +669,20 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     // a seed radius, not a hard cap: unsettled queries are re-queued for whole
     // waves unless asked to search again in place (default launch shape only)
     const bool requeue = seed && !capped && n > 0 && !(opts && opts->seed_retry == 1) && block == 64 && depth == 16;
-    if (own_order || requeue)
+    // no seed: a persistent launch taking 64-query chunks (default launch shape)
+    const bool persist = !seed && n > 0 && block == 64 && depth == 16;
+    if (own_order || requeue || persist)
         SNCH_HIP(scratch_begin(s, st), "snch_closest_silhouette: stream wait");
     if (own_order)
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
-    if (requeue) {
+    if (persist) {
+        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
+        const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
+        SNCH_HIP(launch_closest_silhouette_persist(s->view(), (int)s->dim, q, n, flip, out_sil, out_dist, perm,
+                                                   opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(), n_cu, st),
+                 "snch_closest_silhouette");
+    } else if (requeue) {
         SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
         SNCH_HIP(s->rq_queue.ensure(sizeof(uint32_t) * (size_t)n), "snch_closest_silhouette: allocation");
         const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
@@ -687,7 +695,7 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
                                            block, depth, perm, opts ? opts->stats : nullptr, st),
                  "snch_closest_silhouette");
     }
-    if (own_order || requeue)
+    if (own_order || requeue || persist)
         SNCH_HIP(scratch_end(s, st), "snch_closest_silhouette: event");
     return SNCH_OK;
 }

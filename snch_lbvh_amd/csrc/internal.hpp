@@ -159,6 +159,11 @@ hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const f
                                              const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
                                              uint64_t* stats, uint32_t* ctr, uint32_t* queue, uint32_t n_cu,
                                              hipStream_t st, uint32_t* err = nullptr);
+// plain search on a persistent launch taking 64-query chunks (ctr: 10 words,
+// rewritten at launch)
+hipError_t launch_closest_silhouette_persist(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
+                                             uint32_t* sil, float* dist, const uint32_t* perm, uint64_t* stats,
+                                             uint32_t* ctr, uint32_t n_cu, hipStream_t st, uint32_t* err = nullptr);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);
