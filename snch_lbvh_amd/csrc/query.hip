@@ -612,6 +612,27 @@ __global__ __launch_bounds__(256) void k_requeue_init(uint32_t* __restrict__ ctr
 // wave-uniform copy of lane 0's value (a scalar register)
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// the next 64-query chunk of nchunks for a one-wave block of a persistent
+// launch: its own XCD's contiguous range first (x0: the block's XCD), then
+// the other XCDs' ranges; xi counts the ranges exhausted (wave-uniform).
+// ctr: [0, 8) the ranges' counters.  INVALID when none is left.
+__device__ __forceinline__ uint32_t take_chunk(uint32_t* ctr, uint32_t nchunks, uint32_t x0, uint32_t& xi)
+{
+    while (xi < 8u) {
+        const uint32_t xx = (x0 + xi) & 7u;
+        const uint32_t lo = (uint32_t)((uint64_t)nchunks * xx / 8u);
+        const uint32_t hi = (uint32_t)((uint64_t)nchunks * (xx + 1u) / 8u);
+        uint32_t k = 0;
+        if (threadIdx.x == 0)
+            k = atomicAdd(ctr + xx, 1u);
+        k = uniform(k);
+        if (k < hi - lo)
+            return lo + k;
+        ++xi;
+    }
+    return INVALID;
+}
+
 template <int SSZ, bool STATS, int DIM>
 __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, const float* __restrict__ q,
                                                                  uint32_t n, int flip, const float* __restrict__ seed,
@@ -756,26 +777,12 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_persist(TreeView t, c
     bool ovf = false;
     Counts cnt;
     for (;;) {
-        uint32_t c = INVALID;
-        while (xi < 8u) {
-            const uint32_t xx = (x0 + xi) & 7u;
-            const uint32_t lo = (uint32_t)((uint64_t)nchunks * xx / 8u);
-            const uint32_t hi = (uint32_t)((uint64_t)nchunks * (xx + 1u) / 8u);
-            uint32_t k = 0;
-            if (lane == 0)
-                k = atomicAdd(ctr + xx, 1u);
-            k = uniform(k);
-            if (k < hi - lo) {
-                c = lo + k;
-                break;
-            }
-            ++xi;
-        }
+        const uint32_t c = take_chunk(ctr, nchunks, x0, xi);
         if (c == INVALID)
             break;
         const uint32_t slot = c * 64u + lane;
-        if (slot < n) {
-            const uint32_t qi = perm ? perm[slot] : slot;
+        const uint32_t qi = slot < n ? (perm ? perm[slot] : slot) : INVALID;
+        if (qi < n) {
             float best;
             uint32_t best_e;
             sil_query<64, SSZ, STATS, DIM, false, true>(t, load_point<DIM>(q, qi), flip != 0, F_INF, stk + lane,
