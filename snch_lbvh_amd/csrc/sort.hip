@@ -5,7 +5,7 @@
 //   k_sort_init   clears the per-pass digit totals and tile counters;
 //   k_hist_all    one read of the keys (and segment ids) counts the digits
 //                 of every pass at once, and clears the look-back words;
-//   k_onesweep    per pass: a workgroup takes the next tile (4096 pairs) in
+//   k_onesweep    per pass: a workgroup takes the next tile (4096 or 8192 pairs) in
 //                 launch order, ranks its keys with eight wave64 ballots per
 //                 64-key row (the lanes holding the same digit form a peer
 //                 mask) and per-wave running counts in LDS, publishes the
@@ -36,8 +36,15 @@ namespace snch {
 namespace {
 
 constexpr int ST = 512;          // threads per tile
-constexpr int SI = 8;            // keys per thread
-constexpr int TILE = ST * SI;    // 4096 keys
+// keys per thread: tiles of 4096 keys below LARGE_N, 8192 from there on.
+// Larger tiles spread the per-tile fixed costs (tile number, digit base,
+// look-back) over twice the keys and leave longer runs of equal digits per
+// store; below ~1.5M keys they would leave CUs without a tile.  Measured per
+// sort of 30-bit keys: 8.4M 304 -> 255 us, 4.2M 195 -> 158, 2.1M 126 -> 109,
+// 1M 87 against 91 with the larger tiles.
+constexpr int SI_SMALL = 8, SI_LARGE = 16;
+constexpr uint32_t LARGE_N = 3u << 19;
+constexpr int TILE_MIN = ST * SI_SMALL; // sizes the look-back words
 constexpr int RADIX = 256;
 constexpr int WAVES = ST / 64;
 constexpr int MAX_PASSES = 8;    // up to 4 key digits and 4 segment digits
@@ -61,11 +68,13 @@ __global__ void k_sort_init(uint32_t* __restrict__ ghist, uint32_t* __restrict__
 // codes of neighbouring primitives mostly share their high bytes), one lane
 // adds the wave's total.  Same-address LDS atomics from many lanes run one
 // lane at a time, and the LDS is shared by the CU's four SIMDs.
+template <int SI>
 __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                  const uint32_t* __restrict__ seg, uint32_t n, int key_passes,
                                                  int seg_passes, uint32_t ntiles, uint32_t* __restrict__ ghist,
                                                  uint32_t* __restrict__ status)
 {
+    constexpr int TILE = ST * SI;
     __shared__ uint32_t h[MAX_PASSES][RADIX];
     const uint32_t tid = threadIdx.x;
     const int passes = key_passes + seg_passes;
@@ -79,10 +88,14 @@ __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ ke
         uint32_t kk[SI], sg[SI];
         uint32_t nv = 0; // valid keys of this thread (a prefix of its run)
         if (base + SI <= n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0u) {
-            const uint4 a = *reinterpret_cast<const uint4*>(keys + base);
-            const uint4 b = *reinterpret_cast<const uint4*>(keys + base + 4);
-            kk[0] = a.x; kk[1] = a.y; kk[2] = a.z; kk[3] = a.w;
-            kk[4] = b.x; kk[5] = b.y; kk[6] = b.z; kk[7] = b.w;
+#pragma unroll
+            for (int c = 0; c < SI / 4; ++c) {
+                const uint4 a = *reinterpret_cast<const uint4*>(keys + base + 4 * c);
+                kk[4 * c] = a.x;
+                kk[4 * c + 1] = a.y;
+                kk[4 * c + 2] = a.z;
+                kk[4 * c + 3] = a.w;
+            }
             nv = SI;
         } else {
 #pragma unroll
@@ -93,7 +106,7 @@ __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ ke
 #pragma unroll
         for (int i = 0; i < SI; ++i)
             sg[i] = seg_passes > 0 && (uint32_t)i < nv ? seg[vals[base + i]] : 0u;
-        static_assert(SI == 8, "two 16-byte loads per thread");
+        static_assert(SI % 4 == 0, "16-byte loads per thread");
         for (int p = 0; p < passes; ++p) {
             const bool on_key = p < key_passes;
             const int sh = 8 * (on_key ? p : p - key_passes);
@@ -137,20 +150,21 @@ __global__ __launch_bounds__(ST) void k_hist_all(const uint32_t* __restrict__ ke
 }
 
 // one pass.  SEG: the digit comes from seg[value] instead of the key
-template <bool SEG>
+template <bool SEG, int SI>
 __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   const uint32_t* __restrict__ seg, uint32_t* __restrict__ kout,
                                                   uint32_t* __restrict__ vout, uint32_t n, int shift,
                                                   const uint32_t* __restrict__ ghist, uint32_t* status,
                                                   uint32_t* tile_ctr, uint32_t* err)
 {
+    constexpr int TILE = ST * SI;
     __shared__ uint32_t wcnt[WAVES][RADIX];
     __shared__ uint32_t toff[RADIX];
     __shared__ uint32_t wsum[WAVES];
     __shared__ uint32_t tile_s;
     __shared__ uint32_t gdelta[RADIX];
     __shared__ uint32_t sk[TILE], svl[TILE];
-    __shared__ uint8_t sd[TILE];
+    __shared__ uint8_t sd[SEG ? TILE : 1]; // the digit (from seg[value]); keys carry their own
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0)
         tile_s = atomicAdd(tile_ctr, 1u);
@@ -292,7 +306,8 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
             const uint32_t lp = min(wcnt[w][d] + rank[i], (uint32_t)TILE - 1u); // < TILE when ranks agree
             sk[lp] = k[i];
             svl[lp] = v[i];
-            sd[lp] = (uint8_t)d;
+            if (SEG)
+                sd[lp] = (uint8_t)d;
         }
     }
     __syncthreads();
@@ -301,7 +316,7 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
     for (int i = 0; i < SI; ++i) {
         const uint32_t j = i * ST + tid;
         if (j < tile_n) {
-            const uint32_t dst = j + gdelta[sd[j]];
+            const uint32_t dst = j + gdelta[SEG ? (uint32_t)sd[j] : (sk[j] >> shift) & 0xFFu];
             if (dst < n) { // always true when histogram and ranks agree
                 kout[dst] = sk[j];
                 vout[dst] = svl[j];
@@ -314,7 +329,7 @@ __global__ __launch_bounds__(ST) void k_onesweep(const uint32_t* __restrict__ ki
 
 size_t radix_sort_scratch_bytes(uint32_t n, int passes)
 {
-    const size_t ntiles = (n + TILE - 1) / TILE;
+    const size_t ntiles = (n + TILE_MIN - 1) / TILE_MIN;
     return sizeof(uint32_t) * ((size_t)MAX_PASSES * RADIX + 2 * MAX_PASSES + (size_t)passes * ntiles * RADIX);
 }
 
@@ -352,25 +367,37 @@ hipError_t radix_sort_pairs_in(uint32_t* keys, uint32_t* vals, uint32_t* keys_al
         return hipErrorInvalidValue;
     if (passes == 0)
         return hipSuccess;
-    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    const bool large = n >= LARGE_N;
+    const uint32_t tile_keys = ST * (large ? SI_LARGE : SI_SMALL);
+    const uint32_t ntiles = (n + tile_keys - 1) / tile_keys;
     uint32_t* ghist = hist;
     uint32_t* tile_ctr = ghist + MAX_PASSES * RADIX;
     uint32_t* status = tile_ctr + 2 * MAX_PASSES;
     if (!pre_cleared)
         hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, st, ghist, tile_ctr);
-    hipLaunchKernelGGL(k_hist_all, dim3(std::min(ntiles, HIST_BLOCKS)), dim3(ST), 0, st, keys, vals, seg, n,
-                       key_passes, seg_passes, ntiles, ghist, status);
+    if (large)
+        hipLaunchKernelGGL(k_hist_all<SI_LARGE>, dim3(std::min(ntiles, HIST_BLOCKS)), dim3(ST), 0, st, keys, vals,
+                           seg, n, key_passes, seg_passes, ntiles, ghist, status);
+    else
+        hipLaunchKernelGGL(k_hist_all<SI_SMALL>, dim3(std::min(ntiles, HIST_BLOCKS)), dim3(ST), 0, st, keys, vals,
+                           seg, n, key_passes, seg_passes, ntiles, ghist, status);
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
     for (int pass = 0; pass < passes; ++pass) {
         const bool sp = pass >= key_passes;
         const int shift = 8 * (sp ? pass - key_passes : pass);
         uint32_t* pstat = status + (size_t)pass * ntiles * RADIX;
-        if (sp)
-            hipLaunchKernelGGL(k_onesweep<true>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
-                               ghist + pass * RADIX, pstat, tile_ctr + pass, err);
+#define SNCH_ONESWEEP(SEG_, SI_)                                                                                 \
+    hipLaunchKernelGGL((k_onesweep<SEG_, SI_>), dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,      \
+                       ghist + pass * RADIX, pstat, tile_ctr + pass, err)
+        if (sp && large)
+            SNCH_ONESWEEP(true, SI_LARGE);
+        else if (sp)
+            SNCH_ONESWEEP(true, SI_SMALL);
+        else if (large)
+            SNCH_ONESWEEP(false, SI_LARGE);
         else
-            hipLaunchKernelGGL(k_onesweep<false>, dim3(ntiles), dim3(ST), 0, st, ki, vi, seg, ko, vo, n, shift,
-                               ghist + pass * RADIX, pstat, tile_ctr + pass, err);
+            SNCH_ONESWEEP(false, SI_SMALL);
+#undef SNCH_ONESWEEP
         uint32_t* t = ki;
         ki = ko;
         ko = t;
