@@ -669,8 +669,8 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     // a seed radius, not a hard cap: unsettled queries are re-queued for whole
     // waves unless asked to search again in place (default launch shape only)
     const bool requeue = seed && !capped && n > 0 && !(opts && opts->seed_retry == 1) && block == 64 && depth == 16;
-    // no seed: a persistent launch taking 64-query chunks (default launch shape)
-    const bool persist = !seed && n > 0 && block == 64 && depth == 16;
+    // no seed or a hard cap: a persistent launch taking 64-query chunks (default launch shape)
+    const bool persist = (!seed || capped) && n > 0 && block == 64 && depth == 16;
     if (own_order || requeue || persist)
         SNCH_HIP(scratch_begin(s, st), "snch_closest_silhouette: stream wait");
     if (own_order)
@@ -679,8 +679,9 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
     if (persist) {
         SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
         const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
-        SNCH_HIP(launch_closest_silhouette_persist(s->view(), (int)s->dim, q, n, flip, out_sil, out_dist, perm,
-                                                   opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(), n_cu, st),
+        SNCH_HIP(launch_closest_silhouette_persist(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist,
+                                                   perm, opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(), n_cu,
+                                                   st),
                  "snch_closest_silhouette");
     } else if (requeue) {
         SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");

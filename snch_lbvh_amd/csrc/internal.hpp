@@ -159,11 +159,14 @@ hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const f
                                              const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
                                              uint64_t* stats, uint32_t* ctr, uint32_t* queue, uint32_t n_cu,
                                              hipStream_t st, uint32_t* err = nullptr);
-// plain search on a persistent launch taking 64-query chunks (ctr: 10 words,
-// rewritten at launch)
+// the search on a persistent launch taking 64-query chunks: plain (seed
+// null) or within the radius seed[qi] only (a hard cap); over the first
+// min(n, *t.n_dev) queries when the count is on the device (capped only).
+// ctr: 10 words, rewritten at launch (by a kernel: graph capture is fine)
 hipError_t launch_closest_silhouette_persist(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                             uint32_t* sil, float* dist, const uint32_t* perm, uint64_t* stats,
-                                             uint32_t* ctr, uint32_t n_cu, hipStream_t st, uint32_t* err = nullptr);
+                                             const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
+                                             uint64_t* stats, uint32_t* ctr, uint32_t n_cu, hipStream_t st,
+                                             uint32_t* err = nullptr);
 hipError_t launch_ray(const TreeView& t, int dim, const float* o, const float* d, const float* tmax, uint32_t n,
                       int any_hit, uint8_t* hit, float* tout, float* uv, uint32_t* idx, uint64_t* stats,
                       hipStream_t st);

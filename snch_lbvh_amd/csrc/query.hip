@@ -759,10 +759,14 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
 // 64-query block per chunk.  Handing the few lanes left in a wave to fresh
 // waves (suspend, resume from the trail) was measured slower: 81-90 ms with
 // the suspended queries kept in visiting order, ~140 ms in suspension order.
-// ctr: [0, 8) the chunk counters of the XCD ranges (zero at launch).
-template <int SSZ, bool STATS, int DIM>
-__global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_persist(TreeView t, const float* __restrict__ q,
-                                                                 uint32_t n, int flip, uint32_t* __restrict__ out_e,
+// CAPPED: the search within the radius seed[qi] only (as k_closest_silhouette's
+// MODE 2; walk on stars), over the first min(n, *t.n_dev) queries when the
+// count is on the device.  ctr: [0, 8) the chunk counters of the XCD ranges
+// (zero at launch).
+template <int SSZ, bool STATS, int DIM, bool CAPPED>
+__global__ __launch_bounds__(64, STATS || CAPPED ? 5 : 6) void k_sil_persist(TreeView t, const float* __restrict__ q,
+                                                                 uint32_t n, int flip, const float* __restrict__ seed,
+                                                                 uint32_t* __restrict__ out_e,
                                                                  float* __restrict__ out_d,
                                                                  const uint32_t* __restrict__ perm, uint32_t* ctr,
                                                                  uint32_t* __restrict__ err,
@@ -771,6 +775,8 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_persist(TreeView t, c
     __shared__ uint32_t stk[SSZ * 64];
     __shared__ EdgeLds elds[1];
     const uint32_t lane = threadIdx.x;
+    if (CAPPED && t.n_dev)
+        n = min(n, *t.n_dev);
     const uint32_t nchunks = (n + 63) / 64;
     const uint32_t x0 = blockIdx.x % 8u; // blocks are dispatched round-robin over the 8 XCDs
     uint32_t xi = 0;                     // ranges exhausted so far, own range first (wave-uniform)
@@ -785,8 +791,15 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_persist(TreeView t, c
         if (qi < n) {
             float best;
             uint32_t best_e;
-            sil_query<64, SSZ, STATS, DIM, false, true>(t, load_point<DIM>(q, qi), flip != 0, F_INF, stk + lane,
-                                                        DIM == 3 ? elds : nullptr, best, best_e, cnt, ovf);
+            float cap = F_INF;
+            if (CAPPED) {
+                cap = seed[qi];
+                if (!(cap >= 0.0f))
+                    cap = F_INF;
+            }
+            // hard-capped searches are short: leaves are tested at once (no parking)
+            sil_query<64, SSZ, STATS, DIM, false, !CAPPED>(t, load_point<DIM>(q, qi), flip != 0, cap, stk + lane,
+                                                           DIM == 3 ? elds : nullptr, best, best_e, cnt, ovf);
             if (out_e)
                 out_e[qi] = best_e;
             out_d[qi] = best;
@@ -1476,24 +1489,25 @@ hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const f
     return hipGetLastError();
 }
 
-template <int DIM, bool STATS>
-static void launch_sil_persist_cfg(const TreeView& t, const float* q, uint32_t n, int flip, uint32_t* sil,
-                                   float* dist, const uint32_t* perm, unsigned long long* stats, uint32_t* ctr,
-                                   uint32_t n_cu, uint32_t* err, hipStream_t st)
+template <int DIM, bool STATS, bool CAPPED>
+static void launch_sil_persist_cfg(const TreeView& t, const float* q, uint32_t n, int flip, const float* seed,
+                                   uint32_t* sil, float* dist, const uint32_t* perm, unsigned long long* stats,
+                                   uint32_t* ctr, uint32_t n_cu, uint32_t* err, hipStream_t st)
 {
     // as many one-wave blocks as are resident at once; more would only wait
     static int per_cu = 0;
-    if (per_cu == 0 &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sil_persist<16, STATS, DIM>, 64, 0) != hipSuccess)
+    if (per_cu == 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sil_persist<16, STATS, DIM, CAPPED>,
+                                                                     64, 0) != hipSuccess)
         per_cu = 1;
     const uint32_t grid = std::max(1u, std::min(n_cu * (uint32_t)std::max(per_cu, 1), (n + 63) / 64));
-    hipLaunchKernelGGL((k_sil_persist<16, STATS, DIM>), dim3(grid), dim3(64), 0, st, t, q, n, flip, sil, dist, perm,
-                       ctr, err, stats);
+    hipLaunchKernelGGL((k_sil_persist<16, STATS, DIM, CAPPED>), dim3(grid), dim3(64), 0, st, t, q, n, flip, seed, sil,
+                       dist, perm, ctr, err, stats);
 }
 
 hipError_t launch_closest_silhouette_persist(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
-                                             uint32_t* sil, float* dist, const uint32_t* perm, uint64_t* stats,
-                                             uint32_t* ctr, uint32_t n_cu, hipStream_t st, uint32_t* err)
+                                             const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
+                                             uint64_t* stats, uint32_t* ctr, uint32_t n_cu, hipStream_t st,
+                                             uint32_t* err)
 {
     if (n == 0)
         return hipSuccess;
@@ -1502,10 +1516,14 @@ hipError_t launch_closest_silhouette_persist(const TreeView& t, int dim, const f
         err = traversal_error_word();
     hipLaunchKernelGGL(k_requeue_init, dim3(1), dim3(256), 0, st, ctr, nullptr, 0u); // the chunk counters
     SNCH_DISPATCH_DIM(dim, {
-        if (sp)
-            launch_sil_persist_cfg<DIM, true>(t, q, n, flip, sil, dist, perm, sp, ctr, n_cu, err, st);
+        if (sp && seed)
+            launch_sil_persist_cfg<DIM, true, true>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, n_cu, err, st);
+        else if (sp)
+            launch_sil_persist_cfg<DIM, true, false>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, n_cu, err, st);
+        else if (seed)
+            launch_sil_persist_cfg<DIM, false, true>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, n_cu, err, st);
         else
-            launch_sil_persist_cfg<DIM, false>(t, q, n, flip, sil, dist, perm, sp, ctr, n_cu, err, st);
+            launch_sil_persist_cfg<DIM, false, false>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, n_cu, err, st);
     });
     return hipGetLastError();
 }

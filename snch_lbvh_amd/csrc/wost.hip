@@ -273,7 +273,8 @@ struct snch_wost {
     const snch_scene* neu = nullptr;
     uint32_t cap = 0;
     DeviceBuffer pos, nrm, value, steps, live0, live1, count, qpos, cp_idx, cp_dist, sil_idx, sil_dist, ray_o, ray_d,
-        ray_t, ray_hit, ray_idx, sph_r, sph_u, smp_idx, smp_pdf, star_r, ocodes, oidx, ocodes_alt, oidx_alt, ohist;
+        ray_t, ray_hit, ray_idx, sph_r, sph_u, smp_idx, smp_pdf, star_r, ocodes, oidx, ocodes_alt, oidx_alt, ohist,
+        sil_ctr;
     Walks w{};
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -301,6 +302,7 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
     TreeView vd = s->dir->view();
     TreeView vn = s->neu ? s->neu->view() : TreeView{};
     const int has_n = s->neu ? 1 : 0;
+    const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
     hipError_t e = hipSuccess;
     for (uint32_t step = 0; step < max_steps; ++step) {
         const int cur = step & 1;
@@ -319,9 +321,11 @@ hipError_t enqueue_steps(snch_wost* s, uint32_t max_steps, float eps, uint64_t s
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, s->w, cur, cap);
         if ((e = launch_closest_point(vd, 3, s->w.qpos, cap, s->w.cp_idx, s->w.cp_dist, 64, 16, nullptr, nullptr, st)) != hipSuccess)
             return e;
-        if (has_n &&
-            (e = launch_closest_silhouette(vn, 3, s->w.qpos, cap, 0, s->w.cp_dist, true, s->w.sil_idx, s->w.sil_dist, 64, 16,
-                                           nullptr, nullptr, st)) != hipSuccess)
+        // the silhouette within the star radius bound (the closest-point
+        // distance) on a persistent launch: capped searches vary widely in length
+        if (has_n && (e = launch_closest_silhouette_persist(vn, 3, s->w.qpos, cap, 0, s->w.cp_dist, s->w.sil_idx,
+                                                            s->w.sil_dist, nullptr, nullptr, s->sil_ctr.as<uint32_t>(),
+                                                            n_cu, st)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_star, dim3(nb), dim3(256), 0, st, s->w, cur, cap, s->dir->dV.as<float4>(),
                            s->dir->dF.as<int32_t>(), eps, seed, has_n);
@@ -376,7 +380,9 @@ int snch_wost_create(snch_context* ctx, const snch_scene* dirichlet, const snch_
                 {&s->smp_pdf, 4 * n}, {&s->star_r, 4 * n}, {&s->ocodes, 4 * n}, {&s->oidx, 4 * n},
                 {&s->ocodes_alt, 4 * n}, {&s->oidx_alt, 4 * n},
                 // radix-sort digit counts and look-back words (four 8-bit passes)
-                {&s->ohist, radix_sort_scratch_bytes(n, 4)}};
+                {&s->ohist, radix_sort_scratch_bytes(n, 4)},
+                // chunk counters of the persistent silhouette launch
+                {&s->sil_ctr, 64}};
     for (auto& b : bufs) {
         if ((e = b.b->ensure(b.bytes)) != hipSuccess) {
             delete s;
