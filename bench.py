@@ -588,7 +588,7 @@ def main():
                               "bytes_per_query_reference_order (it tests ~4x more boxes)",
                 "bytes_per_query_reference_order": ref_bytes,
                 "traffic_frac": (traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-                "kernel": "k_sil_persist<16, false, 3>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
+                "kernel": "k_sil_persist<16, false, 3, false>", "kernel_ms": kernel_ms, "step_ms_events": step_ms,
                 "query_order_ms": order_ms, "input_order_kernel_ms": input_order_ms, "shape_sweep_ms": sweep,
                 "visits_per_query": {"nodes": node, "cone_records": cone, "leaves": leaf, "edge_tests": edge}}
 
