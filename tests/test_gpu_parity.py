@@ -285,6 +285,12 @@ def test_seeded_silhouette_identical(S):
         np.testing.assert_array_equal(d1[within], d0[within])
         np.testing.assert_array_equal(e1[within], e0[within])
         assert np.all(d1[~within] > cap[~within])
+        # the default shape (persistent launch) equals one block per chunk bit
+        # for bit, beyond the cap too
+        e2, d2 = (t.cpu().numpy() for t in sc.closest_silhouette(q, seed_radius=cap, seed_is_cap=True,
+                                                                  block_size=128))
+        np.testing.assert_array_equal(e2, e1)
+        np.testing.assert_array_equal(d2, d1)
 
 
 def test_visiting_order_invariance(S, oracle):
