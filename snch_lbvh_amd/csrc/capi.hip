@@ -677,15 +677,15 @@ int snch_closest_silhouette_ex(const snch_scene* s, const float* q, uint32_t n, 
         SNCH_HIP(query_order(s->view(), (int)s->dim, q, n, s->qscratch, st, &perm),
                  "snch_closest_silhouette: ordering");
     if (persist) {
-        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
+        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 32), "snch_closest_silhouette: allocation");
         const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
         SNCH_HIP(launch_closest_silhouette_persist(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist,
                                                    perm, opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(), n_cu,
                                                    st),
                  "snch_closest_silhouette");
     } else if (requeue) {
-        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 16), "snch_closest_silhouette: allocation");
-        SNCH_HIP(s->rq_queue.ensure(sizeof(uint32_t) * (size_t)n), "snch_closest_silhouette: allocation");
+        SNCH_HIP(s->rq_ctr.ensure(sizeof(uint32_t) * 32), "snch_closest_silhouette: allocation");
+        SNCH_HIP(s->rq_queue.ensure(sizeof(uint32_t) * 8 * (size_t)n), "snch_closest_silhouette: allocation");
         const uint32_t n_cu = s->ctx && s->ctx->n_cu > 0 ? (uint32_t)s->ctx->n_cu : 256u;
         SNCH_HIP(launch_closest_silhouette_requeue(s->view(), (int)s->dim, q, n, flip, seed, out_sil, out_dist, perm,
                                                    opts ? opts->stats : nullptr, s->rq_ctr.as<uint32_t>(),

@@ -153,8 +153,8 @@ hipError_t launch_closest_silhouette(const TreeView& t, int dim, const float* q,
                                      const float* seed, bool capped, uint32_t* sil, float* dist, uint32_t block,
                                      uint32_t depth, const uint32_t* perm, uint64_t* stats, hipStream_t st,
                                      uint32_t* err = nullptr);
-// seeded search with unsettled queries re-queued for whole waves (ctr: 10
-// words, queue: n words; both rewritten at launch)
+// seeded search with unsettled queries re-queued for whole waves (ctr: 24
+// words, queue: 8 n words; both rewritten at launch)
 hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
                                              const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
                                              uint64_t* stats, uint32_t* ctr, uint32_t* queue, uint32_t n_cu,
@@ -162,7 +162,7 @@ hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const f
 // the search on a persistent launch taking 64-query chunks: plain (seed
 // null) or within the radius seed[qi] only (a hard cap); over the first
 // min(n, *t.n_dev) queries when the count is on the device (capped only).
-// ctr: 10 words, rewritten at launch (by a kernel: graph capture is fine)
+// ctr: 24 words, rewritten at launch (by a kernel: graph capture is fine)
 hipError_t launch_closest_silhouette_persist(const TreeView& t, int dim, const float* q, uint32_t n, int flip,
                                              const float* seed, uint32_t* sil, float* dist, const uint32_t* perm,
                                              uint64_t* stats, uint32_t* ctr, uint32_t n_cu, hipStream_t st,

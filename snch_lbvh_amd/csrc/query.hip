@@ -593,17 +593,22 @@ __global__ __launch_bounds__(BS, MODE == 0 && !STATS ? 6 : 5) void k_closest_sil
 // empty.  Every appended entry is taken: its appender itself goes on to drain
 // the queue after its last append.  A taken slot may still be being written
 // by its appender, which has reserved it and is running, so the wait for it
-// is short (and bounded).  ctr: [0, 8) next chunk of each XCD's range,
-// [8] queue head, [9] queue tail; queue: n entries, RQ_EMPTY at launch.
-// Results equal the unseeded search's.
+// is short (and bounded).  Each XCD appends to its own queue and drains it
+// first: its waves work through a contiguous stretch of the visiting order,
+// so a wave of its queued queries stays in one region of the scene, whose
+// nodes its L2 holds (one queue for all mixed the regions of all eight).
+// ctr: [0, 8) next chunk of each XCD's range, [8 + 2x] and [9 + 2x] head and
+// tail of XCD x's queue (RQ_CTR_WORDS words in all); queue: 8 queues of n
+// entries, RQ_EMPTY at launch.  Results equal the unseeded search's.
 constexpr uint32_t RQ_EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t RQ_SPIN = 1u << 20;
+constexpr uint32_t RQ_CTR_WORDS = 24;
 
 __global__ __launch_bounds__(256) void k_requeue_init(uint32_t* __restrict__ ctr, uint32_t* __restrict__ queue,
                                                       uint32_t n)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < 10)
+    if (i < RQ_CTR_WORDS)
         ctr[i] = 0u;
     for (uint32_t k = i; k < n; k += gridDim.x * 256)
         queue[k] = RQ_EMPTY;
@@ -648,6 +653,7 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
     const uint32_t nchunks = (n + 63) / 64;
     const uint32_t x0 = blockIdx.x % 8u; // blocks are dispatched round-robin over the 8 XCDs
     uint32_t xi = 0;                     // ranges exhausted so far, own range first (wave-uniform)
+    uint32_t di = 0;                     // queues found empty while draining, own XCD's first
     bool draining = false, ovf = false;
     Counts cnt;
     for (;;) {
@@ -681,18 +687,21 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
                     cap = F_INF;
             }
         } else {
-            // take up to 64 queued queries: [base, base + k) below the tail
+            // take up to 64 queued queries of queue dq (this XCD's first):
+            // [base, base + k) below its tail
+            const uint32_t dq = (x0 + di) & 7u;
+            uint32_t* head = ctr + 8 + 2 * dq;
             uint32_t base = 0, k = 0, empty = 0;
             if (lane == 0) {
-                uint32_t h = __hip_atomic_load(ctr + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 for (uint32_t tries = 0; tries < 64u; ++tries) {
-                    const uint32_t tl = __hip_atomic_load(ctr + 9, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t tl = __hip_atomic_load(head + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (h >= tl) {
                         empty = 1u;
                         break;
                     }
                     const uint32_t want = min(64u, tl - h);
-                    const uint32_t old = atomicCAS(ctr + 8, h, h + want); // a failure means another wave took some
+                    const uint32_t old = atomicCAS(head, h, h + want); // a failure means another wave took some
                     if (old == h) {
                         base = h;
                         k = want;
@@ -701,8 +710,11 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
                     h = old;
                 }
             }
-            if (uniform(empty))
-                break;
+            if (uniform(empty)) {
+                if (++di >= 8u)
+                    break;
+                continue;
+            }
             base = uniform(base);
             k = uniform(k);
             if (k == 0u)
@@ -710,7 +722,8 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
             if (lane < k) {
                 uint32_t v = RQ_EMPTY;
                 for (uint32_t s = 0; s < RQ_SPIN && v == RQ_EMPTY; ++s) {
-                    v = __hip_atomic_load(queue + base + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    v = __hip_atomic_load(queue + (size_t)dq * n + base + lane, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
                     if (v == RQ_EMPTY)
                         __builtin_amdgcn_s_sleep(2);
                 }
@@ -735,12 +748,12 @@ __global__ __launch_bounds__(64, STATS ? 5 : 6) void k_sil_requeue(TreeView t, c
         if (um) {
             uint32_t b = 0;
             if (lane == 0)
-                b = atomicAdd(ctr + 9, (uint32_t)__popcll(um));
+                b = atomicAdd(ctr + 9 + 2 * x0, (uint32_t)__popcll(um)); // this XCD's queue
             b = uniform(b);
             // rank among the unsettled lanes below this one
             const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(um >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)um, 0u));
             if (unsettled)
-                __hip_atomic_store(queue + b + r, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(queue + (size_t)x0 * n + b + r, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     if (ovf && err)
@@ -1479,7 +1492,7 @@ hipError_t launch_closest_silhouette_requeue(const TreeView& t, int dim, const f
     auto* sp = reinterpret_cast<unsigned long long*>(stats);
     if (!err)
         err = traversal_error_word();
-    hipLaunchKernelGGL(k_requeue_init, dim3(std::min(nblocks(n, 256), 1024u)), dim3(256), 0, st, ctr, queue, n);
+    hipLaunchKernelGGL(k_requeue_init, dim3(std::min(nblocks(n, 256), 1024u)), dim3(256), 0, st, ctr, queue, 8 * n);
     SNCH_DISPATCH_DIM(dim, {
         if (sp)
             launch_sil_requeue_cfg<DIM, true>(t, q, n, flip, seed, sil, dist, perm, sp, ctr, queue, n_cu, err, st);

@@ -381,8 +381,8 @@ int snch_wost_create(snch_context* ctx, const snch_scene* dirichlet, const snch_
                 {&s->ocodes_alt, 4 * n}, {&s->oidx_alt, 4 * n},
                 // radix-sort digit counts and look-back words (four 8-bit passes)
                 {&s->ohist, radix_sort_scratch_bytes(n, 4)},
-                // chunk counters of the persistent silhouette launch
-                {&s->sil_ctr, 64}};
+                // chunk counters of the persistent silhouette launch (RQ_CTR_WORDS)
+                {&s->sil_ctr, 128}};
     for (auto& b : bufs) {
         if ((e = b.b->ensure(b.bytes)) != hipSuccess) {
             delete s;
