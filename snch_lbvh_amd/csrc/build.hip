@@ -609,12 +609,18 @@ __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbo
 constexpr int KW = 512;
 constexpr int KWL = 256 + 2 * KW;
 struct CodeWindow {
-    const uint32_t* lds;
+    const uint32_t* lds;  // sorted codes of positions [lo, lo + KWL)
+    const uint32_t* lidx; // their primitive ids (ties between equal codes)
     int lo;
     __device__ __forceinline__ uint32_t at(const uint32_t* __restrict__ codes_all, uint32_t pos) const
     {
         const uint32_t u = (uint32_t)((int)pos - lo);
         return u < (uint32_t)KWL ? lds[u] : codes_all[pos];
+    }
+    __device__ __forceinline__ uint32_t id(const uint32_t* __restrict__ idx_all, uint32_t pos) const
+    {
+        const uint32_t u = (uint32_t)((int)pos - lo);
+        return u < (uint32_t)KWL ? lidx[u] : idx_all[pos];
     }
 };
 
@@ -627,7 +633,7 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes_all, co
     const uint32_t ca = w.at(codes_all, off + a), cb = w.at(codes_all, off + b);
     if (ca != cb)
         return __clz(ca ^ cb);
-    return 32 + __clz((idx_all[off + a] - off) ^ (idx_all[off + b] - off));
+    return 32 + __clz((w.id(idx_all, off + a) - off) ^ (w.id(idx_all, off + b) - off));
 }
 
 constexpr uint32_t RT = 1024;  // leaves of one refit subtree ("cut")
@@ -635,6 +641,19 @@ constexpr uint32_t RB = 512;   // threads of one k_refit workgroup (RT / RB leav
 constexpr uint32_t RM = 16384; // leaves of a mid set's subtree (the tops below it: k_refit_mid)
 constexpr uint32_t MID_CAP = 1024; // members of a mid set (at most 960, see k_refit_mid)
 
+
+// a list of at most two node ids in registers (an array indexed by its count
+// would live in private memory)
+struct List2 {
+    uint32_t a = INVALID, b = INVALID, n = 0;
+    __device__ __forceinline__ void push(bool yes, uint32_t v)
+    {
+        b = yes && n == 1u ? v : b;
+        a = yes && n == 0u ? v : a;
+        n += yes ? 1u : 0u;
+    }
+    __device__ __forceinline__ uint32_t at(uint32_t j) const { return j == 0u ? a : b; }
+};
 
 // bvh.cuh:109-229 (determine_range, find_split, construct_internal_nodes)
 // for one internal node: children, leaf range and parent links; the cuts it
@@ -646,8 +665,8 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
                                             const uint32_t* __restrict__ idx_all, const CodeWindow& w, uint32_t t,
                                             const MeshDesc* __restrict__ desc, uint32_t M,
                                             uint32_t* __restrict__ parent, uint2* __restrict__ child,
-                                            uint2* __restrict__ range, uint32_t& n_cut, uint32_t* cut,
-                                            uint32_t& top, uint32_t& n_mid, uint32_t* mid, uint32_t& midtop)
+                                            uint2* __restrict__ range, List2& cut, uint32_t& top, List2& mid,
+                                            uint32_t& midtop)
 {
     const uint32_t m = mesh_search<false>(desc, M, t);
     const MeshDesc D = desc[m];
@@ -706,27 +725,20 @@ __device__ __forceinline__ void karras_node(const uint32_t* __restrict__ codes_a
     parent[left] = self;
     parent[right] = self;
     const uint32_t size = last - first + 1, lsize = g - first + 1, rsize = last - g;
-    if (size > RM) {
+    // cuts: left, right (children of at most RT leaves below a node of more),
+    // self (a mesh root of at most RT); mid sets: self (a mesh root of at most
+    // RM), left, right (children of at most RM below a node of more)
+    const bool big = size > RM, medium = size > RT && !big;
+    if (big)
         top = self;
-        if (lsize <= RT)
-            cut[n_cut++] = left;
-        else if (lsize <= RM)
-            mid[n_mid++] = left;
-        if (rsize <= RT)
-            cut[n_cut++] = right;
-        else if (rsize <= RM)
-            mid[n_mid++] = right;
-    } else if (size > RT) {
+    if (medium)
         midtop = self;
-        if (i == 0)
-            mid[n_mid++] = self;
-        if (lsize <= RT)
-            cut[n_cut++] = left;
-        if (rsize <= RT)
-            cut[n_cut++] = right;
-    } else if (i == 0) {
-        cut[n_cut++] = self;
-    }
+    cut.push(size > RT && lsize <= RT, left);
+    cut.push(size > RT && rsize <= RT, right);
+    cut.push(size <= RT && i == 0, self);
+    mid.push(medium && i == 0, self);
+    mid.push(big && lsize > RT && lsize <= RM, left);
+    mid.push(big && rsize > RT && rsize <= RM, right);
 }
 
 // One thread per internal node of every mesh (internal-record slot t);
@@ -748,23 +760,29 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
     static_assert(BT == 256, "the code window spans BT nodes");
     // cuts, tops, mids and midtops of this block: counts, then list offsets
     __shared__ uint32_t s_n[4], s_base[4];
-    __shared__ uint32_t win[KWL];
+    __shared__ uint32_t win[KWL], widx[KWL];
     if (threadIdx.x < 4)
         s_n[threadIdx.x] = 0u;
     // internal-record slot t of mesh m is the node at sorted position t + m
     const uint32_t t0 = blockIdx.x * BT;
     const int lo = (int)(t0 + mesh_search<false>(desc, M, t0)) - KW;
+    // the codes and primitive ids around the block's nodes: sorted Morton
+    // codes repeat often (54 % of neighbouring pairs at 8.4M triangles), and
+    // every tie reads the ids
     for (int u = threadIdx.x; u < KWL; u += BT) {
         const int pos = lo + u;
-        win[u] = pos >= 0 && pos < (int)n_all ? codes_all[pos] : 0u; // positions outside [0, n_all) are never read
+        const bool in = pos >= 0 && pos < (int)n_all; // positions outside [0, n_all) are never read
+        win[u] = in ? codes_all[pos] : 0u;
+        widx[u] = in ? idx_all[pos] : 0u;
     }
     __syncthreads();
-    const CodeWindow w{win, lo};
+    const CodeWindow w{win, widx, lo};
     const uint32_t t = t0 + threadIdx.x;
-    uint32_t n_cut = 0, cut[2] = {INVALID, INVALID}, self = INVALID; // self: set for a top
-    uint32_t n_mid = 0, mid[2] = {INVALID, INVALID}, midtop = INVALID;
+    List2 cut, mid;
+    uint32_t self = INVALID, midtop = INVALID; // self: set for a top
     if (t < n_internal)
-        karras_node(codes_all, idx_all, w, t, desc, M, parent, child, range, n_cut, cut, self, n_mid, mid, midtop);
+        karras_node(codes_all, idx_all, w, t, desc, M, parent, child, range, cut, self, mid, midtop);
+    const uint32_t n_cut = cut.n, n_mid = mid.n;
     // one list append per block and list (the counters are shared by all blocks)
     const uint32_t cut_k = n_cut ? atomicAdd(&s_n[0], n_cut) : 0u;
     const uint32_t top_k = self != INVALID ? atomicAdd(&s_n[1], 1u) : 0u;
@@ -780,12 +798,13 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
         s_base[threadIdx.x] = n ? atomicAdd(ctr, n) : 0u;
     }
     __syncthreads();
-    for (uint32_t j = 0; j < n_mid; ++j) {
+#pragma unroll
+    for (uint32_t j = 0; j < 2u; ++j) {
         const uint32_t k = s_base[2] + mid_k + j;
-        if (k < mid_cap) {
-            mids[k] = mid[j];
+        if (j < n_mid && k < mid_cap) {
+            mids[k] = mid.at(j);
             mids[mid_cap + 1 + k] = 0u;
-            flags[mid[j]] = k;
+            flags[mid.at(j)] = k;
         }
     }
     if (midtop != INVALID) {
@@ -793,10 +812,11 @@ __global__ __launch_bounds__(BT) void k_karras(const uint32_t* __restrict__ code
         if (k < top_cap)
             midtops[k] = midtop;
     }
-    for (uint32_t j = 0; j < n_cut; ++j) {
+#pragma unroll
+    for (uint32_t j = 0; j < 2u; ++j) {
         const uint32_t k = s_base[0] + cut_k + j;
-        if (k < cut_cap)
-            cuts[k] = cut[j];
+        if (j < n_cut && k < cut_cap)
+            cuts[k] = cut.at(j);
     }
     if (self != INVALID) {
         const uint32_t k = s_base[1] + top_k;
