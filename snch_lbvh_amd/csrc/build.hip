@@ -281,8 +281,8 @@ struct LeafEdgeLds {
 // sums and maxima are formed exactly as in the per-triangle loop.
 __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                                 const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
-                                                uint32_t nF, int conservative, float4* __restrict__ leafbox,
-                                                float4* __restrict__ leafcone, float4* __restrict__ tedge,
+                                                uint32_t nF, int conservative, float4* __restrict__ leafrec,
+                                                float4* __restrict__ tedge,
                                                 int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh,
                                                 const MeshDesc* __restrict__ desc, float* __restrict__ part)
 {
@@ -430,10 +430,10 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
                 r.half_angle = smax(0.0f, dm::acosf_(cmin));
             }
         }
-        leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, b.lo.z, 0.0f);
-        leafbox[2 * i + 1] = make_float4(b.hi.x, b.hi.y, b.hi.z, 0.0f);
-        leafcone[2 * i] = make_float4(r.axis.x, r.axis.y, r.axis.z, r.half_angle);
-        leafcone[2 * i + 1] = make_float4(r.radius, 0.0f, 0.0f, 0.0f);
+        // the leaf record: box and cone in three float4 (no padding lanes)
+        leafrec[3 * i] = make_float4(b.lo.x, b.lo.y, b.lo.z, b.hi.x);
+        leafrec[3 * i + 1] = make_float4(b.hi.y, b.hi.z, r.axis.x, r.axis.y);
+        leafrec[3 * i + 2] = make_float4(r.axis.z, r.half_angle, r.radius, 0.0f);
     }
     // scene AABB of the primitive's mesh
     float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
@@ -446,8 +446,8 @@ __global__ __launch_bounds__(BT) void k_leafgen(const float4* __restrict__ V, co
 // (prev, v, next, -1).  Also writes the packed silhouette-vertex records.
 __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, const int32_t* __restrict__ S,
                                                   const int4* __restrict__ slots, const uint32_t* __restrict__ estart,
-                                                  uint32_t nS, float4* __restrict__ leafbox,
-                                                  float4* __restrict__ leafcone, float4* __restrict__ tedge,
+                                                  uint32_t nS, float4* __restrict__ leafrec,
+                                                  float4* __restrict__ tedge,
                                                   int* __restrict__ scene_bits, const uint32_t* __restrict__ pmesh,
                                                   float* __restrict__ part)
 {
@@ -526,10 +526,9 @@ __global__ __launch_bounds__(BT) void k_leafgen2d(const float4* __restrict__ V, 
                 ha = smax(0.0f, dm::acosf_(cmin));
             }
         }
-        leafbox[2 * i] = make_float4(b.lo.x, b.lo.y, 0.0f, 0.0f);
-        leafbox[2 * i + 1] = make_float4(b.hi.x, b.hi.y, 0.0f, 0.0f);
-        leafcone[2 * i] = make_float4(ax, ay, 0.0f, ha);
-        leafcone[2 * i + 1] = make_float4(rad, 0.0f, 0.0f, 0.0f);
+        leafrec[3 * i] = make_float4(b.lo.x, b.lo.y, 0.0f, b.hi.x);
+        leafrec[3 * i + 1] = make_float4(b.hi.y, 0.0f, ax, ay);
+        leafrec[3 * i + 2] = make_float4(0.0f, ha, rad, 0.0f);
     }
     float v[4] = {b.lo.x, b.lo.y, b.hi.x, b.hi.y};
     const int slot_of[4] = {0, 1, 3, 4};
@@ -551,7 +550,7 @@ __device__ __forceinline__ void mesh_of_one_root(const MeshDesc* __restrict__ de
         cuts[k] = desc[m].nb;
 }
 
-__global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
+__global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leafrec, const int* __restrict__ bits_all,
                                                  uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
                                                  const uint32_t* __restrict__ pmesh, const MeshDesc* __restrict__ desc,
                                                  uint32_t M, uint32_t* __restrict__ parent, uint32_t* __restrict__ cuts,
@@ -564,8 +563,8 @@ __global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leaf
     const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
     const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]);
     const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]);
-    const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
-    float px = (c.x + a.x) * 0.5f, py = (c.y + a.y) * 0.5f;
+    const float4 r0 = leafrec[3 * i], r1 = leafrec[3 * i + 1]; // lo.xy . hi.x | hi.y
+    float px = (r0.w + r0.x) * 0.5f, py = (r1.x + r0.y) * 0.5f;
     px -= lox;
     py -= loy;
     px /= (hix - lox);
@@ -575,7 +574,7 @@ __global__ __launch_bounds__(BT) void k_morton2d(const float4* __restrict__ leaf
 }
 
 // bvh.cuh:292-302 and morton_code.cuh:61-70
-__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbox, const int* __restrict__ bits_all,
+__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafrec, const int* __restrict__ bits_all,
                                                uint32_t n, uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
                                                const uint32_t* __restrict__ pmesh, const MeshDesc* __restrict__ desc,
                                                uint32_t M, uint32_t* __restrict__ parent, uint32_t* __restrict__ cuts,
@@ -588,10 +587,10 @@ __global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ leafbo
     const int* scene_bits = bits_all + 8 * mesh_of_prim(pmesh, i);
     const float lox = ord2f(scene_bits[0]), loy = ord2f(scene_bits[1]), loz = ord2f(scene_bits[2]);
     const float hix = ord2f(scene_bits[3]), hiy = ord2f(scene_bits[4]), hiz = ord2f(scene_bits[5]);
-    const float4 a = leafbox[2 * i], c = leafbox[2 * i + 1];
+    const float4 r0 = leafrec[3 * i], r1 = leafrec[3 * i + 1]; // lo.xyz hi.x | hi.yz
     box3 b;
-    b.lo = mk(a.x, a.y, a.z);
-    b.hi = mk(c.x, c.y, c.z);
+    b.lo = mk(r0.x, r0.y, r0.z);
+    b.hi = mk(r0.w, r1.x, r1.y);
     v3 p = box_centroid(b);
     p.x -= lox;
     p.y -= loy;
@@ -1030,8 +1029,8 @@ __device__ __forceinline__ void collect_mid_members(uint32_t k0, uint32_t stride
 template <int DIM>
 __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, const int32_t* __restrict__ F,
                                               const uint32_t* __restrict__ estart,
-                                              const uint32_t* __restrict__ sidx, const float4* __restrict__ leafbox,
-                                              const float4* __restrict__ leafcone, const uint32_t* __restrict__ parent,
+                                              const uint32_t* __restrict__ sidx, const float4* __restrict__ leafrec,
+                                              const uint32_t* __restrict__ parent,
                                               const uint2* __restrict__ child, const uint2* __restrict__ range,
                                               const MeshDesc* __restrict__ desc, uint32_t M, uint32_t N_all,
                                               int conservative, uint32_t leaf_size, float4* nbox, float4* ncone,
@@ -1114,8 +1113,9 @@ __global__ __launch_bounds__(RB) void k_refit(const float4* __restrict__ V, cons
                 continue;
             }
             const uint32_t node = D.nb + (D.n - 1) + (j - D.off);
-            const float4 b0 = leafbox[2 * orig], b1 = leafbox[2 * orig + 1];
-            const float4 c0 = leafcone[2 * orig], c1 = leafcone[2 * orig + 1];
+            const float4 r0 = leafrec[3 * orig], r1 = leafrec[3 * orig + 1], r2 = leafrec[3 * orig + 2];
+            const float4 b0 = make_float4(r0.x, r0.y, r0.z, 0.0f), b1 = make_float4(r0.w, r1.x, r1.y, 0.0f);
+            const float4 c0 = make_float4(r1.z, r1.w, r2.x, r2.y), c1 = make_float4(r2.z, 0.0f, 0.0f, 0.0f);
             if (node == root) { // a cut of one leaf; other leaves live in their parents' records
                 nbox[2 * node] = b0;
                 nbox[2 * node + 1] = b1;
@@ -1493,8 +1493,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
         if ((e = (x)) != hipSuccess) \
             return e;            \
     } while (0)
-    SNCH_TRY(w.leafbox.ensure(sizeof(float4) * 2 * N));
-    SNCH_TRY(w.leafcone.ensure(sizeof(float4) * 2 * N));
+    SNCH_TRY(w.leafbox.ensure(sizeof(float4) * 3 * N)); // the leaf records (box + cone)
     SNCH_TRY(w.scene_bits.ensure(sizeof(int) * 8 * M));
     SNCH_TRY(w.boxpart.ensure(sizeof(float) * 8 * blocks(N)));
     SNCH_TRY(w.codes.ensure(sizeof(uint32_t) * N));
@@ -1538,7 +1537,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     if (s->dim == 3) {
         hipLaunchKernelGGL(k_leafgen, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, conservative, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc,
+                           s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh, desc,
                            w.boxpart.as<float>());
         hipLaunchKernelGGL(k_box_final, dim3((blocks(N) + BF - 1) / BF), dim3(BF), 0, st, w.boxpart.as<float>(),
                            blocks(N), w.scene_bits.as<int>());
@@ -1548,7 +1547,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
     } else {
         hipLaunchKernelGGL(k_leafgen2d, dim3(blocks(N)), dim3(BT), 0, st, s->dV.as<float4>(), s->dF.as<int32_t>(),
                            s->dSlots.as<int4>(), s->dEstart.as<uint32_t>(), N, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh,
+                           s->tedge.as<float4>(), w.scene_bits.as<int>(), pmesh,
                            w.boxpart.as<float>());
         hipLaunchKernelGGL(k_box_final, dim3((blocks(N) + BF - 1) / BF), dim3(BF), 0, st, w.boxpart.as<float>(),
                            blocks(N), w.scene_bits.as<int>());
@@ -1588,7 +1587,7 @@ hipError_t build_tree(snch_scene* s, uint32_t flags, uint32_t leaf_size, hipStre
             std::max(1u, std::min(2u * ((N + RT - 1) / RT) + M, (uint32_t)std::max(per_cu, 1) * n_cu));
         hipLaunchKernelGGL(k_refit<DIM>, dim3(refit_blocks), dim3(RB), 0, st, s->dV.as<float4>(),
                            s->dF.as<int32_t>(), s->dEstart.as<uint32_t>(), sidx, w.leafbox.as<float4>(),
-                           w.leafcone.as<float4>(), w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
+                           w.parent.as<uint32_t>(), w.child.as<uint2>(), w.range.as<uint2>(),
                            desc, M, N, conservative, leaf_size, s->nbox.as<float4>(), s->ncone.as<float4>(),
                            s->tnode.as<float4>(), s->tcone.as<float4>(), s->tleaf.as<float4>(), cuts, cut_cap,
                            midtops, top_cap, mids, mid_cap, w.members.as<uint32_t>(), w.flags.as<uint32_t>());

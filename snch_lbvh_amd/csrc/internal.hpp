@@ -39,7 +39,7 @@ struct BatchConfig {
 };
 
 struct BuildScratch {
-    DeviceBuffer leafbox, leafcone, scene_bits, boxpart, codes, idx, codes_alt, idx_alt, hist, parent, child, range,
+    DeviceBuffer leafbox /* leaf records: box + cone, 3 float4 */, scene_bits, boxpart, codes, idx, codes_alt, idx_alt, hist, parent, child, range,
         flags, top, tops, status, mids, midtops, members;
 };
 
