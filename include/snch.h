@@ -63,7 +63,11 @@ typedef struct snch_build_opts {
 
 /* Traversal tuning (chosen by the autotuner; defaults are safe) */
 typedef struct snch_query_opts {
-    uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (64) */
+    uint32_t block_size; /* threads per block: 64, 128, 256 or 512; 0 = default (64).  The
+                            closest silhouette without a seed (or with a hard cap) runs, at
+                            block size 64 and stack depth 16, as a persistent launch whose
+                            wavefronts take 64-query chunks; other shapes launch one block
+                            per chunk.  Outputs are the same. */
     uint32_t coherent;   /* visiting order: 0 = Morton order of the query points when
                             n >= SNCH_ORDER_MIN_QUERIES, 1 = always Morton order,
                             2 = input order.  Results are written in input order and
@@ -82,8 +86,9 @@ typedef struct snch_query_opts {
                                   (the answer is exact when it lies within the radius; otherwise
                                   the returned distance exceeds the radius, possibly +inf) */
     uint32_t seed_retry;       /* closest silhouette with seed_radius (not a cap): 0 = queries not
-                                  settled within their radius are queued and searched again 64 at a
-                                  time by whole wavefronts of the same persistent launch (default;
+                                  settled within their radius are queued (one queue per XCD) and
+                                  searched again 64 at a time by whole wavefronts of the same
+                                  persistent launch (default;
                                   block size 64 and stack depth 16 only), 1 = each is searched again
                                   at once by its own lane.  Outputs are the same. */
 } snch_query_opts;
